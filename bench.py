@@ -1,0 +1,231 @@
+"""Benchmark of the MI355X PSM hot path (D3Q19 Noble-Torczynski collide+stream,
+sphere voxelization, per-particle force/torque sum) on one GPU.
+
+Default workload: the periodic dense suspension -- 256^3 D3Q19, tau 0.7,
+2000 non-overlapping spheres with radii U[4,8] (seeded rejection sampling),
+velocities N(0, 0.01) and angular velocities N(0, 1e-3) per component,
+8^3 sub-cell sampling, prescribed kinematics (x += v each step), FP64.
+One step = voxelize + fused collide/stream + force/torque sum, all
+resident in HBM.  `value` is whole-step MLUPS of that FP64 run; the FP32
+deviation and FP16-storage variants of the same run, and the
+512x512x768 channel in FP32 deviation storage, are reported beside it with
+their drag error against FP64.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "lbdemcoupling-public_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import lbdem_psm as L  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS of PSM step (FP64 vs FP32-dev) + %HBM roofline; drag rel-err vs FP64"
+BYTES_PER_POP = {"fp64": 8, "fp32_dev": 4, "fp16_store": 2}
+
+WORKLOADS = {
+    "dense256": dict(n=(256, 256, 256), periodic=(1, 1, 1), tau=0.7, count=2000,
+                     law=L.RADIUS_UNIFORM, a=4.0, b=8.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
+                     desc="periodic dense suspension 256^3, tau 0.7, 2000 spheres r~U[4,8]"),
+    "poly384": dict(n=(384, 384, 384), periodic=(1, 1, 1), tau=0.6, count=20000,
+                    law=L.RADIUS_LOGNORMAL, a=5.0, b=0.25, lo=3.0, hi=10.0, body=(0.0, 0.0, 0.0),
+                    desc="periodic polydisperse 384^3, tau 0.6, 20000 spheres lognormal(5,0.25) in [3,10]"),
+    "channel512": dict(n=(512, 512, 768), periodic=(0, 0, 1), tau=0.55, count=100000,
+                       law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 1e-6),
+                       desc="channel 512x512x768, walls x/y, body force 1e-6 along z, tau 0.55, "
+                            "100000 spheres r~U[3,6]"),
+    "cube512": dict(n=(512, 512, 512), periodic=(1, 1, 1), tau=0.55, count=66667,
+                    law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
+                    desc="periodic 512^3, tau 0.55, 66667 spheres r~U[3,6] (channel512 density)"),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_spheres(wl, seed=2024):
+    return L.generate_spheres(wl["n"], wl["count"], wl["law"], a=wl["a"], b=wl["b"], lo=wl["lo"],
+                              hi=wl["hi"] if wl["hi"] > 0 else 1e30, periodic=wl["periodic"],
+                              gap=0.0, wall_gap=1.0, sigma_v=0.01, sigma_w=1e-3, seed=seed)
+
+
+def run_gpu(wl, precision, steps, warmup, spheres, slices=8):
+    n = wl["n"]
+    nc = int(np.prod(n))
+    x, v, w, r, ids = spheres
+    lat = L.Lattice(*n, periodic=wl["periodic"], tau=wl["tau"], precision=precision,
+                    slices_per_dim=slices, body_accel=wl["body"])
+    try:
+        lat.set_spheres(x, v, w, r, ids)
+        lat.step(warmup, advance=True)
+        lat.synchronize()
+        lat.profile(True)
+        lat.profile_read(reset=True)
+        t0 = time.perf_counter()
+        lat.step(steps, advance=True)
+        lat.synchronize()
+        t1 = time.perf_counter()
+        prof = lat.profile_read(reset=True)
+        lat.profile(False)
+        F, T = lat.particle_forces()
+        st = lat.stats()
+        rc, umax, nonfin = lat.check_health()
+    finally:
+        lat.close()
+    wall = t1 - t0
+    cs_ms, cs_n = prof["collide_stream"]
+    cs_avg_s = cs_ms / max(cs_n, 1) * 1e-3
+    b_lu = 2 * 19 * BYTES_PER_POP[precision]
+    return {
+        "precision": precision,
+        "mlups": nc * steps / wall / 1e6,
+        "ms_per_step": wall / steps * 1e3,
+        "kernel_mlups": nc / cs_avg_s / 1e6,
+        "kernel_ms": cs_avg_s * 1e3,
+        "kernel_gbs": b_lu * nc / cs_avg_s / 1e9,
+        "voxelize_ms": prof["voxelize"][0] / max(prof["voxelize"][1], 1),
+        "force_ms": prof["force_sum"][0] / max(prof["force_sum"][1], 1),
+        "advance_ms": prof["advance"][0] / max(prof["advance"][1], 1),
+        "bytes_per_lu": b_lu,
+        "records": st["records"],
+        "images": st["images"],
+        "orphans": st["orphans"],
+        "max_u": umax,
+        "health_rc": rc,
+        "F": F,
+        "T": T,
+    }
+
+
+def rel(a, b):
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / den) if den > 0 else 0.0
+
+
+def cpu_baseline(wl, spheres, nsteps):
+    from oracle import oracle as O
+    n = wl["n"]
+    nc = int(np.prod(n))
+    x, v, w, r, ids = spheres
+    o = O.OracleLattice(*n, periodic=wl["periodic"], tau=wl["tau"], slices_per_dim=8,
+                        body_accel=wl["body"])
+    o.init_equilibrium()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    for s in range(nsteps):
+        if s:
+            x = L.advance_positions(x, v, n, wl["periodic"])
+        o.set_spheres(x, v, w, r, ids)
+        o.collide_stream(1, threads)
+        o.sum_force_torque(x)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": nc * nsteps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} full coupled FP64 steps (voxelize + collide/stream + force sum) of the "
+                      f"same {n[0]}x{n[1]}x{n[2]} workload by the CPU restatement of the reference "
+                      f"algorithm; collide/stream on {threads} OpenMP threads, voxelization and force "
+                      f"sum single-threaded as in the reference; {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="dense256", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-extra", action="store_true", help="skip the 512^3-class FP32 line")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU slab decomposition is a design, not a measured path)", file=sys.stderr)
+        sys.exit(2)
+
+    wl = WORKLOADS[args.workload]
+    n = wl["n"]
+    nc = int(np.prod(n))
+    log(f"workload {args.workload}: generating {wl['count']} spheres")
+    sph = make_spheres(wl)
+    res = {}
+    for prec in ("fp64", "fp32_dev", "fp16_store"):
+        log(f"{prec}: {args.warmup} warmup + {args.steps} timed steps")
+        res[prec] = run_gpu(wl, prec, args.steps, args.warmup, sph)
+        log(f"{prec}: {res[prec]['mlups']:.0f} MLUPS full step, kernel {res[prec]['kernel_gbs']:.0f} GB/s")
+    f64 = res["fp64"]
+
+    def summary(rr):
+        return {"mlups": round(rr["mlups"], 1), "ms_per_step": round(rr["ms_per_step"], 4),
+                "kernel_mlups": round(rr["kernel_mlups"], 1), "kernel_ms": round(rr["kernel_ms"], 4),
+                "kernel_gbs": round(rr["kernel_gbs"], 1),
+                "kernel_roofline_frac": round(rr["kernel_gbs"] / HBM_PEAK_GBS, 4),
+                "step_roofline_frac": round(rr["mlups"] * 1e6 * rr["bytes_per_lu"] / 1e9 / HBM_PEAK_GBS, 4),
+                "voxelize_ms": round(rr["voxelize_ms"], 4), "force_ms": round(rr["force_ms"], 4),
+                "drag_rel_err_vs_fp64": rel(rr["F"], f64["F"]),
+                "torque_rel_err_vs_fp64": rel(rr["T"], f64["T"])}
+
+    extra = {}
+    if not args.no_extra and args.workload == "dense256":
+        wl5 = WORKLOADS["channel512"]
+        log("channel512: generating 100000 spheres")
+        sph5 = make_spheres(wl5)
+        steps5 = max(10, args.steps // 4)
+        r32 = run_gpu(wl5, "fp32_dev", steps5, max(2, args.warmup // 2), sph5)
+        log(f"channel512 fp32_dev: {r32['mlups']:.0f} MLUPS, kernel {r32['kernel_gbs']:.0f} GB/s")
+        r64 = run_gpu(wl5, "fp64", steps5, max(2, args.warmup // 2), sph5)
+        log(f"channel512 fp64: {r64['mlups']:.0f} MLUPS")
+        e32 = {k: v for k, v in summary(r32).items()}
+        e32["drag_rel_err_vs_fp64"] = rel(r32["F"], r64["F"])
+        e32["torque_rel_err_vs_fp64"] = rel(r32["T"], r64["T"])
+        extra["channel512"] = {"workload": wl5["desc"], "steps": steps5,
+                               "fp32_dev": e32,
+                               "fp64": {"mlups": round(r64["mlups"], 1),
+                                        "kernel_gbs": round(r64["kernel_gbs"], 1)},
+                               "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3)}
+
+    log(f"cpu baseline: {args.cpu_steps} oracle steps")
+    cpu = cpu_baseline(wl, sph, args.cpu_steps)
+    log(f"cpu baseline: {cpu['value']:.1f} MLUPS")
+
+    out = {
+        "metric": METRIC,
+        "value": round(f64["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f64["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["desc"], "cells": nc, "spheres": wl["count"], "sub_cell_samples": 8,
+                   "kinematics": "prescribed x += v", "precision": "fp64",
+                   "step": "voxelize + fused collide/stream + force/torque sum"},
+        "roofline": {"bound": "hbm", "achieved": round(f64["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(f64["kernel_gbs"] / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_collide_stream<FP64>",
+                     "algorithmic_bytes_per_launch": 304 * nc},
+        "cpu_baseline": cpu,
+        "fp64": summary(f64),
+        "fp32_dev": summary(res["fp32_dev"]),
+        "fp16_store": summary(res["fp16_store"]),
+        "fp32_over_fp64_mlups": round(res["fp32_dev"]["mlups"] / f64["mlups"], 3),
+        "records": f64["records"], "images": f64["images"], "orphans": f64["orphans"],
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

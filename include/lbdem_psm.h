@@ -1,0 +1,161 @@
+/*
+ * lbdem_psm -- C ABI of the MI355X (gfx950, HIP) partially-saturated-cells
+ * (PSM) hot path of LBDEMcoupling: D3Q19 Noble-Torczynski collision fused
+ * with streaming, sub-cell sphere voxelization into per-cell coupling
+ * records, and per-particle force/torque summation.
+ *
+ * Conventions
+ *  - Every function returns an int: LBDEM_PSM_OK (0) or a negative error
+ *    code; lbdem_psm_last_error() gives the message.  No exception crosses
+ *    the ABI and nothing aborts.
+ *  - Caller-owned host arrays are read or written during the call only and
+ *    never retained (the borrow-for-the-call lifetime of the reference's
+ *    SetSingleSphere3D / SumForceTorque3D pointer arguments).
+ *  - Device memory and one HIP stream are owned by the handle.  One host
+ *    thread per handle at a time.
+ *  - Lattice frame: cell (i,j,k) is centred at (i,j,k); cell index
+ *    c = (i*ny + j)*nz + k (z fastest).  Host population arrays are
+ *    pop-major, f[q*Nc + c], in the deviation form f_q - t_q, 19 D3Q19
+ *    populations in the Palabos order (c_{q+9} = -c_q).  Per-cell vector
+ *    fields are component-major, v[d*Nc + c].  Particle arrays are
+ *    particle-major, x[3*p + d].
+ */
+#ifndef LBDEM_PSM_H
+#define LBDEM_PSM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBDEM_PSM_OK 0
+#define LBDEM_PSM_ERR_INVALID_ARG -1
+#define LBDEM_PSM_ERR_HIP -2
+#define LBDEM_PSM_ERR_OUT_OF_MEMORY -3
+#define LBDEM_PSM_ERR_CAPACITY -4
+#define LBDEM_PSM_ERR_NO_DEVICE -5
+#define LBDEM_PSM_ERR_STATE -6
+#define LBDEM_PSM_ERR_NUMERIC -7
+
+/* Population storage precision. */
+#define LBDEM_PSM_FP64 0        /* f - t in FP64, FP64 arithmetic */
+#define LBDEM_PSM_FP32_DEV 1    /* f - t in FP32, FP32 arithmetic */
+#define LBDEM_PSM_FP16_STORE 2  /* f - t stored in FP16, FP32 arithmetic */
+
+/* lbdem_psm_set_spheres flags */
+#define LBDEM_PSM_INIT_VEL 1    /* initVelFlag of setSpheresOnLattice */
+#define LBDEM_PSM_AUTO_IMAGES 2 /* library appends periodic images (no ghosts given) */
+
+/* lbdem_psm_step flags */
+#define LBDEM_PSM_STEP_ADVANCE 1 /* prescribed kinematics: x += v (periodic wrap) first */
+
+typedef struct lbdem_psm lbdem_psm;
+
+typedef struct lbdem_psm_desc {
+    int32_t nx, ny, nz;          /* lattice size (cells) */
+    int32_t periodic[3];         /* 1 periodic, 0 half-way bounce-back walls */
+    double tau;                  /* BGK relaxation time, omega = 1/tau, > 0.5 */
+    int32_t precision;           /* LBDEM_PSM_FP64 / _FP32_DEV / _FP16_STORE */
+    int32_t slices_per_dim;      /* sub-cell samples per axis (reference: 5) */
+    double body_accel[3];        /* Guo body-force acceleration, lattice units */
+    int32_t device;              /* HIP device ordinal */
+    int32_t reserved;
+    int64_t max_solid_cells;     /* coupling-record capacity, 0 = automatic */
+} lbdem_psm_desc;
+
+/* Defaults: 1x1x1, periodic, tau 1, FP64, 5 slices, no force, device 0. */
+int lbdem_psm_desc_init(lbdem_psm_desc* desc);
+
+/* Creates the lattice (populations at rest, rho = 1) -- the Palabos
+ * MultiBlockLattice3D with IBcompositeDynamics(new BGKdynamics(omega)) on
+ * every cell, sedimentingSphere.cpp:39,181; src/ibCompositeDynamics3D.h:37-38. */
+int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out);
+int lbdem_psm_destroy(lbdem_psm* h);
+/* Message of the last error on h, or of the last failed create when h is NULL. */
+const char* lbdem_psm_last_error(const lbdem_psm* h);
+const char* lbdem_psm_version(void);
+
+/* Equilibrium populations from rho (Nc) and u (3*Nc); NULL means rho = 1 /
+ * u = 0.  Replaces lattice.initialize() and the initial-field functionals. */
+int lbdem_psm_init_equilibrium(lbdem_psm* h, const double* rho, const double* u);
+/* Raw population access (checkpoint/restore, parity): 19*Nc doubles. */
+int lbdem_psm_set_populations(lbdem_psm* h, const double* f);
+int lbdem_psm_get_populations(lbdem_psm* h, double* f);
+
+/* setSpheresOnLattice + AttributeFunctional
+ * (src/ibDataExchangeWrappers3D.hh:33-146): one SetSingleSphere3D per
+ * particle in array order (src/ibProcessors3D.h:37-45,
+ * src/ibProcessors3D.hh:52-184), run as one ordered per-cell fold on the GPU.
+ * x3, v3, w3: positions / velocities / angular velocities in lattice units;
+ * r: radii; id: LIGGGHTS tags (>= 1); type may be NULL; exclude lists the
+ * excluded types (excludeType).  flags: LBDEM_PSM_INIT_VEL,
+ * LBDEM_PSM_AUTO_IMAGES.  The particles stay resident on the device. */
+int lbdem_psm_set_spheres(lbdem_psm* h, int64_t n, const double* x3, const double* v3,
+                          const double* w3, const double* r, const int64_t* id,
+                          const int32_t* type, const int32_t* exclude, int32_t n_exclude,
+                          int32_t flags);
+
+/* lattice.collideAndStream() with IBcompositeDynamics::collide
+ * (src/ibCompositeDynamics3D.hh:130-203) on every cell, nsteps times. */
+int lbdem_psm_collide_stream(lbdem_psm* h, int32_t nsteps);
+
+/* SumForceTorque3D (src/ibProcessors3D.hh:202-275) + the map(tag) fold of
+ * getForcesFromLattice (src/ibDataExchangeWrappers3D.hh:160-243): force and
+ * torque on each of the n particles of the last set_spheres, in lattice
+ * units, torque arms about x3[map(id)] with the minimum image.  x3 may be
+ * NULL (use the resident positions).  F3, T3: 3*n outputs. */
+int lbdem_psm_sum_force_torque(lbdem_psm* h, int64_t n, const double* x3, double* F3,
+                               double* T3);
+
+/* One full coupled step on resident data, nsteps times: [x += v] ->
+ * voxelize -> collide+stream -> force/torque sum (kept on the device; read
+ * with lbdem_psm_get_particle_forces).  The loop of
+ * sedimentingSphere.cpp:212-225 without the DEM substeps. */
+int lbdem_psm_step(lbdem_psm* h, int32_t nsteps, int32_t flags);
+int lbdem_psm_get_particle_forces(lbdem_psm* h, int64_t n, double* F3, double* T3);
+int lbdem_psm_get_particles(lbdem_psm* h, int64_t n, double* x3);
+
+/* Dense per-cell coupling record (IBdynamicsParticleData,
+ * src/ibDynamicsParticleData.h:26-37): sf (Nc), id (Nc), us (3*Nc),
+ * F (3*Nc).  Any pointer may be NULL. */
+int lbdem_psm_get_cell_records(lbdem_psm* h, double* sf, int64_t* id, double* us,
+                               double* F);
+/* Restores records (checkpoint, or a prepared ownership history). */
+int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id,
+                               const double* us);
+
+/* calcSolidFraction (src/ibProcessors3D.hh:116-158) evaluated on the GPU for
+ * n (offset, radius) queries: d3 = cell centre - particle centre.  Outputs
+ * sf, the sub-cell count and the branch (0 outside, 1 inside, 2 counted). */
+int lbdem_psm_subcell_counts(lbdem_psm* h, int64_t n, const double* d3, const double* r,
+                             double* sf, int32_t* count, int32_t* branch);
+
+/* FP64 on-device global sums: mass deviation sum(f - t) and momentum. */
+int lbdem_psm_conservation(lbdem_psm* h, double* mass, double* mom3);
+/* Health check: non-finite populations and max |u|; returns
+ * LBDEM_PSM_ERR_NUMERIC when a value is non-finite or max |u| >= 0.3. */
+int lbdem_psm_check_health(lbdem_psm* h, double* max_u, int64_t* n_nonfinite);
+
+/* Counters of the last voxelization: [0] coupling records, [1] candidate
+ * entries, [2] voxelized particle images, [3] orphan records. */
+int lbdem_psm_stats(lbdem_psm* h, int64_t* out4);
+int lbdem_psm_synchronize(lbdem_psm* h);
+
+/* Per-kernel timing with HIP events on the handle's stream.  Kernel
+ * classes: 0 collide_stream, 1 voxelize (bin + fold), 2 force_sum,
+ * 3 advance.  read: total ms and launch count per class; reset clears. */
+int lbdem_psm_profile(lbdem_psm* h, int32_t enable);
+int lbdem_psm_profile_read(lbdem_psm* h, double* ms4, int64_t* count4, int32_t reset);
+
+/* Host-side argument check of set_spheres, no device needed. */
+int lbdem_psm_check_spheres(const lbdem_psm_desc* desc, int64_t n, const double* x3,
+                            const double* v3, const double* w3, const double* r,
+                            const int64_t* id);
+/* Host-side check of a descriptor, no device needed. */
+int lbdem_psm_check_desc(const lbdem_psm_desc* desc);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,716 @@
+/*
+ * C ABI of the gfx950 PSM library: handle life cycle, device memory,
+ * host-side argument checks and the host halves of the exchange calls
+ * (setSpheresOnLattice / getForcesFromLattice,
+ * src/ibDataExchangeWrappers3D.hh:33-246).
+ */
+#include "psm_common.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unordered_map>
+
+namespace psm {
+size_t scan_temp_bytes(int64_t n);
+
+static std::string g_create_err;
+
+int set_err(lbdem_psm* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+int hip_check(lbdem_psm* h, hipError_t e, const char* what) {
+    std::string m = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return set_err(h, LBDEM_PSM_ERR_OUT_OF_MEMORY, m);
+    return set_err(h, LBDEM_PSM_ERR_HIP, m);
+}
+
+void prof_begin(lbdem_psm* h, int cls, hipEvent_t* a) {
+    (void)cls;
+    *a = nullptr;
+    if (!h->profiling) return;
+    if (h->ev_pool.empty()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        h->ev_pool.push_back(e);
+    }
+    *a = h->ev_pool.back();
+    h->ev_pool.pop_back();
+    (void)hipEventRecord(*a, h->stream);
+}
+
+void prof_end(lbdem_psm* h, int cls, hipEvent_t a) {
+    if (!h->profiling || !a) return;
+    hipEvent_t b;
+    if (h->ev_pool.empty()) {
+        if (hipEventCreate(&b) != hipSuccess) return;
+    } else {
+        b = h->ev_pool.back();
+        h->ev_pool.pop_back();
+    }
+    (void)hipEventRecord(b, h->stream);
+    h->spans.push_back({cls, a, b});
+}
+
+template <typename P>
+static hipError_t dalloc(P** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    return hipMalloc((void**)p, bytes);
+}
+
+static void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+static int alloc_records(lbdem_psm* h, Records& r) {
+    const int64_t cap = h->rec_cap;
+    PSM_HIP(h, dalloc(&r.sf, cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&r.id, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&r.us, 3 * cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&r.F, 3 * cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&r.cell, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&r.cand, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&r.bin_start, h->nbins * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&r.bin_count, h->nbins * sizeof(int32_t)));
+    PSM_HIP(h, hipMemset(r.bin_start, 0, h->nbins * sizeof(int32_t)));
+    PSM_HIP(h, hipMemset(r.bin_count, 0, h->nbins * sizeof(int32_t)));
+    return LBDEM_PSM_OK;
+}
+
+static void free_records(Records& r) {
+    dfree(r.sf); dfree(r.id); dfree(r.us); dfree(r.F); dfree(r.cell); dfree(r.cand);
+    dfree(r.bin_start); dfree(r.bin_count);
+    memset(&r, 0, sizeof(r));
+}
+
+/* Upper bound of the (image, bin) candidate entries of particles with these
+ * radii, valid for any positions. */
+static int64_t entry_bound(const lbdem_psm* h, int64_t n, const double* r, bool auto_images) {
+    int64_t total = 0;
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t ext = (int64_t)std::floor(2.0 * r[q]) + 6;
+        int64_t prod = 1;
+        for (int d = 0; d < 3; ++d) {
+            const int64_t span = (ext + BIN - 1) / BIN + 1;
+            int64_t m;
+            if (auto_images && h->g.periodic[d]) m = std::min<int64_t>(span + 2, 3 * (int64_t)h->g.nb[d]);
+            else m = std::min<int64_t>(span, h->g.nb[d]);
+            prod *= m;
+        }
+        total += prod;
+    }
+    return total + 1024;
+}
+
+static int ensure_particles(lbdem_psm* h, int64_t n) {
+    if (n <= h->p_cap && h->px) return LBDEM_PSM_OK;
+    const int64_t cap = std::max<int64_t>(n, std::max<int64_t>(2 * h->p_cap, 256));
+    dfree(h->px); dfree(h->pv); dfree(h->pw); dfree(h->pr); dfree(h->pid); dfree(h->pexcl);
+    dfree(h->powner); dfree(h->p_nent); dfree(h->p_ent_start); dfree(h->pft); dfree(h->psum_x);
+    PSM_HIP(h, dalloc(&h->px, 3 * cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&h->pv, 3 * cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&h->pw, 3 * cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&h->pr, cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&h->pid, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->pexcl, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->powner, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->p_nent, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->p_ent_start, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->pft, 6 * cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&h->psum_x, 3 * cap * sizeof(double)));
+    h->p_cap = cap;
+    const size_t need = std::max(scan_temp_bytes(h->nbins), scan_temp_bytes(cap));
+    if (need > h->scan_tmp_bytes) {
+        dfree(h->scan_tmp);
+        PSM_HIP(h, dalloc(&h->scan_tmp, need));
+        h->scan_tmp_bytes = need;
+    }
+    return LBDEM_PSM_OK;
+}
+
+static int ensure_entries(lbdem_psm* h, int64_t need) {
+    if (need > (int64_t)INT32_MAX)
+        return set_err(h, LBDEM_PSM_ERR_CAPACITY, "candidate entries exceed 2^31");
+    if (need <= h->ent_cap && h->cand_key) return LBDEM_PSM_OK;
+    dfree(h->cand_key); dfree(h->cand_bin); dfree(h->owner_ent); dfree(h->partial);
+    PSM_HIP(h, dalloc(&h->cand_key, need * sizeof(uint32_t)));
+    PSM_HIP(h, dalloc(&h->cand_bin, need * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->owner_ent, need * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->partial, 6 * need * sizeof(double)));
+    h->ent_cap = need;
+    return LBDEM_PSM_OK;
+}
+
+static int read_counters(lbdem_psm* h) {
+    PSM_HIP(h, hipMemcpyAsync(h->h_counters, h->counters, 8 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    return LBDEM_PSM_OK;
+}
+
+static int check_overflow(lbdem_psm* h) {
+    int rc = read_counters(h);
+    if (rc) return rc;
+    if (h->h_counters[1] & 1)
+        return set_err(h, LBDEM_PSM_ERR_CAPACITY,
+                       "coupling-record capacity exceeded (raise max_solid_cells); lattice state is invalid");
+    if (h->h_counters[1] & 2)
+        return set_err(h, LBDEM_PSM_ERR_CAPACITY, "candidate-entry capacity exceeded");
+    return LBDEM_PSM_OK;
+}
+
+static int fold_forces_to_host(lbdem_psm* h, int64_t n, double* F3, double* T3) {
+    std::vector<double> pft(6 * (size_t)n);
+    if (n > 0) {
+        PSM_HIP(h, hipMemcpyAsync(pft.data(), h->pft, 6 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        PSM_HIP(h, hipStreamSynchronize(h->stream));
+    }
+    /* getForcesFromLattice: f[map(tag[i])] += force[i]; the ghost slots end 0. */
+    for (int64_t q = 0; q < n; ++q)
+        for (int d = 0; d < 3; ++d) {
+            if (F3) F3[3 * q + d] = 0.0;
+            if (T3) T3[3 * q + d] = 0.0;
+        }
+    for (int64_t q = 0; q < n; ++q) {
+        const int64_t o = h->h_owner[q];
+        for (int d = 0; d < 3; ++d) {
+            if (F3) F3[3 * o + d] += pft[6 * q + d];
+            if (T3) T3[3 * o + d] += pft[6 * q + 3 + d];
+        }
+    }
+    return LBDEM_PSM_OK;
+}
+
+}  // namespace psm
+
+using namespace psm;
+
+extern "C" {
+
+const char* lbdem_psm_version(void) { return "lbdem_psm 0.1.0 (gfx950 HIP)"; }
+
+int lbdem_psm_desc_init(lbdem_psm_desc* d) {
+    if (!d) return LBDEM_PSM_ERR_INVALID_ARG;
+    memset(d, 0, sizeof(*d));
+    d->nx = d->ny = d->nz = 1;
+    d->periodic[0] = d->periodic[1] = d->periodic[2] = 1;
+    d->tau = 1.0;
+    d->precision = LBDEM_PSM_FP64;
+    d->slices_per_dim = 5;
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_check_desc(const lbdem_psm_desc* d) {
+    if (!d) return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "desc is NULL");
+    if (d->nx < 1 || d->ny < 1 || d->nz < 1)
+        return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "lattice sizes must be >= 1");
+    const int64_t nc = (int64_t)d->nx * d->ny * d->nz;
+    if (nc > (int64_t)INT32_MAX)
+        return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "more than 2^31-1 cells");
+    if (!(d->tau > 0.5) || !std::isfinite(d->tau))
+        return set_err(nullptr, LBDEM_PSM_ERR_NUMERIC, "tau must be finite and > 0.5 (omega < 2)");
+    if (d->precision < LBDEM_PSM_FP64 || d->precision > LBDEM_PSM_FP16_STORE)
+        return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "unknown precision");
+    if (d->slices_per_dim < 1 || d->slices_per_dim > 16)
+        return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "slices_per_dim must be in [1,16]");
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(d->body_accel[k]))
+            return set_err(nullptr, LBDEM_PSM_ERR_NUMERIC, "body_accel must be finite");
+    if (d->max_solid_cells < 0 || d->max_solid_cells > nc)
+        return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "max_solid_cells must be in [0, cells]");
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_check_spheres(const lbdem_psm_desc* d, int64_t n, const double* x3, const double* v3,
+                            const double* w3, const double* r, const int64_t* id) {
+    if (!d) return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "desc is NULL");
+    if (n < 0) return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "negative particle count");
+    if (n >= (int64_t)QMASK) return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "at most 2^27-2 particles");
+    if (n > 0 && (!x3 || !v3 || !w3 || !r || !id))
+        return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "NULL particle array");
+    const int nn[3] = {d->nx, d->ny, d->nz};
+    for (int64_t q = 0; q < n; ++q) {
+        if (!(r[q] > 0.0) || !std::isfinite(r[q]))
+            return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "particle radius must be finite and > 0");
+        if (id[q] < 1 || id[q] >= (int64_t)INT32_MAX)
+            return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "particle ids must be in [1, 2^31-1)");
+        for (int k = 0; k < 3; ++k) {
+            const double x = x3[3 * q + k];
+            if (!std::isfinite(x) || !std::isfinite(v3[3 * q + k]) || !std::isfinite(w3[3 * q + k]))
+                return set_err(nullptr, LBDEM_PSM_ERR_NUMERIC, "non-finite particle state");
+            if (x < -(double)nn[k] || x >= 2.0 * nn[k])
+                return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "particle outside [-n, 2n) of the lattice");
+            if (std::fabs(v3[3 * q + k]) >= 0.3)
+                return set_err(nullptr, LBDEM_PSM_ERR_NUMERIC, "particle velocity >= 0.3 lattice units");
+        }
+        if (r[q] + 2.0 >= 1e6) return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "radius too large");
+    }
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
+    if (!out) return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = lbdem_psm_check_desc(desc);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return set_err(nullptr, LBDEM_PSM_ERR_NO_DEVICE, "no HIP device available");
+    if (desc->device < 0 || desc->device >= ndev)
+        return set_err(nullptr, LBDEM_PSM_ERR_NO_DEVICE, "device ordinal out of range");
+    if (hipSetDevice(desc->device) != hipSuccess)
+        return set_err(nullptr, LBDEM_PSM_ERR_NO_DEVICE, "hipSetDevice failed");
+
+    lbdem_psm* h = new lbdem_psm();
+    h->desc = *desc;
+    const int nn[3] = {desc->nx, desc->ny, desc->nz};
+    for (int d = 0; d < 3; ++d) {
+        h->g.n[d] = nn[d];
+        h->g.periodic[d] = desc->periodic[d] ? 1 : 0;
+        h->g.nb[d] = (nn[d] + BIN - 1) / BIN;
+    }
+    h->g.nc = (int64_t)nn[0] * nn[1] * nn[2];
+    h->nbins = (int64_t)h->g.nb[0] * h->g.nb[1] * h->g.nb[2];
+    h->omega = 1.0 / desc->tau;
+    h->ooo = 1.0 / h->omega - 0.5;          /* src/ibCompositeDynamics3D.hh:178 */
+    h->sqrt3half = std::sqrt(3.1) / 2.0;    /* src/ibProcessors3D.hh:127 */
+    h->has_force = desc->body_accel[0] != 0.0 || desc->body_accel[1] != 0.0 || desc->body_accel[2] != 0.0;
+    h->precision = desc->precision;
+    const size_t S = h->precision == LBDEM_PSM_FP64 ? 8 : (h->precision == LBDEM_PSM_FP32_DEV ? 4 : 2);
+    h->pop_bytes = (size_t)Q * h->g.nc * S;
+    h->rec_cap = desc->max_solid_cells > 0
+                     ? desc->max_solid_cells
+                     : std::min<int64_t>(h->g.nc, std::max<int64_t>(h->g.nc / 2, 1 << 20));
+
+    hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { rc = hip_check(nullptr, e, "hipStreamCreate"); delete h; return rc; }
+    std::string msg;
+#define CREATE_CHECK(call)                                                   \
+    do {                                                                     \
+        hipError_t ee = (call);                                              \
+        if (ee != hipSuccess) {                                              \
+            rc = hip_check(nullptr, ee, #call);                              \
+            msg = g_create_err;                                              \
+            lbdem_psm_destroy(h);                                            \
+            g_create_err = msg;                                              \
+            return rc;                                                       \
+        }                                                                    \
+    } while (0)
+    CREATE_CHECK(dalloc(&h->f[0], h->pop_bytes));
+    CREATE_CHECK(dalloc(&h->f[1], h->pop_bytes));
+    CREATE_CHECK(dalloc(&h->slot, h->g.nc * sizeof(int32_t)));
+    CREATE_CHECK(hipMemset(h->slot, 0xff, h->g.nc * sizeof(int32_t)));
+    CREATE_CHECK(dalloc(&h->orphan, h->rec_cap * sizeof(int32_t)));
+    CREATE_CHECK(dalloc(&h->bin_cand_count, h->nbins * sizeof(int32_t)));
+    CREATE_CHECK(dalloc(&h->bin_cand_start, h->nbins * sizeof(int32_t)));
+    CREATE_CHECK(dalloc(&h->bin_fill, h->nbins * sizeof(int32_t)));
+    CREATE_CHECK(hipMemset(h->bin_cand_count, 0, h->nbins * sizeof(int32_t)));
+    CREATE_CHECK(hipMemset(h->bin_cand_start, 0, h->nbins * sizeof(int32_t)));
+    CREATE_CHECK(dalloc(&h->counters, 8 * sizeof(int64_t)));
+    CREATE_CHECK(hipMemset(h->counters, 0, 8 * sizeof(int64_t)));
+    CREATE_CHECK(hipHostMalloc((void**)&h->h_counters, 8 * sizeof(int64_t), hipHostMallocDefault));
+    CREATE_CHECK(dalloc(&h->red_tmp, (6 * 4096 + 8) * sizeof(double)));
+    for (int k = 0; k < 2; ++k) {
+        rc = alloc_records(h, h->rec[k]);
+        if (rc) { msg = h->err; lbdem_psm_destroy(h); g_create_err = msg; return rc; }
+    }
+    rc = ensure_particles(h, 256);
+    if (rc) { msg = h->err; lbdem_psm_destroy(h); g_create_err = msg; return rc; }
+    rc = ensure_entries(h, 1024);
+    if (rc) { msg = h->err; lbdem_psm_destroy(h); g_create_err = msg; return rc; }
+    rc = launch_init_equilibrium(h, nullptr, nullptr);
+    if (rc) { msg = h->err; lbdem_psm_destroy(h); g_create_err = msg; return rc; }
+    CREATE_CHECK(hipStreamSynchronize(h->stream));
+#undef CREATE_CHECK
+    *out = h;
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_destroy(lbdem_psm* h) {
+    if (!h) return LBDEM_PSM_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    dfree(h->f[0]); dfree(h->f[1]); dfree(h->slot);
+    for (int k = 0; k < 2; ++k) free_records(h->rec[k]);
+    dfree(h->px); dfree(h->pv); dfree(h->pw); dfree(h->pr); dfree(h->pid); dfree(h->pexcl);
+    dfree(h->powner); dfree(h->id_map); dfree(h->bin_cand_count); dfree(h->bin_cand_start);
+    dfree(h->bin_fill); dfree(h->cand_key); dfree(h->cand_bin); dfree(h->p_nent);
+    dfree(h->p_ent_start); dfree(h->owner_ent); dfree(h->partial); dfree(h->pft); dfree(h->psum_x);
+    dfree(h->scan_tmp); dfree(h->counters); dfree(h->red_tmp); dfree(h->orphan);
+    dfree(h->dv_cell); dfree(h->dv_us);
+    if (h->h_counters) (void)hipHostFree(h->h_counters);
+    for (auto& s : h->spans) { (void)hipEventDestroy(s.a); (void)hipEventDestroy(s.b); }
+    for (auto e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return LBDEM_PSM_OK;
+}
+
+const char* lbdem_psm_last_error(const lbdem_psm* h) {
+    return h ? h->err.c_str() : g_create_err.c_str();
+}
+
+int lbdem_psm_init_equilibrium(lbdem_psm* h, const double* rho, const double* u) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    const int64_t nc = h->g.nc;
+    double *d_rho = nullptr, *d_u = nullptr;
+    if (rho) {
+        PSM_HIP(h, dalloc(&d_rho, nc * sizeof(double)));
+        PSM_HIP(h, hipMemcpy(d_rho, rho, nc * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (u) {
+        PSM_HIP(h, dalloc(&d_u, 3 * nc * sizeof(double)));
+        PSM_HIP(h, hipMemcpy(d_u, u, 3 * nc * sizeof(double), hipMemcpyHostToDevice));
+    }
+    int rc = launch_init_equilibrium(h, d_rho, d_u);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    dfree(d_rho);
+    dfree(d_u);
+    if (rc) return rc;
+    PSM_HIP(h, e);
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_set_populations(lbdem_psm* h, const double* f) {
+    if (!h || !f) return LBDEM_PSM_ERR_INVALID_ARG;
+    const size_t bytes = (size_t)Q * h->g.nc * sizeof(double);
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->precision == LBDEM_PSM_FP64) {
+        PSM_HIP(h, hipMemcpy(h->f[h->cur_f], f, bytes, hipMemcpyHostToDevice));
+        return LBDEM_PSM_OK;
+    }
+    double* tmp = nullptr;
+    PSM_HIP(h, dalloc(&tmp, bytes));
+    PSM_HIP(h, hipMemcpy(tmp, f, bytes, hipMemcpyHostToDevice));
+    int rc = launch_pop_convert(h, tmp, true);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    dfree(tmp);
+    if (rc) return rc;
+    PSM_HIP(h, e);
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_get_populations(lbdem_psm* h, double* f) {
+    if (!h || !f) return LBDEM_PSM_ERR_INVALID_ARG;
+    const size_t bytes = (size_t)Q * h->g.nc * sizeof(double);
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->precision == LBDEM_PSM_FP64) {
+        PSM_HIP(h, hipMemcpy(f, h->f[h->cur_f], bytes, hipMemcpyDeviceToHost));
+        return LBDEM_PSM_OK;
+    }
+    double* tmp = nullptr;
+    PSM_HIP(h, dalloc(&tmp, bytes));
+    int rc = launch_pop_convert(h, tmp, false);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (!rc && e == hipSuccess) e = hipMemcpy(f, tmp, bytes, hipMemcpyDeviceToHost);
+    dfree(tmp);
+    if (rc) return rc;
+    PSM_HIP(h, e);
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_set_spheres(lbdem_psm* h, int64_t n, const double* x3, const double* v3,
+                          const double* w3, const double* r, const int64_t* id, const int32_t* type,
+                          const int32_t* exclude, int32_t n_exclude, int32_t flags) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    int rc = lbdem_psm_check_spheres(&h->desc, n, x3, v3, w3, r, id);
+    if (rc) return set_err(h, rc, g_create_err);
+    if (n_exclude < 0 || (n_exclude > 0 && !exclude))
+        return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "bad exclude list");
+    /* host copies below go through the null stream: drain the handle's stream first */
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    rc = ensure_particles(h, n);
+    if (rc) return rc;
+    const bool auto_images = (flags & LBDEM_PSM_AUTO_IMAGES) != 0;
+    rc = ensure_entries(h, entry_bound(h, n, r, auto_images));
+    if (rc) return rc;
+
+    /* map(tag): the first index holding each id (owned atoms precede ghosts) */
+    int64_t max_id = 0;
+    for (int64_t q = 0; q < n; ++q) max_id = std::max<int64_t>(max_id, id[q]);
+    std::vector<int32_t> idmap(max_id + 1, -1);
+    h->h_owner.assign(n, 0);
+    h->h_ids.assign(id, id + n);
+    std::vector<int32_t> pid(n), excl(n, 0);
+    for (int64_t q = 0; q < n; ++q) {
+        if (idmap[id[q]] < 0) idmap[id[q]] = (int32_t)q;
+        h->h_owner[q] = idmap[id[q]];
+        pid[q] = (int32_t)id[q];
+        if (type)
+            for (int t = 0; t < n_exclude; ++t)
+                if (type[q] == exclude[t]) { excl[q] = 1; break; }
+    }
+    if (max_id + 1 > h->id_map_size) {
+        dfree(h->id_map);
+        PSM_HIP(h, dalloc(&h->id_map, (max_id + 1) * sizeof(int32_t)));
+        h->id_map_size = max_id + 1;
+    }
+    if (n > 0) {
+        PSM_HIP(h, hipMemcpy(h->px, x3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(h->pv, v3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(h->pw, w3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(h->pr, r, n * sizeof(double), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(h->pid, pid.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(h->pexcl, excl.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(h->powner, h->h_owner.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    PSM_HIP(h, hipMemset(h->id_map, 0xff, h->id_map_size * sizeof(int32_t)));
+    PSM_HIP(h, hipMemcpy(h->id_map, idmap.data(), (max_id + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    h->np = n;
+    h->flags = flags;
+    h->have_particles = true;
+    if ((flags & LBDEM_PSM_INIT_VEL) && !h->dv_cell) {
+        PSM_HIP(h, dalloc(&h->dv_cell, h->rec_cap * sizeof(int32_t)));
+        PSM_HIP(h, dalloc(&h->dv_us, 3 * h->rec_cap * sizeof(double)));
+    }
+    PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
+    rc = run_voxelize(h);
+    if (rc) return rc;
+    rc = check_overflow(h);
+    if (rc) return rc;
+    h->voxelized = true;
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_collide_stream(lbdem_psm* h, int32_t nsteps) {
+    if (!h || nsteps < 0) return LBDEM_PSM_ERR_INVALID_ARG;
+    for (int s = 0; s < nsteps; ++s) {
+        int rc = launch_collide_stream(h);
+        if (rc) return rc;
+    }
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_sum_force_torque(lbdem_psm* h, int64_t n, const double* x3, double* F3, double* T3) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    if (!h->have_particles) return set_err(h, LBDEM_PSM_ERR_STATE, "sum_force_torque before set_spheres");
+    if (n != h->np) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "particle count differs from set_spheres");
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    if (n > 0) {
+        if (x3) PSM_HIP(h, hipMemcpy(h->psum_x, x3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+        else PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->px, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    }
+    int rc = run_force_sum(h);
+    if (rc) return rc;
+    return fold_forces_to_host(h, n, F3, T3);
+}
+
+int lbdem_psm_step(lbdem_psm* h, int32_t nsteps, int32_t flags) {
+    if (!h || nsteps < 0) return LBDEM_PSM_ERR_INVALID_ARG;
+    if (!h->have_particles) return set_err(h, LBDEM_PSM_ERR_STATE, "step before set_spheres");
+    PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
+    for (int s = 0; s < nsteps; ++s) {
+        int rc;
+        if (flags & LBDEM_PSM_STEP_ADVANCE) {
+            rc = launch_advance(h);
+            if (rc) return rc;
+        }
+        rc = run_voxelize(h);
+        if (rc) return rc;
+        rc = launch_collide_stream(h);
+        if (rc) return rc;
+        if (h->np > 0)
+            PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->px, 3 * h->np * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        rc = run_force_sum(h);
+        if (rc) return rc;
+    }
+    return check_overflow(h);
+}
+
+int lbdem_psm_get_particle_forces(lbdem_psm* h, int64_t n, double* F3, double* T3) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    if (n != h->np) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "particle count differs from set_spheres");
+    return fold_forces_to_host(h, n, F3, T3);
+}
+
+int lbdem_psm_get_particles(lbdem_psm* h, int64_t n, double* x3) {
+    if (!h || !x3) return LBDEM_PSM_ERR_INVALID_ARG;
+    if (n != h->np) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "particle count differs from set_spheres");
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    if (n > 0) PSM_HIP(h, hipMemcpy(x3, h->px, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_get_cell_records(lbdem_psm* h, double* sf, int64_t* id, double* us, double* F) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    const int64_t nc = h->g.nc, cap = h->rec_cap;
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    std::vector<int32_t> slot(nc);
+    PSM_HIP(h, hipMemcpy(slot.data(), h->slot, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
+    int64_t m = 0;
+    for (int64_t c = 0; c < nc; ++c) m = std::max<int64_t>(m, (int64_t)slot[c] + 1);
+    const Records& r = h->rec[h->cur_rec];
+    std::vector<double> rsf(m), rus(3 * m), rF(3 * m);
+    std::vector<int32_t> rid(m);
+    if (m > 0) {
+        PSM_HIP(h, hipMemcpy(rsf.data(), r.sf, m * sizeof(double), hipMemcpyDeviceToHost));
+        PSM_HIP(h, hipMemcpy(rid.data(), r.id, m * sizeof(int32_t), hipMemcpyDeviceToHost));
+        for (int d = 0; d < 3; ++d) {
+            PSM_HIP(h, hipMemcpy(&rus[d * m], r.us + d * cap, m * sizeof(double), hipMemcpyDeviceToHost));
+            PSM_HIP(h, hipMemcpy(&rF[d * m], r.F + d * cap, m * sizeof(double), hipMemcpyDeviceToHost));
+        }
+    }
+    for (int64_t c = 0; c < nc; ++c) {
+        const int32_t s = slot[c];
+        if (sf) sf[c] = s >= 0 ? rsf[s] : 0.0;
+        if (id) id[c] = s >= 0 ? rid[s] : 0;
+        for (int d = 0; d < 3; ++d) {
+            if (us) us[d * nc + c] = s >= 0 ? rus[d * m + s] : 0.0;
+            if (F) F[d * nc + c] = s >= 0 ? rF[d * m + s] : 0.0;
+        }
+    }
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id, const double* us) {
+    if (!h || !sf) return LBDEM_PSM_ERR_INVALID_ARG;
+    const int64_t nc = h->g.nc, cap = h->rec_cap, nbins = h->nbins;
+    const int n0 = h->g.n[0], n1 = h->g.n[1], n2 = h->g.n[2];
+    std::vector<int32_t> slot(nc, -1), bstart(nbins, 0), bcount(nbins, 0);
+    std::vector<double> rsf, rus0, rus1, rus2;
+    std::vector<int32_t> rid, rcell;
+    int64_t m = 0;
+    for (int64_t b = 0; b < nbins; ++b) {
+        const int bz = (int)(b % h->g.nb[2]), by = (int)((b / h->g.nb[2]) % h->g.nb[1]);
+        const int bx = (int)(b / ((int64_t)h->g.nb[2] * h->g.nb[1]));
+        bstart[b] = (int32_t)m;
+        for (int l = 0; l < BIN_CELLS; ++l) {
+            const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
+            if (X >= n0 || Y >= n1 || Z >= n2) continue;
+            const int64_t c = ((int64_t)X * n1 + Y) * n2 + Z;
+            if (!(sf[c] > SOLFRAC_MIN)) continue;
+            if (m >= cap) return set_err(h, LBDEM_PSM_ERR_CAPACITY, "records exceed max_solid_cells");
+            const int64_t pid = id ? id[c] : 0;
+            if (pid < 0 || pid >= (int64_t)INT32_MAX) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "bad record id");
+            slot[c] = (int32_t)m;
+            rsf.push_back(sf[c]);
+            rid.push_back((int32_t)pid);
+            rus0.push_back(us ? us[c] : 0.0);
+            rus1.push_back(us ? us[nc + c] : 0.0);
+            rus2.push_back(us ? us[2 * nc + c] : 0.0);
+            rcell.push_back((int32_t)c);
+            ++m;
+        }
+        bcount[b] = (int32_t)(m - bstart[b]);
+    }
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    const Records& r = h->rec[h->cur_rec];
+    std::vector<int32_t> cand(m, -1), orph(m);
+    for (int64_t k = 0; k < m; ++k) orph[k] = (int32_t)k;
+    if (m > 0) {
+        PSM_HIP(h, hipMemcpy(r.sf, rsf.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(r.id, rid.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(r.us, rus0.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(r.us + cap, rus1.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(r.us + 2 * cap, rus2.data(), m * sizeof(double), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(r.cell, rcell.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(r.cand, cand.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpy(h->orphan, orph.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
+    PSM_HIP(h, hipMemcpy(r.bin_start, bstart.data(), nbins * sizeof(int32_t), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(r.bin_count, bcount.data(), nbins * sizeof(int32_t), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(h->slot, slot.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
+    /* Until the next voxelization every record is summed on the orphan
+     * path: no candidate lists describe these records. */
+    PSM_HIP(h, hipMemset(h->bin_cand_count, 0, nbins * sizeof(int32_t)));
+    if (h->p_cap > 0) PSM_HIP(h, hipMemset(h->p_nent, 0, h->p_cap * sizeof(int32_t)));
+    int64_t hc[8] = {m, 0, m, 0, 0, 0, 0, 0};
+    PSM_HIP(h, hipMemcpy(h->counters, hc, sizeof(hc), hipMemcpyHostToDevice));
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_subcell_counts(lbdem_psm* h, int64_t n, const double* d3, const double* r, double* sf,
+                             int32_t* count, int32_t* branch) {
+    if (!h || n < 0 || (n > 0 && (!d3 || !r || !sf || !count || !branch))) return LBDEM_PSM_ERR_INVALID_ARG;
+    if (n == 0) return LBDEM_PSM_OK;
+    double *dd = nullptr, *dr = nullptr, *dsf = nullptr;
+    int32_t *dc = nullptr, *db = nullptr;
+    PSM_HIP(h, dalloc(&dd, 3 * n * sizeof(double)));
+    PSM_HIP(h, dalloc(&dr, n * sizeof(double)));
+    PSM_HIP(h, dalloc(&dsf, n * sizeof(double)));
+    PSM_HIP(h, dalloc(&dc, n * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&db, n * sizeof(int32_t)));
+    PSM_HIP(h, hipMemcpy(dd, d3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(dr, r, n * sizeof(double), hipMemcpyHostToDevice));
+    int rc = launch_subcell(h, n, dd, dr, dsf, dc, db);
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (!rc && e == hipSuccess) e = hipMemcpy(sf, dsf, n * sizeof(double), hipMemcpyDeviceToHost);
+    if (!rc && e == hipSuccess) e = hipMemcpy(count, dc, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (!rc && e == hipSuccess) e = hipMemcpy(branch, db, n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    dfree(dd); dfree(dr); dfree(dsf); dfree(dc); dfree(db);
+    if (rc) return rc;
+    PSM_HIP(h, e);
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_conservation(lbdem_psm* h, double* mass, double* mom3) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    return launch_conservation(h, mass, mom3, nullptr, nullptr);
+}
+
+int lbdem_psm_check_health(lbdem_psm* h, double* max_u, int64_t* n_nonfinite) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    double mu = 0.0;
+    int64_t nf = 0;
+    int rc = launch_conservation(h, nullptr, nullptr, &mu, &nf);
+    if (rc) return rc;
+    if (max_u) *max_u = mu;
+    if (n_nonfinite) *n_nonfinite = nf;
+    if (nf > 0) return set_err(h, LBDEM_PSM_ERR_NUMERIC, "non-finite populations");
+    if (!(mu < 0.3)) return set_err(h, LBDEM_PSM_ERR_NUMERIC, "max |u| >= 0.3 lattice units");
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_stats(lbdem_psm* h, int64_t* out4) {
+    if (!h || !out4) return LBDEM_PSM_ERR_INVALID_ARG;
+    int rc = read_counters(h);
+    if (rc) return rc;
+    int32_t last_start = 0, last_count = 0;
+    if (h->nbins > 0) {
+        PSM_HIP(h, hipMemcpy(&last_start, h->bin_cand_start + h->nbins - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+        PSM_HIP(h, hipMemcpy(&last_count, h->bin_cand_count + h->nbins - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    out4[0] = h->h_counters[0];
+    out4[1] = (int64_t)last_start + last_count;
+    out4[2] = h->h_counters[4];
+    out4[3] = h->h_counters[2];
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_synchronize(lbdem_psm* h) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_profile(lbdem_psm* h, int32_t enable) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    h->profiling = enable != 0;
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_profile_read(lbdem_psm* h, double* ms4, int64_t* count4, int32_t reset) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    for (auto& s : h->spans) {
+        float ms = 0.0f;
+        PSM_HIP(h, hipEventElapsedTime(&ms, s.a, s.b));
+        h->prof_ms[s.cls] += ms;
+        h->prof_n[s.cls] += 1;
+        h->ev_pool.push_back(s.a);
+        h->ev_pool.push_back(s.b);
+    }
+    h->spans.clear();
+    for (int k = 0; k < 4; ++k) {
+        if (ms4) ms4[k] = h->prof_ms[k];
+        if (count4) count4[k] = h->prof_n[k];
+    }
+    if (reset)
+        for (int k = 0; k < 4; ++k) { h->prof_ms[k] = 0.0; h->prof_n[k] = 0; }
+    return LBDEM_PSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,320 @@
+/*
+ * Shared definitions of the gfx950 PSM library: D3Q19 tables, the per-cell
+ * PSM arithmetic as device functions, the device-side views the kernels
+ * take, and the handle.
+ *
+ * Floating-point contract: the FP64 path is compiled with
+ * -ffp-contract=off and evaluates every expression in the written order,
+ * so that it reproduces the FP64 CPU restatement bit for bit.
+ */
+#ifndef LBDEM_PSM_COMMON_H
+#define LBDEM_PSM_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/lbdem_psm.h"
+
+namespace psm {
+
+constexpr int Q = 19;
+constexpr int BIN = 8;              /* voxelization bin edge (cells) */
+constexpr int BIN_CELLS = BIN * BIN * BIN;
+constexpr int QBITS = 27;           /* candidate key: rank << 27 | particle */
+constexpr uint32_t QMASK = (1u << QBITS) - 1u;
+constexpr double SOLFRAC_MIN = 0.001;   /* src/ibDef.h:26 */
+constexpr double SOLFRAC_MAX = 0.999;   /* src/ibDef.h:27 */
+
+/* D3Q19, Palabos order: c_{i+9} = -c_i. */
+__host__ __device__ constexpr int cx(int i) {
+    constexpr int v[Q] = {0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+    return v[i];
+}
+__host__ __device__ constexpr int cy(int i) {
+    constexpr int v[Q] = {0, 0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+    return v[i];
+}
+__host__ __device__ constexpr int cz(int i) {
+    constexpr int v[Q] = {0, 0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+    return v[i];
+}
+__host__ __device__ constexpr int cc(int i, int d) { return d == 0 ? cx(i) : (d == 1 ? cy(i) : cz(i)); }
+__host__ __device__ constexpr int opp(int i) { return i == 0 ? 0 : (i <= 9 ? i + 9 : i - 9); }
+
+template <typename T>
+__host__ __device__ inline T wgt(int i) {
+    return i == 0 ? T(1.0 / 3.0) : ((i <= 3 || (i >= 10 && i <= 12)) ? T(1.0 / 18.0) : T(1.0 / 36.0));
+}
+
+/* ---- per-cell arithmetic (same operation order as the CPU restatement) ---- */
+
+template <typename T>
+__device__ __forceinline__ void moments(const T* f, T& rhoBar, T* j) {
+    rhoBar = T(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) rhoBar = rhoBar + f[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        T s = T(0);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (cc(i, d) == 1) s = s + f[i];
+            else if (cc(i, d) == -1) s = s - f[i];
+        }
+        j[d] = s;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T cdot(int i, const T* v) {
+    T s = T(0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (cc(i, d) == 1) s = s + v[d];
+        else if (cc(i, d) == -1) s = s - v[d];
+    }
+    return s;
+}
+
+template <typename T>
+__device__ __forceinline__ T norm_sqr(const T* v) { return (v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]; }
+
+template <typename T>
+__device__ __forceinline__ T equilibrium(int i, T rhoBar, T p, const T* j, T jSqr) {
+    const T cj = cdot<T>(i, j);
+    const T a = T(3) * cj;
+    const T b = a * cj;
+    const T q = b - jSqr;
+    return wgt<T>(i) * ((rhoBar + a) + p * q);
+}
+
+/* Base dynamics: BGK, with Guo forcing when FORCE. */
+template <typename T, bool FORCE>
+__device__ __forceinline__ void bgk(T* f, T omega, const T* acc) {
+    T rhoBar, j[3];
+    moments<T>(f, rhoBar, j);
+    const T rho = rhoBar + T(1);
+    const T invRho = T(1) / rho;
+    T jEq[3] = {j[0], j[1], j[2]};
+    T u[3] = {T(0), T(0), T(0)};
+    if (FORCE) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            u[d] = j[d] * invRho + T(0.5) * acc[d];
+            jEq[d] = rho * u[d];
+        }
+    }
+    const T jSqr = norm_sqr<T>(jEq);
+    const T p = T(1.5) * invRho;
+    const T omm = T(1) - omega;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = f[i] * omm + omega * equilibrium<T>(i, rhoBar, p, jEq, jSqr);
+    if (FORCE) {
+        const T pref = (T(1) - T(0.5) * omega) * rho;
+        const T ua = (u[0] * acc[0] + u[1] * acc[1]) + u[2] * acc[2];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const T ca = cdot<T>(i, acc);
+            const T cu = cdot<T>(i, u);
+            const T g = T(3) * (ca - ua) + T(9) * (cu * ca);
+            f[i] = f[i] + (pref * wgt<T>(i)) * g;
+        }
+    }
+}
+
+/* IBcompositeDynamics::collide (src/ibCompositeDynamics3D.hh:130-203) for a
+ * cell with a coupling record (sf > SOLFRAC_MIN).  fPre: pre-collision
+ * populations; f: output.  full = (sf > SOLFRAC_MAX) and the weight
+ * B = sf (tau - 1/2) / ((1 - sf) + (tau - 1/2)) (:177-179) are decided by
+ * the caller in FP64. */
+template <typename T, bool FORCE>
+__device__ __forceinline__ void psm_solid(const T* fPre, T* f, T omega, bool full, T B,
+                                          const T* us, const T* acc, T* F) {
+    F[0] = T(0); F[1] = T(0); F[2] = T(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[i] = fPre[i];
+    if (!full) bgk<T, FORCE>(f, omega, acc);
+    T rhoBar, j[3];
+    moments<T>(f, rhoBar, j);
+    const T invRho = T(1) / (rhoBar + T(1));
+    const T jSqr = norm_sqr<T>(j);
+    const T p = T(1.5) * invRho;
+    const T onePlus = T(1) + rhoBar;
+    T uP[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uP[d] = us[d] * onePlus;
+    const T uPSqr = norm_sqr<T>(uP);
+    if (full) {
+        const T bb0 = equilibrium<T>(0, rhoBar, p, j, jSqr) - equilibrium<T>(0, rhoBar, p, uP, uPSqr);
+        f[0] = fPre[0] + bb0;
+#pragma unroll
+        for (int i = 1; i <= 9; ++i) {
+            const int o = i + 9;
+            const T Ei = equilibrium<T>(i, rhoBar, p, j, jSqr);
+            const T Eo = equilibrium<T>(o, rhoBar, p, j, jSqr);
+            const T Si = equilibrium<T>(i, rhoBar, p, uP, uPSqr);
+            const T So = equilibrium<T>(o, rhoBar, p, uP, uPSqr);
+            const T bb = (fPre[o] - Eo) - (fPre[i] - Si);
+            const T bbo = (fPre[i] - Ei) - (fPre[o] - So);
+            f[i] = fPre[i] + bb;
+            f[o] = fPre[o] + bbo;
+            const T diff = bb - bbo;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (cc(i, d) == 1) F[d] = F[d] - diff;
+                else if (cc(i, d) == -1) F[d] = F[d] + diff;
+            }
+        }
+    } else {
+        const T oneMinB = T(1) - B;
+        const T bb0 = equilibrium<T>(0, rhoBar, p, j, jSqr) - equilibrium<T>(0, rhoBar, p, uP, uPSqr);
+        f[0] = (fPre[0] + oneMinB * (f[0] - fPre[0])) + B * bb0;
+#pragma unroll
+        for (int i = 1; i <= 9; ++i) {
+            const int o = i + 9;
+            const T Ei = equilibrium<T>(i, rhoBar, p, j, jSqr);
+            const T Eo = equilibrium<T>(o, rhoBar, p, j, jSqr);
+            const T Si = equilibrium<T>(i, rhoBar, p, uP, uPSqr);
+            const T So = equilibrium<T>(o, rhoBar, p, uP, uPSqr);
+            const T bb = B * ((fPre[o] - Eo) - (fPre[i] - Si));
+            const T bbo = B * ((fPre[i] - Ei) - (fPre[o] - So));
+            f[i] = (fPre[i] + oneMinB * (f[i] - fPre[i])) + bb;
+            f[o] = (fPre[o] + oneMinB * (f[o] - fPre[o])) + bbo;
+            const T diff = bb - bbo;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (cc(i, d) == 1) F[d] = F[d] - diff;
+                else if (cc(i, d) == -1) F[d] = F[d] + diff;
+            }
+        }
+    }
+}
+
+/* ---- storage conversions ---- */
+template <int PREC> struct Store;
+template <> struct Store<LBDEM_PSM_FP64> {
+    typedef double S; typedef double T;
+    static __device__ __forceinline__ T load(const S* p) { return *p; }
+    static __device__ __forceinline__ void store(S* p, T v) { *p = v; }
+};
+template <> struct Store<LBDEM_PSM_FP32_DEV> {
+    typedef float S; typedef float T;
+    static __device__ __forceinline__ T load(const S* p) { return *p; }
+    static __device__ __forceinline__ void store(S* p, T v) { *p = v; }
+};
+template <> struct Store<LBDEM_PSM_FP16_STORE> {
+    typedef _Float16 S; typedef float T;
+    static __device__ __forceinline__ T load(const S* p) { return (float)(*p); }
+    static __device__ __forceinline__ void store(S* p, T v) { *p = (_Float16)v; }
+};
+
+/* Coupling records, structure of arrays, one buffer of the double buffer. */
+struct Records {
+    double* sf;      /* solid fraction */
+    int32_t* id;     /* particle tag */
+    double* us;      /* 3*cap: particle velocity at the cell, component-major */
+    double* F;       /* 3*cap: hydrodynamic force of the last collision */
+    int32_t* cell;   /* cell index */
+    int32_t* cand;   /* candidate entry of the setter / owner, -1 = none */
+    int32_t* bin_start;  /* per bin: first record */
+    int32_t* bin_count;  /* per bin: number of records */
+};
+
+struct Geom {
+    int n[3];
+    int periodic[3];
+    int nb[3];       /* bins per axis */
+    int64_t nc;
+};
+
+}  // namespace psm
+
+/* The handle. */
+struct lbdem_psm {
+    lbdem_psm_desc desc;
+    psm::Geom g;
+    double omega, ooo, sqrt3half;
+    bool has_force;
+    int precision;
+    size_t pop_bytes;          /* bytes of one population buffer */
+    hipStream_t stream = nullptr;
+    void* f[2] = {nullptr, nullptr};
+    int cur_f = 0;
+    int32_t* slot = nullptr;
+    psm::Records rec[2];
+    int cur_rec = 0;
+    int64_t rec_cap = 0;
+    int64_t nbins = 0;
+
+    /* particles (device) */
+    int64_t np = 0, p_cap = 0;
+    double *px = nullptr, *pv = nullptr, *pw = nullptr, *pr = nullptr;
+    int32_t *pid = nullptr, *pexcl = nullptr, *powner = nullptr;
+    int32_t* id_map = nullptr;       /* id -> first particle index */
+    int64_t id_map_size = 0;
+    std::vector<int32_t> h_owner;    /* host copy of powner */
+    std::vector<int64_t> h_ids;
+    int32_t flags = 0;
+    bool have_particles = false;
+    bool voxelized = false;
+
+    /* binning */
+    int32_t *bin_cand_count = nullptr, *bin_cand_start = nullptr, *bin_fill = nullptr;
+    uint32_t* cand_key = nullptr;
+    int32_t* cand_bin = nullptr;
+    int64_t ent_cap = 0;
+    int32_t *p_nent = nullptr, *p_ent_start = nullptr, *owner_ent = nullptr;
+    int32_t* orphan = nullptr;       /* record indices without a candidate this step */
+    double* partial = nullptr;       /* 6 per candidate entry */
+    double* pft = nullptr;           /* 6 per particle: F[3], T[3] */
+    double* psum_x = nullptr;        /* positions for torque arms */
+    void* scan_tmp = nullptr;
+    size_t scan_tmp_bytes = 0;
+    int64_t* counters = nullptr;     /* device: [0] records, [1] overflow flags, [2] orphans,
+                                        [3] entries, [4] images, [5] defineVelocity cells */
+    int64_t* h_counters = nullptr;   /* pinned host mirror */
+    double* red_tmp = nullptr;       /* reduction scratch */
+    int32_t* dv_cell = nullptr;      /* defineVelocity list (initVel), lazily allocated */
+    double* dv_us = nullptr;
+    int64_t red_blocks = 0;
+
+    /* profiling */
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    struct Span { int cls; hipEvent_t a, b; };
+    std::vector<Span> spans;
+    double prof_ms[4] = {0, 0, 0, 0};
+    int64_t prof_n[4] = {0, 0, 0, 0};
+
+    std::string err;
+};
+
+/* internal entry points across translation units */
+namespace psm {
+int launch_collide_stream(lbdem_psm* h);
+int launch_init_equilibrium(lbdem_psm* h, const double* d_rho, const double* d_u);
+int launch_pop_convert(lbdem_psm* h, double* d_f64, bool to_device_store);
+int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
+                        int64_t* n_nonfinite);
+int run_voxelize(lbdem_psm* h);
+int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_r, double* d_sf,
+                   int32_t* d_cnt, int32_t* d_br);
+int run_force_sum(lbdem_psm* h);
+int launch_advance(lbdem_psm* h);
+int launch_define_velocity(lbdem_psm* h);
+int set_err(lbdem_psm* h, int code, const std::string& msg);
+int hip_check(lbdem_psm* h, hipError_t e, const char* what);
+void prof_begin(lbdem_psm* h, int cls, hipEvent_t* a);
+void prof_end(lbdem_psm* h, int cls, hipEvent_t a);
+}  // namespace psm
+
+#define PSM_HIP(h, call)                                              \
+    do {                                                              \
+        hipError_t e_ = (call);                                       \
+        if (e_ != hipSuccess) return psm::hip_check((h), e_, #call);  \
+    } while (0)
+
+#endif

@@ -1,0 +1,191 @@
+/*
+ * Per-particle hydrodynamic force and torque on gfx950.
+ *
+ * Reference: SumForceTorque3D::process (src/ibProcessors3D.hh:202-275),
+ * which scans every cell and adds the per-cell force F and d x F
+ * (d = cell - x[map(id)] with the minimum image over the block size,
+ * :235-251) into the particle's slot.
+ *
+ * Here the sum runs over the compact record list only, in two
+ * deterministic stages:
+ *  1. per bin (one workgroup): the bin's records are staged in LDS; each
+ *     wavefront takes some of the bin's candidate entries and sums, for
+ *     each, the records that candidate set this step, with a fixed 64-lane
+ *     shuffle tree in FP64.  One 6-vector partial per candidate entry.
+ *  2. per particle: its candidate entries are added in the fixed
+ *     (image rank, bin) order.
+ * Records whose owner set no candidate of their bin this step (stale
+ * records the reference also keeps) are added afterwards with FP64 atomics.
+ */
+#include "psm_common.h"
+
+namespace psm {
+
+struct ForceArgs {
+    int n[3];
+    int nb[3];
+    const int32_t* bin_rec_start;
+    const int32_t* bin_rec_count;
+    const int32_t* bin_cand_start;
+    const int32_t* bin_cand_count;
+    const uint32_t* key;
+    const int32_t* rec_cell;
+    const int32_t* rec_cand;
+    const int32_t* rec_id;
+    const double* rF;
+    int64_t rcap;
+    const int32_t* powner;
+    const double* xs;      /* positions for the torque arms */
+    double* partial;
+    int64_t ent_cap;
+};
+
+__device__ __forceinline__ double min_image(double d, int n) {
+    /* src/ibProcessors3D.hh:238-243, integer halves of the block size */
+    if (d > (double)(n / 2)) d -= (double)n;
+    else if (d < (double)(-n / 2)) d += (double)n;
+    return d;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
+    __shared__ double sF[BIN_CELLS][3];
+    __shared__ int sX[BIN_CELLS][3];
+    __shared__ int sE[BIN_CELLS];
+    const int64_t b = blockIdx.x;
+    const int ncand = a.bin_cand_count[b];
+    if (ncand == 0) return;
+    const int cstart = a.bin_cand_start[b];
+    if ((int64_t)cstart + ncand > a.ent_cap) return;
+    const int cnt = a.bin_rec_count[b];
+    const int rstart = a.bin_rec_start[b];
+    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+        const int r = rstart + k;
+        sF[k][0] = a.rF[r];
+        sF[k][1] = a.rF[a.rcap + r];
+        sF[k][2] = a.rF[2 * a.rcap + r];
+        const int c = a.rec_cell[r];
+        const int z = c % a.n[2];
+        const int t = c / a.n[2];
+        sX[k][0] = t / a.n[1];
+        sX[k][1] = t % a.n[1];
+        sX[k][2] = z;
+        sE[k] = a.rec_cand[r];
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int j = wave; j < ncand; j += 4) {
+        const int e = cstart + j;
+        const int q = (int)(a.key[e] & QMASK);
+        const int o = a.powner[q];
+        const double xo = a.xs[3 * o], yo = a.xs[3 * o + 1], zo = a.xs[3 * o + 2];
+        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int k = lane; k < cnt; k += 64) {
+            if (sE[k] != e) continue;
+            const double dx = min_image((double)sX[k][0] - xo, a.n[0]);
+            const double dy = min_image((double)sX[k][1] - yo, a.n[1]);
+            const double dz = min_image((double)sX[k][2] - zo, a.n[2]);
+            const double fx = sF[k][0], fy = sF[k][1], fz = sF[k][2];
+            acc[0] += fx;
+            acc[1] += fy;
+            acc[2] += fz;
+            acc[3] += dy * fz - dz * fy;
+            acc[4] += -dx * fz + dz * fx;
+            acc[5] += dx * fy - dy * fx;
+        }
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc[m] = wave_sum(acc[m]);
+        if (lane == 0) {
+#pragma unroll
+            for (int m = 0; m < 6; ++m) a.partial[6 * (int64_t)e + m] = acc[m];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_force_particle(int64_t np, const int32_t* __restrict__ p_ent_start,
+                                                        const int32_t* __restrict__ p_nent,
+                                                        const int32_t* __restrict__ owner_ent,
+                                                        const double* __restrict__ partial,
+                                                        double* __restrict__ pft, int64_t ent_cap) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int64_t s = p_ent_start[q];
+    const int n = p_nent[q];
+    for (int k = 0; k < n; ++k) {
+        if (s + k >= ent_cap) break;
+        const int e = owner_ent[s + k];
+        if (e < 0) continue;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) acc[m] += partial[6 * (int64_t)e + m];
+    }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) pft[6 * q + m] = acc[m];
+}
+
+__global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_t* __restrict__ orphan,
+                                                       const int64_t* __restrict__ counters,
+                                                       const int32_t* __restrict__ id_map,
+                                                       int64_t id_map_size, double* __restrict__ pft) {
+    const int64_t n = counters[2] < a.rcap ? counters[2] : a.rcap;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int r = orphan[k];
+        const int id = a.rec_id[r];
+        if (id < 1 || id >= id_map_size) continue;
+        const int o = id_map[id];
+        if (o < 0) continue;
+        const int c = a.rec_cell[r];
+        const int z = c % a.n[2];
+        const int t = c / a.n[2];
+        const double dx = min_image((double)(t / a.n[1]) - a.xs[3 * o], a.n[0]);
+        const double dy = min_image((double)(t % a.n[1]) - a.xs[3 * o + 1], a.n[1]);
+        const double dz = min_image((double)z - a.xs[3 * o + 2], a.n[2]);
+        const double fx = a.rF[r], fy = a.rF[a.rcap + r], fz = a.rF[2 * a.rcap + r];
+        atomicAdd(&pft[6 * o + 0], fx);
+        atomicAdd(&pft[6 * o + 1], fy);
+        atomicAdd(&pft[6 * o + 2], fz);
+        atomicAdd(&pft[6 * o + 3], dy * fz - dz * fy);
+        atomicAdd(&pft[6 * o + 4], -dx * fz + dz * fx);
+        atomicAdd(&pft[6 * o + 5], dx * fy - dy * fx);
+    }
+}
+
+int run_force_sum(lbdem_psm* h) {
+    hipEvent_t ev = nullptr;
+    prof_begin(h, 2, &ev);
+    const Records& r = h->rec[h->cur_rec];
+    ForceArgs a;
+    for (int d = 0; d < 3; ++d) { a.n[d] = h->g.n[d]; a.nb[d] = h->g.nb[d]; }
+    a.bin_rec_start = r.bin_start;
+    a.bin_rec_count = r.bin_count;
+    a.bin_cand_start = h->bin_cand_start;
+    a.bin_cand_count = h->bin_cand_count;
+    a.key = h->cand_key;
+    a.rec_cell = r.cell;
+    a.rec_cand = r.cand;
+    a.rec_id = r.id;
+    a.rF = r.F;
+    a.rcap = h->rec_cap;
+    a.powner = h->powner;
+    a.xs = h->psum_x;
+    a.partial = h->partial;
+    a.ent_cap = h->ent_cap;
+    hipLaunchKernelGGL(k_force_bin, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, a);
+    if (h->np > 0) {
+        hipLaunchKernelGGL(k_force_particle, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->stream,
+                           h->np, h->p_ent_start, h->p_nent, h->owner_ent, h->partial, h->pft, h->ent_cap);
+        hipLaunchKernelGGL(k_force_orphans, dim3(64), dim3(256), 0, h->stream, a, h->orphan, h->counters,
+                           h->id_map, h->id_map_size, h->pft);
+    }
+    PSM_HIP(h, hipGetLastError());
+    prof_end(h, 2, ev);
+    return LBDEM_PSM_OK;
+}
+
+}  // namespace psm

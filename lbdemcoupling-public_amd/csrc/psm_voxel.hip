@@ -1,0 +1,609 @@
+/*
+ * Sphere voxelization on gfx950: SetSingleSphere3D for every particle, as
+ * an ordered per-cell fold.
+ *
+ * Reference: src/ibProcessors3D.hh:52-184 (process, calcSolidFraction,
+ * setValues, setToZero), src/ibProcessors3D.h:43-45 (bounding box),
+ * src/ibDataExchangeWrappers3D.hh:62-140 (particle loop, excludeType,
+ * box intersection).
+ *
+ * The reference visits particles one after another and updates the
+ * per-cell record in place; the outcome of a cell depends on the order of
+ * the particles whose bounding box holds it and on the record left by the
+ * previous step.  Here every particle image is binned into 8^3-cell bins
+ * (key = image rank << 27 | particle index, which is the canonical order:
+ * the given particles by index, then periodic images by shift rank), each
+ * bin's candidate list is sorted, and one workgroup per bin stages the
+ * candidates in LDS and folds them, per cell and in key order, through the
+ * reference's four-case rule.  The surviving records are compacted per bin
+ * (cell order inside the bin) into the next record buffer.
+ *
+ * The sub-cell count is evaluated in FP64 with the reference's expression
+ * order and without contraction (this file is compiled with
+ * -ffp-contract=off), so counts are bit-exact.
+ */
+#include "psm_common.h"
+
+#include <hipcub/hipcub.hpp>
+
+namespace psm {
+
+struct VoxArgs {
+    int n[3];
+    int per[3];
+    int nb[3];
+    int auto_images;
+    int64_t np;
+    const double* px;
+    const double* pr;
+    const int32_t* pexcl;
+};
+
+/* Truncated, clipped bounding box of image `rank` of a particle; false when
+ * the image does not exist or misses the lattice.  Shift digits: 0 none,
+ * 1 -> +n_d (image of the lower-side pass), 2 -> -n_d. */
+__device__ __forceinline__ bool image_box(const double* x, double r, int rank, const VoxArgs& a,
+                                          int* lo, int* hi, double* xi) {
+    int dig[3] = {rank % 3, (rank / 3) % 3, rank / 9};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const int sh = dig[d] == 0 ? 0 : (dig[d] == 1 ? 1 : -1);
+        if (sh != 0 && !a.per[d]) return false;
+        xi[d] = sh == 0 ? x[d] : x[d] + (double)sh * (double)a.n[d];
+        int l = (int)((xi[d] - r) - 2.0);
+        int h = (int)((xi[d] + r) + 2.0);
+        l = l < 0 ? 0 : l;
+        h = h > a.n[d] - 1 ? a.n[d] - 1 : h;
+        if (l > h) return false;
+        lo[d] = l;
+        hi[d] = h;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(VoxArgs a, int32_t* __restrict__ bin_count,
+                                                   int32_t* __restrict__ p_nent) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.np) return;
+    if (a.pexcl[q]) { p_nent[q] = 0; return; }
+    const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
+    const double r = a.pr[q];
+    const int nrank = a.auto_images ? 27 : 1;
+    int total = 0;
+    for (int rank = 0; rank < nrank; ++rank) {
+        int lo[3], hi[3];
+        double xi[3];
+        if (!image_box(x, r, rank, a, lo, hi, xi)) continue;
+        const int b0 = lo[0] / BIN, b1 = hi[0] / BIN;
+        const int c0 = lo[1] / BIN, c1 = hi[1] / BIN;
+        const int d0 = lo[2] / BIN, d1 = hi[2] / BIN;
+        for (int bx = b0; bx <= b1; ++bx)
+            for (int by = c0; by <= c1; ++by)
+                for (int bz = d0; bz <= d1; ++bz)
+                    atomicAdd(&bin_count[((int64_t)bx * a.nb[1] + by) * a.nb[2] + bz], 1);
+        total += (b1 - b0 + 1) * (c1 - c0 + 1) * (d1 - d0 + 1);
+    }
+    p_nent[q] = total;
+}
+
+__global__ __launch_bounds__(256) void k_bin_fill(VoxArgs a, const int32_t* __restrict__ bin_start,
+                                                  int32_t* __restrict__ bin_fill,
+                                                  uint32_t* __restrict__ key, int32_t* __restrict__ kbin,
+                                                  int64_t cap, int64_t* __restrict__ counters) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.np || a.pexcl[q]) return;
+    const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
+    const double r = a.pr[q];
+    const int nrank = a.auto_images ? 27 : 1;
+    for (int rank = 0; rank < nrank; ++rank) {
+        int lo[3], hi[3];
+        double xi[3];
+        if (!image_box(x, r, rank, a, lo, hi, xi)) continue;
+        if (rank > 0) atomicAdd((unsigned long long*)&counters[4], 1ull);
+        for (int bx = lo[0] / BIN; bx <= hi[0] / BIN; ++bx)
+            for (int by = lo[1] / BIN; by <= hi[1] / BIN; ++by)
+                for (int bz = lo[2] / BIN; bz <= hi[2] / BIN; ++bz) {
+                    const int64_t b = ((int64_t)bx * a.nb[1] + by) * a.nb[2] + bz;
+                    const int64_t pos = (int64_t)bin_start[b] + atomicAdd(&bin_fill[b], 1);
+                    if (pos < cap) {
+                        key[pos] = ((uint32_t)rank << QBITS) | (uint32_t)q;
+                        kbin[pos] = (int32_t)b;
+                    } else {
+                        atomicOr((unsigned long long*)&counters[1], 2ull);
+                    }
+                }
+    }
+}
+
+/* Sort each bin's candidate keys (short lists: insertion sort per bin). */
+__global__ __launch_bounds__(256) void k_bin_sort(const int32_t* __restrict__ bin_start,
+                                                  const int32_t* __restrict__ bin_count,
+                                                  uint32_t* __restrict__ key, int64_t nbins, int64_t cap) {
+    const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nbins) return;
+    const int64_t s = bin_start[b];
+    const int n = bin_count[b];
+    if (s + n > cap) return;
+    uint32_t* k = key + s;
+    for (int i = 1; i < n; ++i) {
+        const uint32_t v = k[i];
+        int j = i - 1;
+        while (j >= 0 && k[j] > v) { k[j + 1] = k[j]; --j; }
+        k[j + 1] = v;
+    }
+}
+
+/* For each particle, the entry indices of all its (image, bin) candidate
+ * entries in (rank, bx, by, bz) order: the fixed summation order of the
+ * per-particle force reduction. */
+__global__ __launch_bounds__(256) void k_owner_entries(VoxArgs a, const int32_t* __restrict__ bin_start,
+                                                       const int32_t* __restrict__ bin_count,
+                                                       const uint32_t* __restrict__ key,
+                                                       const int32_t* __restrict__ p_ent_start,
+                                                       int32_t* __restrict__ owner_ent, int64_t cap) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= a.np || a.pexcl[q]) return;
+    const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
+    const double r = a.pr[q];
+    const int nrank = a.auto_images ? 27 : 1;
+    int64_t out = p_ent_start[q];
+    for (int rank = 0; rank < nrank; ++rank) {
+        int lo[3], hi[3];
+        double xi[3];
+        if (!image_box(x, r, rank, a, lo, hi, xi)) continue;
+        const uint32_t want = ((uint32_t)rank << QBITS) | (uint32_t)q;
+        for (int bx = lo[0] / BIN; bx <= hi[0] / BIN; ++bx)
+            for (int by = lo[1] / BIN; by <= hi[1] / BIN; ++by)
+                for (int bz = lo[2] / BIN; bz <= hi[2] / BIN; ++bz) {
+                    const int64_t b = ((int64_t)bx * a.nb[1] + by) * a.nb[2] + bz;
+                    int lo_i = bin_start[b], hi_i = bin_start[b] + bin_count[b] - 1;
+                    int found = -1;
+                    if ((int64_t)hi_i < cap) {
+                        while (lo_i <= hi_i) {
+                            const int mid = (lo_i + hi_i) >> 1;
+                            const uint32_t v = key[mid];
+                            if (v == want) { found = mid; break; }
+                            if (v < want) lo_i = mid + 1;
+                            else hi_i = mid - 1;
+                        }
+                    }
+                    if (out < cap) owner_ent[out] = found;
+                    ++out;
+                }
+    }
+}
+
+/* ---- the fold ---- */
+
+struct Cand {
+    double x[3];
+    double r;
+    double v[3];
+    double w[3];
+    double rp2, rm2, rsq;
+    int lo[3], hi[3];
+    int id;
+    int e;
+};
+
+struct FoldArgs {
+    int n[3];
+    int nb[3];
+    int N;
+    double sw, frac;
+    double sqrt3half;
+    int init_vel;
+    int auto_images;
+    int per[3];
+    const double* px;
+    const double* pv;
+    const double* pw;
+    const double* pr;
+    const int32_t* pid;
+    const int32_t* bin_cand_start;
+    const int32_t* bin_cand_count;
+    const uint32_t* key;
+    int64_t ent_cap;
+    Records oldr;
+    Records newr;
+    int64_t rcap;
+    int32_t* slot;
+    int64_t* counters;
+    int32_t* orphan;
+    int32_t* dv_cell;
+    double* dv_us;
+};
+
+/* calcSolidFraction, src/ibProcessors3D.hh:116-158 (FP64, no contraction,
+ * left-to-right sums). */
+template <int NT>
+__device__ __forceinline__ double solid_fraction(double dx, double dy, double dz, double rp2,
+                                                 double rm2, double rsq, int Nrt, double sw,
+                                                 double frac, int& n, int& br) {
+    const int N = NT > 0 ? NT : Nrt;
+    const double dist = dx * dx + dy * dy + dz * dz;
+    if (dist > rp2) { n = 0; br = 0; return 0.0; }
+    if (dist < rm2) { n = N * N * N; br = 1; return 1.0; }
+    constexpr int M = NT > 0 ? NT : 16;
+    double xs[M], ys[M], zs[M];
+#pragma unroll
+    for (int i = 0; i < M; ++i) {
+        if (i < N) {
+            const double delta = -0.5 + ((double)i + 0.5) * sw;
+            const double ax = dx + delta; xs[i] = ax * ax;
+            const double ay = dy + delta; ys[i] = ay * ay;
+            const double az = dz + delta; zs[i] = az * az;
+        }
+    }
+    unsigned cnt = 0;
+    if (NT > 0) {
+#pragma unroll
+        for (int i = 0; i < M; ++i)
+#pragma unroll
+            for (int j = 0; j < M; ++j) {
+                const double sxy = xs[i] + ys[j];
+#pragma unroll
+                for (int k = 0; k < M; ++k) cnt += (sxy + zs[k] < rsq) ? 1u : 0u;
+            }
+    } else {
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) {
+                const double sxy = xs[i] + ys[j];
+                for (int k = 0; k < N; ++k) cnt += (sxy + zs[k] < rsq) ? 1u : 0u;
+            }
+    }
+    n = (int)cnt;
+    br = 2;
+    return frac * (double)cnt;
+}
+
+enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
+
+struct CellState {
+    int kind;
+    double sf;
+    int id;
+    double us[3];
+    int e;
+    int oldslot;
+    bool dv;
+    double dvus[3];
+};
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_fold(FoldArgs a) {
+    __shared__ Cand cs[64];
+    __shared__ int scan[256];
+    __shared__ long long s_base;
+
+    const int64_t b = blockIdx.x;
+    const int bz = (int)(b % a.nb[2]);
+    const int by = (int)((b / a.nb[2]) % a.nb[1]);
+    const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
+    const int ncand = a.bin_cand_count[b];
+    const int cstart = a.bin_cand_start[b];
+    const int old_cnt = a.oldr.bin_count[b];
+    if (ncand == 0 && old_cnt == 0) {
+        if (threadIdx.x == 0) { a.newr.bin_count[b] = 0; a.newr.bin_start[b] = 0; }
+        return;
+    }
+    if ((int64_t)cstart + ncand > a.ent_cap) return;   /* overflow already flagged */
+
+    CellState st[2];
+    int X[2], Y[2], Z[2];
+    bool valid[2];
+    int64_t cidx[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int l = 2 * threadIdx.x + k;
+        X[k] = bx * BIN + (l >> 6);
+        Y[k] = by * BIN + ((l >> 3) & 7);
+        Z[k] = bz * BIN + (l & 7);
+        valid[k] = X[k] < a.n[0] && Y[k] < a.n[1] && Z[k] < a.n[2];
+        cidx[k] = valid[k] ? ((int64_t)X[k] * a.n[1] + Y[k]) * a.n[2] + Z[k] : 0;
+        st[k].dv = false;
+        st[k].e = -1;
+        st[k].oldslot = valid[k] ? a.slot[cidx[k]] : -1;
+        if (st[k].oldslot >= 0) {
+            st[k].kind = ST_OLD;
+            st[k].sf = a.oldr.sf[st[k].oldslot];
+            st[k].id = a.oldr.id[st[k].oldslot];
+        } else {
+            st[k].kind = ST_ZERO;
+            st[k].sf = 0.0;
+            st[k].id = 0;
+        }
+        st[k].us[0] = st[k].us[1] = st[k].us[2] = 0.0;
+    }
+
+    for (int c0 = 0; c0 < ncand; c0 += 64) {
+        const int nchunk = min(64, ncand - c0);
+        __syncthreads();
+        if ((int)threadIdx.x < nchunk) {
+            const int e = cstart + c0 + threadIdx.x;
+            const uint32_t kk = a.key[e];
+            const int q = (int)(kk & QMASK);
+            const int rank = (int)(kk >> QBITS);
+            Cand& cd = cs[threadIdx.x];
+            const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
+            const double r = a.pr[q];
+            VoxArgs va;
+            for (int d = 0; d < 3; ++d) { va.n[d] = a.n[d]; va.per[d] = a.per[d]; va.nb[d] = a.nb[d]; }
+            va.auto_images = a.auto_images;
+            double xi[3];
+            int lo[3], hi[3];
+            if (!image_box(x, r, rank, va, lo, hi, xi)) { lo[0] = 1; hi[0] = 0; lo[1] = lo[2] = 1; hi[1] = hi[2] = 0; xi[0] = xi[1] = xi[2] = 0.0; }
+            for (int d = 0; d < 3; ++d) {
+                cd.x[d] = xi[d]; cd.lo[d] = lo[d]; cd.hi[d] = hi[d];
+                cd.v[d] = a.pv[3 * q + d]; cd.w[d] = a.pw[3 * q + d];
+            }
+            cd.r = r;
+            const double r_p = r + a.sqrt3half;
+            const double r_m = r - a.sqrt3half;
+            cd.rp2 = r_p * r_p;
+            cd.rm2 = r_m * r_m;
+            cd.rsq = r * r;
+            cd.id = a.pid[q];
+            cd.e = e;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (!valid[k]) continue;
+            CellState& s = st[k];
+            for (int j = 0; j < nchunk; ++j) {
+                const Cand& cd = cs[j];
+                if (X[k] < cd.lo[0] || X[k] > cd.hi[0] || Y[k] < cd.lo[1] || Y[k] > cd.hi[1] ||
+                    Z[k] < cd.lo[2] || Z[k] > cd.hi[2])
+                    continue;
+                const double dx = (double)X[k] - cd.x[0];
+                const double dy = (double)Y[k] - cd.x[1];
+                const double dz = (double)Z[k] - cd.x[2];
+                int nn, br;
+                const double sf = solid_fraction<NT>(dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, a.N, a.sw, a.frac, nn, br);
+                const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (s.sf > SOLFRAC_MIN ? 1 : 0);
+                bool set = false, zero = false;
+                if (flag == 0) zero = true;
+                else if (flag == 1) set = true;
+                else if (flag == 2) zero = (s.id == cd.id);
+                else set = (sf > s.sf || s.id == cd.id);
+                if (zero) {
+                    s.kind = ST_ZERO; s.sf = 0.0; s.id = 0; s.e = -1;
+                    s.us[0] = s.us[1] = s.us[2] = 0.0;
+                }
+                if (set) {   /* setValues, src/ibProcessors3D.hh:162-174 */
+                    s.kind = ST_SET;
+                    s.sf = sf;
+                    s.id = cd.id;
+                    s.e = cd.e;
+                    s.us[0] = cd.v[0] + (cd.w[1] * dz - cd.w[2] * dy);
+                    s.us[1] = cd.v[1] + (-cd.w[0] * dz + cd.w[2] * dx);
+                    s.us[2] = cd.v[2] + (cd.w[0] * dy - cd.w[1] * dx);
+                }
+                if (a.init_vel && sf > SOLFRAC_MAX) {
+                    s.dv = true;
+                    if (s.kind == ST_OLD) {
+                        const int os = s.oldslot;
+                        s.dvus[0] = a.oldr.us[os];
+                        s.dvus[1] = a.oldr.us[a.rcap + os];
+                        s.dvus[2] = a.oldr.us[2 * a.rcap + os];
+                    } else {
+                        s.dvus[0] = s.us[0]; s.dvus[1] = s.us[1]; s.dvus[2] = s.us[2];
+                    }
+                }
+            }
+        }
+    }
+
+    /* compaction in cell order inside the bin */
+    int fl[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) fl[k] = (valid[k] && st[k].kind != ST_ZERO) ? 1 : 0;
+    const int mine = fl[0] + fl[1];
+    __syncthreads();
+    scan[threadIdx.x] = mine;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = (int)threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const int total = scan[255];
+    const int excl = scan[threadIdx.x] - mine;
+    if (threadIdx.x == 0) {
+        long long base = 0;
+        if (total > 0) base = (long long)atomicAdd((unsigned long long*)&a.counters[0], (unsigned long long)total);
+        if (base + total > a.rcap) {
+            atomicOr((unsigned long long*)&a.counters[1], 1ull);
+            base = -1;
+        }
+        s_base = base;
+        a.newr.bin_start[b] = base < 0 ? 0 : (int32_t)base;
+        a.newr.bin_count[b] = base < 0 ? 0 : total;
+    }
+    __syncthreads();
+    const long long base = s_base;
+    int off = excl;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (!valid[k]) continue;
+        CellState& s = st[k];
+        if (!fl[k] || base < 0) {
+            a.slot[cidx[k]] = -1;
+            continue;
+        }
+        const int64_t idx = base + off;
+        ++off;
+        if (s.kind == ST_OLD) {
+            const int os = s.oldslot;
+            s.us[0] = a.oldr.us[os];
+            s.us[1] = a.oldr.us[a.rcap + os];
+            s.us[2] = a.oldr.us[2 * a.rcap + os];
+            const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[2], 1ull);
+            if ((int64_t)p < a.rcap) a.orphan[p] = (int32_t)idx;
+        }
+        a.newr.sf[idx] = s.sf;
+        a.newr.id[idx] = s.id;
+        a.newr.us[idx] = s.us[0];
+        a.newr.us[a.rcap + idx] = s.us[1];
+        a.newr.us[2 * a.rcap + idx] = s.us[2];
+        a.newr.cell[idx] = (int32_t)cidx[k];
+        a.newr.cand[idx] = s.e;
+        a.slot[cidx[k]] = (int32_t)idx;
+        if (s.dv && a.dv_cell) {
+            const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[5], 1ull);
+            if ((int64_t)p < a.rcap) {
+                a.dv_cell[p] = (int32_t)cidx[k];
+                a.dv_us[3 * p] = s.dvus[0];
+                a.dv_us[3 * p + 1] = s.dvus[1];
+                a.dv_us[3 * p + 2] = s.dvus[2];
+            }
+        }
+    }
+    /* cells that end empty but requested defineVelocity keep it too */
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (!valid[k] || fl[k] || !st[k].dv || !a.dv_cell) continue;
+        const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[5], 1ull);
+        if ((int64_t)p < a.rcap) {
+            a.dv_cell[p] = (int32_t)cidx[k];
+            a.dv_us[3 * p] = st[k].dvus[0];
+            a.dv_us[3 * p + 1] = st[k].dvus[1];
+            a.dv_us[3 * p + 2] = st[k].dvus[2];
+        }
+    }
+}
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_subcell(int64_t n, const double* __restrict__ d3,
+                                                 const double* __restrict__ r, int N, double sw,
+                                                 double frac, double h, double* __restrict__ sf,
+                                                 int32_t* __restrict__ cnt, int32_t* __restrict__ br) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double rr = r[k];
+    const double r_p = rr + h, r_m = rr - h;
+    int nn, bb;
+    sf[k] = solid_fraction<NT>(d3[3 * k], d3[3 * k + 1], d3[3 * k + 2], r_p * r_p, r_m * r_m, rr * rr, N, sw, frac, nn, bb);
+    cnt[k] = nn;
+    br[k] = bb;
+}
+
+/* Prescribed kinematics: x += v, wrapped into [-1/2, n - 1/2) on periodic axes. */
+__global__ __launch_bounds__(256) void k_advance(int64_t np, double* __restrict__ px,
+                                                 const double* __restrict__ pv, int n0, int n1, int n2,
+                                                 int p0, int p1, int p2) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const int n[3] = {n0, n1, n2};
+    const int per[3] = {p0, p1, p2};
+    for (int d = 0; d < 3; ++d) {
+        double x = px[3 * q + d] + pv[3 * q + d];
+        if (per[d]) {
+            const double L = (double)n[d];
+            if (x >= L - 0.5) x = x - L;
+            else if (x < -0.5) x = x + L;
+        }
+        px[3 * q + d] = x;
+    }
+}
+
+static VoxArgs vox_args(lbdem_psm* h) {
+    VoxArgs a;
+    for (int d = 0; d < 3; ++d) { a.n[d] = h->g.n[d]; a.per[d] = h->g.periodic[d]; a.nb[d] = h->g.nb[d]; }
+    a.auto_images = (h->flags & LBDEM_PSM_AUTO_IMAGES) ? 1 : 0;
+    a.np = h->np;
+    a.px = h->px; a.pr = h->pr; a.pexcl = h->pexcl;
+    return a;
+}
+
+int run_voxelize(lbdem_psm* h) {
+    hipEvent_t ev = nullptr;
+    prof_begin(h, 1, &ev);
+    const int64_t nbins = h->nbins;
+    VoxArgs va = vox_args(h);
+    /* counters[1] (overflow flags) is cleared by the public entry points only */
+    PSM_HIP(h, hipMemsetAsync(h->counters, 0, sizeof(int64_t), h->stream));
+    PSM_HIP(h, hipMemsetAsync(h->counters + 2, 0, 6 * sizeof(int64_t), h->stream));
+    PSM_HIP(h, hipMemsetAsync(h->bin_cand_count, 0, nbins * sizeof(int32_t), h->stream));
+    PSM_HIP(h, hipMemsetAsync(h->bin_fill, 0, nbins * sizeof(int32_t), h->stream));
+    const unsigned pg = (unsigned)std::max<int64_t>(1, (h->np + 255) / 256);
+    if (h->np > 0)
+        hipLaunchKernelGGL(k_bin_count, dim3(pg), dim3(256), 0, h->stream, va, h->bin_cand_count, h->p_nent);
+    size_t tb = h->scan_tmp_bytes;
+    PSM_HIP(h, hipcub::DeviceScan::ExclusiveSum(h->scan_tmp, tb, h->bin_cand_count, h->bin_cand_start, (int)nbins, h->stream));
+    if (h->np > 0) {
+        tb = h->scan_tmp_bytes;
+        PSM_HIP(h, hipcub::DeviceScan::ExclusiveSum(h->scan_tmp, tb, h->p_nent, h->p_ent_start, (int)h->np, h->stream));
+        hipLaunchKernelGGL(k_bin_fill, dim3(pg), dim3(256), 0, h->stream, va, h->bin_cand_start, h->bin_fill, h->cand_key, h->cand_bin, h->ent_cap, h->counters);
+    }
+    hipLaunchKernelGGL(k_bin_sort, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, h->stream, h->bin_cand_start, h->bin_cand_count, h->cand_key, nbins, h->ent_cap);
+    if (h->np > 0)
+        hipLaunchKernelGGL(k_owner_entries, dim3(pg), dim3(256), 0, h->stream, va, h->bin_cand_start, h->bin_cand_count, h->cand_key, h->p_ent_start, h->owner_ent, h->ent_cap);
+
+    FoldArgs fa;
+    for (int d = 0; d < 3; ++d) { fa.n[d] = h->g.n[d]; fa.nb[d] = h->g.nb[d]; fa.per[d] = h->g.periodic[d]; }
+    fa.N = h->desc.slices_per_dim;
+    fa.sw = 1.0 / (double)fa.N;
+    fa.frac = 1.0 / (double)(fa.N * fa.N * fa.N);
+    fa.sqrt3half = h->sqrt3half;
+    fa.init_vel = (h->flags & LBDEM_PSM_INIT_VEL) ? 1 : 0;
+    fa.auto_images = va.auto_images;
+    fa.px = h->px; fa.pv = h->pv; fa.pw = h->pw; fa.pr = h->pr; fa.pid = h->pid;
+    fa.bin_cand_start = h->bin_cand_start;
+    fa.bin_cand_count = h->bin_cand_count;
+    fa.key = h->cand_key;
+    fa.ent_cap = h->ent_cap;
+    fa.oldr = h->rec[h->cur_rec];
+    fa.newr = h->rec[1 - h->cur_rec];
+    fa.rcap = h->rec_cap;
+    fa.slot = h->slot;
+    fa.counters = h->counters;
+    fa.orphan = h->orphan;
+    fa.dv_cell = fa.init_vel ? h->dv_cell : nullptr;
+    fa.dv_us = h->dv_us;
+    dim3 grid((unsigned)nbins), block(256);
+    if (fa.N == 5) hipLaunchKernelGGL(k_fold<5>, grid, block, 0, h->stream, fa);
+    else if (fa.N == 8) hipLaunchKernelGGL(k_fold<8>, grid, block, 0, h->stream, fa);
+    else hipLaunchKernelGGL(k_fold<0>, grid, block, 0, h->stream, fa);
+    PSM_HIP(h, hipGetLastError());
+    h->cur_rec = 1 - h->cur_rec;
+    prof_end(h, 1, ev);
+    if (fa.init_vel) {
+        int rc = launch_define_velocity(h);
+        if (rc) return rc;
+    }
+    return LBDEM_PSM_OK;
+}
+
+int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_r, double* d_sf,
+                   int32_t* d_cnt, int32_t* d_br) {
+    const int N = h->desc.slices_per_dim;
+    const double sw = 1.0 / (double)N, frac = 1.0 / (double)(N * N * N);
+    dim3 grid((unsigned)std::max<int64_t>(1, (n + 255) / 256)), block(256);
+    if (N == 5) hipLaunchKernelGGL(k_subcell<5>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, d_sf, d_cnt, d_br);
+    else if (N == 8) hipLaunchKernelGGL(k_subcell<8>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, d_sf, d_cnt, d_br);
+    else hipLaunchKernelGGL(k_subcell<0>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, d_sf, d_cnt, d_br);
+    PSM_HIP(h, hipGetLastError());
+    return LBDEM_PSM_OK;
+}
+
+int launch_advance(lbdem_psm* h) {
+    if (h->np == 0) return LBDEM_PSM_OK;
+    hipEvent_t ev = nullptr;
+    prof_begin(h, 3, &ev);
+    hipLaunchKernelGGL(k_advance, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->stream, h->np, h->px, h->pv,
+                       h->g.n[0], h->g.n[1], h->g.n[2], h->g.periodic[0], h->g.periodic[1], h->g.periodic[2]);
+    PSM_HIP(h, hipGetLastError());
+    prof_end(h, 3, ev);
+    return LBDEM_PSM_OK;
+}
+
+size_t scan_temp_bytes(int64_t n) {
+    size_t tb = 0;
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (int32_t*)nullptr, (int32_t*)nullptr, (int)n, (hipStream_t)0);
+    return tb;
+}
+
+}  // namespace psm

@@ -1,0 +1,325 @@
+"""GPU (HIP, through the C ABI) versus the CPU oracle on the same seeded inputs.
+
+Bars: FP64 populations, per-cell records (solid fraction, particle id,
+particle velocity, per-cell force) and sub-cell counts bit-exact (the FP64
+kernels evaluate the oracle's expressions in the same order without
+contraction); per-particle force and torque within 1e-12 relative error
+(only the summation order differs: a parallel tree here, a lexicographic
+scan in the reference).
+"""
+import numpy as np
+import pytest
+
+import lbdem_psm as L
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-12
+
+
+def rel_err(a, b):
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / den) if den > 0 else float(np.abs(a - b).max())
+
+
+def assert_records_equal(rg, ro):
+    assert np.array_equal(rg["sf"], ro["sf"])
+    assert np.array_equal(rg["id"], ro["id"])
+    assert np.array_equal(rg["us"], ro["us"])
+
+
+def make_pair(n, periodic=(1, 1, 1), tau=0.7, N=5, body=(0.0, 0.0, 0.0), precision="fp64"):
+    g = L.Lattice(*n, periodic=periodic, tau=tau, precision=precision, slices_per_dim=N,
+                  body_accel=body)
+    o = O.OracleLattice(*n, periodic=periodic, tau=tau, slices_per_dim=N, body_accel=body)
+    return g, o
+
+
+def random_state(n, seed, u0=0.02):
+    rng = np.random.default_rng(seed)
+    nc = int(np.prod(n))
+    rho = 1 + rng.normal(0, 0.01, nc)
+    u = rng.normal(0, u0, (3, nc))
+    return rho, u
+
+
+@pytest.mark.parametrize("N", [5, 8])
+def test_subcell_counts_bitexact(N):
+    rng = np.random.default_rng(10 + N)
+    m = 20000
+    r = rng.uniform(2.5, 10.0, m)
+    d = rng.normal(0, 1, (m, 3))
+    d *= ((r + rng.uniform(-1.3, 1.3, m)) / np.linalg.norm(d, axis=1))[:, None]
+    # exact grid offsets and dyadic centres too
+    d[:2000] = np.round(d[:2000] * 8) / 8
+    with L.Lattice(4, 4, 4, slices_per_dim=N) as g:
+        sf, cnt, br = g.subcell_counts(d, r)
+    for k in range(m):
+        s, n_, b = O.calc_solid_fraction(*d[k], r[k], N)
+        assert (sf[k], cnt[k], br[k]) == (s, n_, b), k
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 0), (0, 0, 1)])
+def test_fluid_populations_bitexact(periodic):
+    n = (12, 10, 14)
+    g, o = make_pair(n, periodic, tau=0.63)
+    rho, u = random_state(n, 1)
+    g.init_equilibrium(rho, u)
+    o.init_equilibrium(rho, u)
+    f0 = o.get_populations()
+    assert np.array_equal(g.get_populations(), f0)
+    g.collide_stream(10)
+    o.collide_stream(10)
+    assert np.array_equal(g.get_populations(), o.get_populations())
+    mg, jg = g.conservation()
+    mo, jo = o.conservation()
+    assert abs(mg - mo) < 1e-13 and np.allclose(jg, jo, atol=1e-13)
+    g.close()
+
+
+def test_guo_forcing_with_walls_bitexact():
+    n = (9, 6, 8)
+    g, o = make_pair(n, (0, 1, 1), tau=0.9, body=(0.0, 2e-6, 1e-5))
+    rho, u = random_state(n, 2, 0.01)
+    g.init_equilibrium(rho, u)
+    o.init_equilibrium(rho, u)
+    g.collide_stream(20)
+    o.collide_stream(20)
+    assert np.array_equal(g.get_populations(), o.get_populations())
+    g.close()
+
+
+def spheres(n, count, seed, rlo=2.5, rhi=4.5, periodic=(1, 1, 1)):
+    return L.generate_spheres(n, count, L.RADIUS_UNIFORM, a=rlo, b=rhi, periodic=periodic,
+                              gap=0.0, wall_gap=1.0, sigma_v=0.01, sigma_w=1e-3, seed=seed)
+
+
+def run_coupled(n, periodic, N, count, steps, seed, tau=0.7, init_vel=False, body=(0, 0, 0),
+                place_on_faces=True):
+    g, o = make_pair(n, periodic, tau=tau, N=N, body=body)
+    x, v, w, r, ids = spheres(n, count, seed, periodic=periodic)
+    if place_on_faces:
+        # push a few spheres across the periodic faces to exercise images
+        for k, d in enumerate([0, 1, 2]):
+            if periodic[d] and k < count:
+                x[k, d] = -0.3 if k % 2 == 0 else n[d] - 0.6
+    rho, u = random_state(n, seed, 0.01)
+    g.init_equilibrium(rho, u)
+    o.init_equilibrium(rho, u)
+    worst = 0.0
+    for s in range(steps):
+        if s > 0:
+            x = L.advance_positions(x, v, n, periodic)
+        o.set_spheres(x, v, w, r, ids, init_vel=(init_vel and s == 0))
+        g.set_spheres(x, v, w, r, ids, init_vel=(init_vel and s == 0))
+        assert_records_equal(g.cell_records(), o.cell_records())
+        if init_vel and s == 0:
+            assert np.array_equal(g.get_populations(), o.get_populations())
+        o.collide_stream(1)
+        g.collide_stream(1)
+        rg, ro = g.cell_records(), o.cell_records()
+        assert np.array_equal(rg["F"], ro["F"])
+        Fo, To = o.sum_force_torque(x)
+        Fg, Tg = g.sum_force_torque(x)
+        worst = max(worst, rel_err(Fg, Fo), rel_err(Tg, To))
+        assert rel_err(Fg, Fo) < FORCE_RTOL and rel_err(Tg, To) < FORCE_RTOL
+    assert np.array_equal(g.get_populations(), o.get_populations())
+    st = g.stats()
+    g.close()
+    return worst, st
+
+
+@pytest.mark.parametrize("N", [5, 8])
+def test_coupled_steps_periodic(N):
+    worst, st = run_coupled((32, 28, 30), (1, 1, 1), N, 14, 4, seed=3 + N)
+    assert st["images"] > 0 and st["records"] > 0
+    print("worst force rel err", worst, st)
+
+
+def test_coupled_steps_walls_and_force():
+    run_coupled((30, 26, 32), (0, 0, 1), 8, 10, 4, seed=21, body=(0.0, 0.0, 1e-6))
+
+
+def test_coupled_init_vel():
+    run_coupled((24, 24, 24), (1, 1, 1), 5, 6, 2, seed=4, init_vel=True)
+
+
+def test_ownership_history_parity():
+    n = (16, 16, 16)
+    for order in ((0, 1), (1, 0)):
+        for prior in (None, (0.9, 2), (0.2, 1)):
+            g, o = make_pair(n, (0, 0, 0), tau=0.8, N=5)
+            xa = np.array([8.0 - 2.9, 8.0, 8.0])
+            xb = np.array([8.0 + 3.15, 8.0, 8.0])
+            X = np.array([xa, xb])[list(order)]
+            ids = np.array([1, 2])[list(order)]
+            if prior is not None:
+                sf = np.zeros(16 ** 3)
+                pid = np.zeros(16 ** 3, dtype=np.int64)
+                c = (8 * 16 + 8) * 16 + 8
+                sf[c], pid[c] = prior
+                us = np.zeros((3, 16 ** 3))
+                us[:, c] = [0.01, 0.02, 0.03]
+                g.set_cell_records(sf, pid, us)
+                o.set_cell_records(sf, pid, us)
+            V = np.array([[0.01, 0, 0], [-0.01, 0.0, 0.0]])
+            g.set_spheres(X, V, np.zeros((2, 3)), [3.0, 3.0], ids, auto_images=False)
+            o.set_spheres(X, V, np.zeros((2, 3)), [3.0, 3.0], ids, auto_images=False)
+            assert_records_equal(g.cell_records(), o.cell_records())
+            g.collide_stream(1)
+            o.collide_stream(1)
+            Fg, Tg = g.sum_force_torque(X)
+            Fo, To = o.sum_force_torque(X)
+            assert rel_err(Fg, Fo) < FORCE_RTOL and rel_err(Tg, To) < FORCE_RTOL
+            g.close()
+
+
+def test_stale_records_are_kept_and_summed():
+    """A record outside every bounding box keeps its values (the reference only
+    visits cells inside boxes) and still contributes its force to its owner."""
+    n = (16, 16, 16)
+    g, o = make_pair(n, (1, 1, 1), tau=0.8, N=5)
+    sf = np.zeros(16 ** 3)
+    pid = np.zeros(16 ** 3, dtype=np.int64)
+    us = np.zeros((3, 16 ** 3))
+    c = (1 * 16 + 1) * 16 + 1
+    sf[c], pid[c] = 0.5, 1
+    us[:, c] = [0.02, 0.0, -0.01]
+    for lat in (g, o):
+        lat.set_cell_records(sf, pid, us)
+    x = np.array([[9.0, 9.0, 9.0]])
+    for lat in (g, o):
+        lat.set_spheres(x, np.zeros((1, 3)), np.zeros((1, 3)), [2.0], [1])
+    assert_records_equal(g.cell_records(), o.cell_records())
+    assert g.cell_records()["sf"][c] == 0.5
+    g.collide_stream(1)
+    o.collide_stream(1)
+    Fg, Tg = g.sum_force_torque(x)
+    Fo, To = o.sum_force_torque(x)
+    assert g.stats()["orphans"] >= 1
+    assert rel_err(Fg, Fo) < FORCE_RTOL and rel_err(Tg, To) < FORCE_RTOL
+    g.close()
+
+
+def test_user_ghosts_fold_into_owner():
+    """Ghost copies with the owner's tag (LIGGGHTS-style, no automatic images):
+    forces land on map(tag) = the first index, ghost slots stay zero."""
+    n = (20, 20, 20)
+    g, o = make_pair(n, (1, 1, 1), tau=0.7, N=5)
+    x = np.array([[0.4, 10.0, 10.0], [8.0, 8.0, 3.0], [20.4, 10.0, 10.0]])   # 2 = ghost of 0
+    v = np.array([[0.01, 0, 0], [0, -0.01, 0.005], [0.01, 0, 0]])
+    w = np.zeros((3, 3))
+    ids = [5, 6, 5]
+    rho, u = random_state(n, 8, 0.01)
+    for lat in (g, o):
+        lat.init_equilibrium(rho, u)
+        lat.set_spheres(x, v, w, [3.0, 2.5, 3.0], ids, auto_images=False)
+    assert_records_equal(g.cell_records(), o.cell_records())
+    g.collide_stream(1)
+    o.collide_stream(1)
+    Fg, Tg = g.sum_force_torque(x)
+    Fo, To = o.sum_force_torque(x)
+    assert np.all(Fg[2] == 0) and np.all(Tg[2] == 0)
+    assert rel_err(Fg, Fo) < FORCE_RTOL and rel_err(Tg, To) < FORCE_RTOL
+    g.close()
+
+
+def test_exclude_types_parity():
+    n = (20, 20, 20)
+    g, o = make_pair(n, (1, 1, 1), tau=0.7, N=8)
+    x, v, w, r, ids = spheres(n, 6, 12)
+    types = [1, 2, 1, 3, 2, 1]
+    for lat in (g, o):
+        lat.set_spheres(x, v, w, r, ids, types=types, exclude=[2])
+    assert_records_equal(g.cell_records(), o.cell_records())
+    g.close()
+
+
+def test_resident_step_matches_host_loop():
+    n = (32, 32, 32)
+    x, v, w, r, ids = spheres(n, 12, 31)
+    a = L.Lattice(*n, tau=0.7, slices_per_dim=8)
+    b = L.Lattice(*n, tau=0.7, slices_per_dim=8)
+    a.set_spheres(x, v, w, r, ids)
+    b.set_spheres(x, v, w, r, ids)
+    # a: one host-driven step then resident steps; b: host loop throughout
+    a.collide_stream(1)
+    a.sum_force_torque(x)
+    a.step(3, advance=True)
+    b.collide_stream(1)
+    b.sum_force_torque(x)
+    xx = x
+    for _ in range(3):
+        xx = L.advance_positions(xx, v, n, (1, 1, 1))
+        b.set_spheres(xx, v, w, r, ids)
+        b.collide_stream(1)
+        Fb, Tb = b.sum_force_torque(xx)
+    Fa, Ta = a.particle_forces()
+    assert np.array_equal(a.particles(), xx)
+    assert np.array_equal(a.get_populations(), b.get_populations())
+    assert np.array_equal(Fa, Fb) and np.array_equal(Ta, Tb)
+    a.close()
+    b.close()
+
+
+def test_force_sum_is_deterministic():
+    n = (40, 40, 40)
+    x, v, w, r, ids = spheres(n, 30, 77, 3.0, 6.0)
+    out = []
+    for _ in range(2):
+        g = L.Lattice(*n, tau=0.7, slices_per_dim=8)
+        g.set_spheres(x, v, w, r, ids)
+        g.step(3, advance=True)
+        out.append(g.particle_forces())
+        g.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("precision,drag_tol", [("fp32_dev", 1e-4), ("fp16_store", 0.2)])
+def test_precision_variants_track_fp64(precision, drag_tol):
+    n = (48, 48, 48)
+    x, v, w, r, ids = spheres(n, 20, 5, 3.0, 6.0)
+    res = {}
+    for prec in ("fp64", precision):
+        g = L.Lattice(*n, tau=0.7, precision=prec, slices_per_dim=8)
+        g.set_spheres(x, v, w, r, ids)
+        g.step(20, advance=True)
+        F, T = g.particle_forces()
+        res[prec] = (F, T, g.get_populations())
+        g.close()
+    F64, T64, f64 = res["fp64"]
+    F, T, f = res[precision]
+    drag = rel_err(F, F64)
+    torque = rel_err(T, T64)
+    _, u64 = L.macroscopic(f64)
+    _, u = L.macroscopic(f)
+    uerr = float(np.abs(u - u64).max() / np.abs(u64).max())
+    print(precision, "drag", drag, "torque", torque, "velocity", uerr)
+    assert drag < drag_tol
+
+
+def test_numeric_health_and_errors():
+    g = L.Lattice(8, 8, 8, tau=0.7)
+    rc, mu, nf = g.check_health()
+    assert rc == 0 and mu == 0.0 and nf == 0
+    f = g.get_populations()
+    f[3, 17] = np.nan
+    g.set_populations(f)
+    rc, mu, nf = g.check_health()
+    assert rc == L.ERR_NUMERIC and nf == 1
+    with pytest.raises(L.PSMError) as e:
+        g.set_spheres([[1.0, 1.0, 1.0]], [[0.0, 0, 0]], [[0.0, 0, 0]], [-1.0], [1])
+    assert e.value.code == L.ERR_INVALID_ARG
+    with pytest.raises(L.PSMError) as e:
+        g.sum_force_torque()
+    assert e.value.code == L.ERR_STATE
+    g.close()
+
+
+def test_record_capacity_error():
+    g = L.Lattice(24, 24, 24, tau=0.7, max_solid_cells=50)
+    with pytest.raises(L.PSMError) as e:
+        g.set_spheres([[12.0, 12.0, 12.0]], [[0.0, 0, 0]], [[0.0, 0, 0]], [5.0], [1])
+    assert e.value.code == L.ERR_CAPACITY
+    g.close()
