@@ -163,15 +163,15 @@ def main():
         log(f"{prec}: {res[prec]['mlups']:.0f} MLUPS full step, kernel {res[prec]['kernel_gbs']:.0f} GB/s")
     f64 = res["fp64"]
 
-    def summary(rr):
+    def summary(rr, ref=f64):
         return {"mlups": round(rr["mlups"], 1), "ms_per_step": round(rr["ms_per_step"], 4),
                 "kernel_mlups": round(rr["kernel_mlups"], 1), "kernel_ms": round(rr["kernel_ms"], 4),
                 "kernel_gbs": round(rr["kernel_gbs"], 1),
                 "kernel_roofline_frac": round(rr["kernel_gbs"] / HBM_PEAK_GBS, 4),
                 "step_roofline_frac": round(rr["mlups"] * 1e6 * rr["bytes_per_lu"] / 1e9 / HBM_PEAK_GBS, 4),
                 "voxelize_ms": round(rr["voxelize_ms"], 4), "force_ms": round(rr["force_ms"], 4),
-                "drag_rel_err_vs_fp64": rel(rr["F"], f64["F"]),
-                "torque_rel_err_vs_fp64": rel(rr["T"], f64["T"])}
+                "drag_rel_err_vs_fp64": rel(rr["F"], ref["F"]),
+                "torque_rel_err_vs_fp64": rel(rr["T"], ref["T"])}
 
     extra = {}
     if not args.no_extra and args.workload == "dense256":
@@ -183,9 +183,7 @@ def main():
         log(f"channel512 fp32_dev: {r32['mlups']:.0f} MLUPS, kernel {r32['kernel_gbs']:.0f} GB/s")
         r64 = run_gpu(wl5, "fp64", steps5, max(2, args.warmup // 2), sph5)
         log(f"channel512 fp64: {r64['mlups']:.0f} MLUPS")
-        e32 = {k: v for k, v in summary(r32).items()}
-        e32["drag_rel_err_vs_fp64"] = rel(r32["F"], r64["F"])
-        e32["torque_rel_err_vs_fp64"] = rel(r32["T"], r64["T"])
+        e32 = summary(r32, r64)
         extra["channel512"] = {"workload": wl5["desc"], "steps": steps5,
                                "fp32_dev": e32,
                                "fp64": {"mlups": round(r64["mlups"], 1),

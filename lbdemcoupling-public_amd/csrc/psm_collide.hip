@@ -33,7 +33,78 @@ struct CollideArgs {
     int64_t rcap;
 };
 
-template <int PREC, bool FORCE>
+/* BGK output of population i from the pre-collision value and the
+ * pre-collision moments (the bgk() expression, evaluated for one i). */
+template <typename T, bool FORCE>
+struct BgkPre {
+    T rhoBar, rho, p, omm, omega, jSqr;
+    T jEq[3], u[3], acc[3], pref, ua;
+    __device__ __forceinline__ void init(const T* f, T omega_, const T* acc_) {
+        T j[3];
+        moments<T>(f, rhoBar, j);
+        rho = rhoBar + T(1);
+        const T invRho = T(1) / rho;
+        jEq[0] = j[0]; jEq[1] = j[1]; jEq[2] = j[2];
+        u[0] = u[1] = u[2] = T(0);
+        omega = omega_;
+        if (FORCE) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                acc[d] = acc_[d];
+                u[d] = j[d] * invRho + T(0.5) * acc[d];
+                jEq[d] = rho * u[d];
+            }
+        }
+        jSqr = norm_sqr<T>(jEq);
+        p = T(1.5) * invRho;
+        omm = T(1) - omega;
+        if (FORCE) {
+            pref = (T(1) - T(0.5) * omega) * rho;
+            ua = (u[0] * acc[0] + u[1] * acc[1]) + u[2] * acc[2];
+        }
+    }
+    __device__ __forceinline__ T out(int i, T fi) const {
+        T r = fi * omm + omega * equilibrium<T>(i, rhoBar, p, jEq, jSqr);
+        if (FORCE) {
+            const T ca = cdot<T>(i, acc);
+            const T cu = cdot<T>(i, u);
+            const T g = T(3) * (ca - ua) + T(9) * (cu * ca);
+            r = r + (pref * wgt<T>(i)) * g;
+        }
+        return r;
+    }
+};
+
+/* One workgroup per z-row (x, y uniform): per direction the destination
+ * row and the x/y wall test are uniform, only z varies per lane. */
+template <typename S, bool PZ>
+struct Pusher {
+    S* fout;
+    int64_t nc;
+    int x, y, nx, ny, nz, px, py;
+    __device__ __forceinline__ void put(int i, int z, uint32_t c, S v) const {
+        int xd = x + cx(i), yd = y + cy(i);
+        bool wall = false;
+        if (xd < 0 || xd >= nx) { if (px) xd = xd < 0 ? xd + nx : xd - nx; else wall = true; }
+        if (yd < 0 || yd >= ny) { if (py) yd = yd < 0 ? yd + ny : yd - ny; else wall = true; }
+        int zd = z + cz(i);
+        bool zwall = false;
+        if (cz(i) != 0) {
+            if (PZ) zd = zd < 0 ? zd + nz : (zd >= nz ? zd - nz : zd);
+            else zwall = zd < 0 || zd >= nz;
+        }
+        if (wall) {
+            fout[(size_t)opp(i) * nc + c] = v;
+        } else {
+            S* base = fout + (size_t)i * nc;
+            const uint32_t rowd = (uint32_t)(xd * ny + yd) * (uint32_t)nz;
+            if (zwall) fout[(size_t)opp(i) * nc + c] = v;
+            else base[rowd + (uint32_t)zd] = v;
+        }
+    }
+};
+
+template <int PREC, bool FORCE, bool PZ>
 __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PREC>::S* __restrict__ fin,
                                                         typename Store<PREC>::S* __restrict__ fout,
                                                         CollideArgs a) {
@@ -45,51 +116,93 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     const int64_t nc = a.nc;
     const T omega = T(a.omega);
     const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
+    Pusher<S, PZ> push{fout, nc, x, y, a.nx, a.ny, a.nz, a.px, a.py};
 
     for (int z = threadIdx.x; z < a.nz; z += blockDim.x) {
-        const int64_t c = (int64_t)row * a.nz + z;
+        const uint32_t c = (uint32_t)row * (uint32_t)a.nz + (uint32_t)z;
+        const int s = a.slot[c];
         T fp[Q];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) fp[i] = Store<PREC>::load(fin + (int64_t)i * nc + c);
-        T fo[Q];
-        const int s = a.slot[c];
+        for (int i = 0; i < Q; ++i) fp[i] = Store<PREC>::load(fin + (size_t)i * nc + c);
+        BgkPre<T, FORCE> pre;
+        pre.init(fp, omega, acc);
         if (s < 0) {
 #pragma unroll
-            for (int i = 0; i < Q; ++i) fo[i] = fp[i];
-            bgk<T, FORCE>(fo, omega, acc);
+            for (int i = 0; i < Q; ++i) push.put(i, z, c, (S)pre.out(i, fp[i]));
+            continue;
+        }
+        /* cell with a coupling record: IBcompositeDynamics::collide */
+        const double sf = a.rsf[s];
+        const bool full = sf > SOLFRAC_MAX;
+        const T B = T((sf * a.ooo) / ((1.0 - sf) + a.ooo));
+        const T us[3] = {T(a.rus[s]), T(a.rus[a.rcap + s]), T(a.rus[2 * a.rcap + s])};
+        /* moments of the post-BGK cell (the BGK outputs are recomputed below,
+         * bit-identically, instead of being kept) */
+        T rhoB, jB[3];
+        if (full) {
+            rhoB = pre.rhoBar;
+            moments<T>(fp, rhoB, jB);
         } else {
-            const double sf = a.rsf[s];
-            const bool full = sf > SOLFRAC_MAX;
-            const double Bd = (sf * a.ooo) / ((1.0 - sf) + a.ooo);
-            const T us[3] = {T(a.rus[s]), T(a.rus[a.rcap + s]), T(a.rus[2 * a.rcap + s])};
-            T F[3];
-            psm_solid<T, FORCE>(fp, fo, omega, full, T(Bd), us, acc, F);
-            a.rF[s] = double(F[0]);
-            a.rF[a.rcap + s] = double(F[1]);
-            a.rF[2 * a.rcap + s] = double(F[2]);
-        }
-        /* push streaming */
+            rhoB = T(0); jB[0] = jB[1] = jB[2] = T(0);
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            int xd = x + cx(i), yd = y + cy(i), zd = z + cz(i);
-            bool wall = false;
-            if (xd < 0 || xd >= a.nx) {
-                if (a.px) xd = xd < 0 ? xd + a.nx : xd - a.nx;
-                else wall = true;
+            for (int i = 0; i < Q; ++i) {
+                const T fb = pre.out(i, fp[i]);
+                rhoB = rhoB + fb;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (cc(i, d) == 1) jB[d] = jB[d] + fb;
+                    else if (cc(i, d) == -1) jB[d] = jB[d] - fb;
+                }
             }
-            if (yd < 0 || yd >= a.ny) {
-                if (a.py) yd = yd < 0 ? yd + a.ny : yd - a.ny;
-                else wall = true;
-            }
-            if (zd < 0 || zd >= a.nz) {
-                if (a.pz) zd = zd < 0 ? zd + a.nz : zd - a.nz;
-                else wall = true;
-            }
-            S* dst;
-            if (wall) dst = fout + (int64_t)opp(i) * nc + c;
-            else dst = fout + (int64_t)i * nc + ((int64_t)xd * a.ny + yd) * a.nz + zd;
-            Store<PREC>::store(dst, fo[i]);
         }
+        const T invRho = T(1) / (rhoB + T(1));
+        const T jSqr = norm_sqr<T>(jB);
+        const T p = T(1.5) * invRho;
+        const T onePlus = T(1) + rhoB;
+        T uP[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) uP[d] = us[d] * onePlus;
+        const T uPSqr = norm_sqr<T>(uP);
+        const T oneMinB = T(1) - B;
+        T F[3] = {T(0), T(0), T(0)};
+        {
+            const T bb0 = equilibrium<T>(0, rhoB, p, jB, jSqr) - equilibrium<T>(0, rhoB, p, uP, uPSqr);
+            T o0;
+            if (full) o0 = fp[0] + bb0;
+            else o0 = (fp[0] + oneMinB * (pre.out(0, fp[0]) - fp[0])) + B * bb0;
+            push.put(0, z, c, (S)o0);
+        }
+#pragma unroll
+        for (int i = 1; i <= 9; ++i) {
+            const int o = i + 9;
+            const T Ei = equilibrium<T>(i, rhoB, p, jB, jSqr);
+            const T Eo = equilibrium<T>(o, rhoB, p, jB, jSqr);
+            const T Si = equilibrium<T>(i, rhoB, p, uP, uPSqr);
+            const T So = equilibrium<T>(o, rhoB, p, uP, uPSqr);
+            T bb, bbo, oi, oo;
+            if (full) {
+                bb = (fp[o] - Eo) - (fp[i] - Si);
+                bbo = (fp[i] - Ei) - (fp[o] - So);
+                oi = fp[i] + bb;
+                oo = fp[o] + bbo;
+            } else {
+                bb = B * ((fp[o] - Eo) - (fp[i] - Si));
+                bbo = B * ((fp[i] - Ei) - (fp[o] - So));
+                oi = (fp[i] + oneMinB * (pre.out(i, fp[i]) - fp[i])) + bb;
+                oo = (fp[o] + oneMinB * (pre.out(o, fp[o]) - fp[o])) + bbo;
+            }
+            push.put(i, z, c, (S)oi);
+            push.put(o, z, c, (S)oo);
+            const T diff = bb - bbo;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (cc(i, d) == 1) F[d] = F[d] - diff;
+                else if (cc(i, d) == -1) F[d] = F[d] + diff;
+            }
+        }
+        a.rF[s] = double(F[0]);
+        a.rF[a.rcap + s] = double(F[1]);
+        a.rF[2 * a.rcap + s] = double(F[2]);
     }
 }
 
@@ -230,10 +343,13 @@ static void launch_cs(lbdem_psm* h, const CollideArgs& a) {
     const S* fin = (const S*)h->f[h->cur_f];
     S* fout = (S*)h->f[1 - h->cur_f];
     dim3 grid((unsigned)(a.nx * a.ny)), block(256);
-    if (h->has_force)
-        hipLaunchKernelGGL((k_collide_stream<PREC, true>), grid, block, 0, h->stream, fin, fout, a);
-    else
-        hipLaunchKernelGGL((k_collide_stream<PREC, false>), grid, block, 0, h->stream, fin, fout, a);
+    if (h->has_force) {
+        if (a.pz) hipLaunchKernelGGL((k_collide_stream<PREC, true, true>), grid, block, 0, h->stream, fin, fout, a);
+        else hipLaunchKernelGGL((k_collide_stream<PREC, true, false>), grid, block, 0, h->stream, fin, fout, a);
+    } else {
+        if (a.pz) hipLaunchKernelGGL((k_collide_stream<PREC, false, true>), grid, block, 0, h->stream, fin, fout, a);
+        else hipLaunchKernelGGL((k_collide_stream<PREC, false, false>), grid, block, 0, h->stream, fin, fout, a);
+    }
 }
 
 int launch_collide_stream(lbdem_psm* h) {

@@ -107,25 +107,33 @@ __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
     }
 }
 
+/* One wavefront per particle: lane-strided partial sums of its candidate
+ * entries, then the fixed 64-lane shuffle tree. */
 __global__ __launch_bounds__(256) void k_force_particle(int64_t np, const int32_t* __restrict__ p_ent_start,
                                                         const int32_t* __restrict__ p_nent,
                                                         const int32_t* __restrict__ owner_ent,
                                                         const double* __restrict__ partial,
                                                         double* __restrict__ pft, int64_t ent_cap) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= np) return;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = (int)__lane_id();
     double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    const int64_t s = p_ent_start[q];
-    const int n = p_nent[q];
-    for (int k = 0; k < n; ++k) {
-        if (s + k >= ent_cap) break;
-        const int e = owner_ent[s + k];
-        if (e < 0) continue;
+    if (q < np) {
+        const int64_t s = p_ent_start[q];
+        const int n = p_nent[q];
+        for (int k = lane; k < n; k += 64) {
+            if (s + k >= ent_cap) break;
+            const int e = owner_ent[s + k];
+            if (e < 0) continue;
 #pragma unroll
-        for (int m = 0; m < 6; ++m) acc[m] += partial[6 * (int64_t)e + m];
+            for (int m = 0; m < 6; ++m) acc[m] += partial[6 * (int64_t)e + m];
+        }
     }
 #pragma unroll
-    for (int m = 0; m < 6; ++m) pft[6 * q + m] = acc[m];
+    for (int m = 0; m < 6; ++m) acc[m] = wave_sum(acc[m]);
+    if (q < np && lane == 0) {
+#pragma unroll
+        for (int m = 0; m < 6; ++m) pft[6 * q + m] = acc[m];
+    }
 }
 
 __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_t* __restrict__ orphan,
@@ -178,7 +186,7 @@ int run_force_sum(lbdem_psm* h) {
     a.ent_cap = h->ent_cap;
     hipLaunchKernelGGL(k_force_bin, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, a);
     if (h->np > 0) {
-        hipLaunchKernelGGL(k_force_particle, dim3((unsigned)((h->np + 255) / 256)), dim3(256), 0, h->stream,
+        hipLaunchKernelGGL(k_force_particle, dim3((unsigned)((h->np + 3) / 4)), dim3(256), 0, h->stream,
                            h->np, h->p_ent_start, h->p_nent, h->owner_ent, h->partial, h->pft, h->ent_cap);
         hipLaunchKernelGGL(k_force_orphans, dim3(64), dim3(256), 0, h->stream, a, h->orphan, h->counters,
                            h->id_map, h->id_map_size, h->pft);

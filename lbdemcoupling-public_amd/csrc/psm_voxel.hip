@@ -61,57 +61,86 @@ __device__ __forceinline__ bool image_box(const double* x, double r, int rank, c
     return true;
 }
 
+/* One wavefront per particle: lane r < 27 evaluates image rank r; the
+ * images are then visited in rank order and their bins spread over the
+ * lanes, bins of an image in (bx, by, bz) order (bz fastest). */
+struct ImageRanges {
+    unsigned long long mask;   /* valid images */
+    int b0, c0, d0, nyr, nzr, vol;   /* of the lane's own image */
+};
+
+__device__ __forceinline__ ImageRanges lane_image(const VoxArgs& a, int64_t q, bool live) {
+    ImageRanges ir;
+    const int lane = (int)__lane_id();
+    const int nrank = a.auto_images ? 27 : 1;
+    bool ok = false;
+    int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+    if (live && lane < nrank && !a.pexcl[q]) {
+        const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
+        double xi[3];
+        ok = image_box(x, a.pr[q], lane, a, lo, hi, xi);
+    }
+    ir.mask = __ballot(ok);
+    ir.b0 = lo[0] / BIN; ir.c0 = lo[1] / BIN; ir.d0 = lo[2] / BIN;
+    ir.nyr = ok ? hi[1] / BIN - ir.c0 + 1 : 0;
+    ir.nzr = ok ? hi[2] / BIN - ir.d0 + 1 : 0;
+    ir.vol = ok ? (hi[0] / BIN - ir.b0 + 1) * ir.nyr * ir.nzr : 0;
+    return ir;
+}
+
+__device__ __forceinline__ int64_t image_bin(const VoxArgs& a, int b0, int c0, int d0, int nyr, int nzr, int t) {
+    const int bx = b0 + t / (nyr * nzr);
+    const int rem = t - (bx - b0) * nyr * nzr;
+    const int by = c0 + rem / nzr;
+    const int bz = d0 + rem - (by - c0) * nzr;
+    return ((int64_t)bx * a.nb[1] + by) * a.nb[2] + bz;
+}
+
 __global__ __launch_bounds__(256) void k_bin_count(VoxArgs a, int32_t* __restrict__ bin_count,
                                                    int32_t* __restrict__ p_nent) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.np) return;
-    if (a.pexcl[q]) { p_nent[q] = 0; return; }
-    const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
-    const double r = a.pr[q];
-    const int nrank = a.auto_images ? 27 : 1;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const bool live = q < a.np;
+    const ImageRanges ir = lane_image(a, q, live);
+    const int lane = (int)__lane_id();
+    unsigned long long m = ir.mask;
     int total = 0;
-    for (int rank = 0; rank < nrank; ++rank) {
-        int lo[3], hi[3];
-        double xi[3];
-        if (!image_box(x, r, rank, a, lo, hi, xi)) continue;
-        const int b0 = lo[0] / BIN, b1 = hi[0] / BIN;
-        const int c0 = lo[1] / BIN, c1 = hi[1] / BIN;
-        const int d0 = lo[2] / BIN, d1 = hi[2] / BIN;
-        for (int bx = b0; bx <= b1; ++bx)
-            for (int by = c0; by <= c1; ++by)
-                for (int bz = d0; bz <= d1; ++bz)
-                    atomicAdd(&bin_count[((int64_t)bx * a.nb[1] + by) * a.nb[2] + bz], 1);
-        total += (b1 - b0 + 1) * (c1 - c0 + 1) * (d1 - d0 + 1);
+    while (m) {
+        const int r = __ffsll(m) - 1;
+        m &= m - 1;
+        const int b0 = __shfl(ir.b0, r), c0 = __shfl(ir.c0, r), d0 = __shfl(ir.d0, r);
+        const int nyr = __shfl(ir.nyr, r), nzr = __shfl(ir.nzr, r), vol = __shfl(ir.vol, r);
+        for (int t = lane; t < vol; t += 64) atomicAdd(&bin_count[image_bin(a, b0, c0, d0, nyr, nzr, t)], 1);
+        total += vol;
     }
-    p_nent[q] = total;
+    if (live && lane == 0) p_nent[q] = total;
 }
 
 __global__ __launch_bounds__(256) void k_bin_fill(VoxArgs a, const int32_t* __restrict__ bin_start,
                                                   int32_t* __restrict__ bin_fill,
                                                   uint32_t* __restrict__ key, int32_t* __restrict__ kbin,
                                                   int64_t cap, int64_t* __restrict__ counters) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.np || a.pexcl[q]) return;
-    const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
-    const double r = a.pr[q];
-    const int nrank = a.auto_images ? 27 : 1;
-    for (int rank = 0; rank < nrank; ++rank) {
-        int lo[3], hi[3];
-        double xi[3];
-        if (!image_box(x, r, rank, a, lo, hi, xi)) continue;
-        if (rank > 0) atomicAdd((unsigned long long*)&counters[4], 1ull);
-        for (int bx = lo[0] / BIN; bx <= hi[0] / BIN; ++bx)
-            for (int by = lo[1] / BIN; by <= hi[1] / BIN; ++by)
-                for (int bz = lo[2] / BIN; bz <= hi[2] / BIN; ++bz) {
-                    const int64_t b = ((int64_t)bx * a.nb[1] + by) * a.nb[2] + bz;
-                    const int64_t pos = (int64_t)bin_start[b] + atomicAdd(&bin_fill[b], 1);
-                    if (pos < cap) {
-                        key[pos] = ((uint32_t)rank << QBITS) | (uint32_t)q;
-                        kbin[pos] = (int32_t)b;
-                    } else {
-                        atomicOr((unsigned long long*)&counters[1], 2ull);
-                    }
-                }
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const bool live = q < a.np;
+    const ImageRanges ir = lane_image(a, q, live);
+    const int lane = (int)__lane_id();
+    if (lane == 0 && ir.mask > 1ull)
+        atomicAdd((unsigned long long*)&counters[4], (unsigned long long)(__popcll(ir.mask) - (ir.mask & 1ull)));
+    unsigned long long m = ir.mask;
+    while (m) {
+        const int r = __ffsll(m) - 1;
+        m &= m - 1;
+        const int b0 = __shfl(ir.b0, r), c0 = __shfl(ir.c0, r), d0 = __shfl(ir.d0, r);
+        const int nyr = __shfl(ir.nyr, r), nzr = __shfl(ir.nzr, r), vol = __shfl(ir.vol, r);
+        for (int t = lane; t < vol; t += 64) {
+            const int64_t b = image_bin(a, b0, c0, d0, nyr, nzr, t);
+            const int64_t pos = (int64_t)bin_start[b] + atomicAdd(&bin_fill[b], 1);
+            if (pos < cap) {
+                key[pos] = ((uint32_t)r << QBITS) | (uint32_t)q;
+                kbin[pos] = (int32_t)b;
+            } else {
+                atomicOr((unsigned long long*)&counters[1], 2ull);
+            }
+        }
     }
 }
 
@@ -141,35 +170,34 @@ __global__ __launch_bounds__(256) void k_owner_entries(VoxArgs a, const int32_t*
                                                        const uint32_t* __restrict__ key,
                                                        const int32_t* __restrict__ p_ent_start,
                                                        int32_t* __restrict__ owner_ent, int64_t cap) {
-    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= a.np || a.pexcl[q]) return;
-    const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
-    const double r = a.pr[q];
-    const int nrank = a.auto_images ? 27 : 1;
-    int64_t out = p_ent_start[q];
-    for (int rank = 0; rank < nrank; ++rank) {
-        int lo[3], hi[3];
-        double xi[3];
-        if (!image_box(x, r, rank, a, lo, hi, xi)) continue;
-        const uint32_t want = ((uint32_t)rank << QBITS) | (uint32_t)q;
-        for (int bx = lo[0] / BIN; bx <= hi[0] / BIN; ++bx)
-            for (int by = lo[1] / BIN; by <= hi[1] / BIN; ++by)
-                for (int bz = lo[2] / BIN; bz <= hi[2] / BIN; ++bz) {
-                    const int64_t b = ((int64_t)bx * a.nb[1] + by) * a.nb[2] + bz;
-                    int lo_i = bin_start[b], hi_i = bin_start[b] + bin_count[b] - 1;
-                    int found = -1;
-                    if ((int64_t)hi_i < cap) {
-                        while (lo_i <= hi_i) {
-                            const int mid = (lo_i + hi_i) >> 1;
-                            const uint32_t v = key[mid];
-                            if (v == want) { found = mid; break; }
-                            if (v < want) lo_i = mid + 1;
-                            else hi_i = mid - 1;
-                        }
-                    }
-                    if (out < cap) owner_ent[out] = found;
-                    ++out;
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const bool live = q < a.np;
+    const ImageRanges ir = lane_image(a, q, live);
+    const int lane = (int)__lane_id();
+    unsigned long long m = ir.mask;
+    int64_t out = live ? p_ent_start[q] : 0;
+    while (m) {
+        const int r = __ffsll(m) - 1;
+        m &= m - 1;
+        const int b0 = __shfl(ir.b0, r), c0 = __shfl(ir.c0, r), d0 = __shfl(ir.d0, r);
+        const int nyr = __shfl(ir.nyr, r), nzr = __shfl(ir.nzr, r), vol = __shfl(ir.vol, r);
+        const uint32_t want = ((uint32_t)r << QBITS) | (uint32_t)q;
+        for (int t = lane; t < vol; t += 64) {
+            const int64_t b = image_bin(a, b0, c0, d0, nyr, nzr, t);
+            int lo_i = bin_start[b], hi_i = bin_start[b] + bin_count[b] - 1;
+            int found = -1;
+            if ((int64_t)hi_i < cap) {
+                while (lo_i <= hi_i) {
+                    const int mid = (lo_i + hi_i) >> 1;
+                    const uint32_t v = key[mid];
+                    if (v == want) { found = mid; break; }
+                    if (v < want) lo_i = mid + 1;
+                    else hi_i = mid - 1;
                 }
+            }
+            if (out + t < cap) owner_ent[out + t] = found;
+        }
+        out += vol;
     }
 }
 
@@ -214,47 +242,75 @@ struct FoldArgs {
     double* dv_us;
 };
 
-/* calcSolidFraction, src/ibProcessors3D.hh:116-158 (FP64, no contraction,
- * left-to-right sums). */
+/* Sub-cell count of one (offset, radius) query by the whole wavefront:
+ * lane L takes the sample column (i, j) = (L / N, L % N) and tests the N
+ * samples along z; a ballot + popcount per k adds the column counts.  Every
+ * sample test is the reference's FP64 expression
+ * (xs_i + ys_j) + zs_k < r^2 (src/ibProcessors3D.hh:141-155); only the
+ * integer summation order differs, so the count is exact.  The arguments
+ * must be wave-uniform. */
 template <int NT>
-__device__ __forceinline__ double solid_fraction(double dx, double dy, double dz, double rp2,
-                                                 double rm2, double rsq, int Nrt, double sw,
-                                                 double frac, int& n, int& br) {
+__device__ __forceinline__ unsigned wave_count(double dx, double dy, double dz, double rsq, int Nrt,
+                                               double sw) {
     const int N = NT > 0 ? NT : Nrt;
-    const double dist = dx * dx + dy * dy + dz * dz;
-    if (dist > rp2) { n = 0; br = 0; return 0.0; }
-    if (dist < rm2) { n = N * N * N; br = 1; return 1.0; }
-    constexpr int M = NT > 0 ? NT : 16;
-    double xs[M], ys[M], zs[M];
+    const int lane = (int)__lane_id();
+    unsigned total = 0;
+    for (int p0 = 0; p0 < N * N; p0 += 64) {
+        const int pidx = p0 + lane;
+        const bool act = pidx < N * N;
+        const int i = act ? pidx / N : 0;
+        const int j = act ? pidx - i * N : 0;
+        const double ax = dx + (-0.5 + ((double)i + 0.5) * sw);
+        const double ay = dy + (-0.5 + ((double)j + 0.5) * sw);
+        const double sxy = ax * ax + ay * ay;
+        if (NT > 0) {
 #pragma unroll
-    for (int i = 0; i < M; ++i) {
-        if (i < N) {
-            const double delta = -0.5 + ((double)i + 0.5) * sw;
-            const double ax = dx + delta; xs[i] = ax * ax;
-            const double ay = dy + delta; ys[i] = ay * ay;
-            const double az = dz + delta; zs[i] = az * az;
+            for (int k = 0; k < (NT > 0 ? NT : 1); ++k) {
+                const double az = dz + (-0.5 + ((double)k + 0.5) * sw);
+                total += (unsigned)__popcll(__ballot(act && (sxy + az * az < rsq)));
+            }
+        } else {
+            for (int k = 0; k < N; ++k) {
+                const double az = dz + (-0.5 + ((double)k + 0.5) * sw);
+                total += (unsigned)__popcll(__ballot(act && (sxy + az * az < rsq)));
+            }
         }
     }
-    unsigned cnt = 0;
-    if (NT > 0) {
-#pragma unroll
-        for (int i = 0; i < M; ++i)
-#pragma unroll
-            for (int j = 0; j < M; ++j) {
-                const double sxy = xs[i] + ys[j];
-#pragma unroll
-                for (int k = 0; k < M; ++k) cnt += (sxy + zs[k] < rsq) ? 1u : 0u;
-            }
-    } else {
-        for (int i = 0; i < N; ++i)
-            for (int j = 0; j < N; ++j) {
-                const double sxy = xs[i] + ys[j];
-                for (int k = 0; k < N; ++k) cnt += (sxy + zs[k] < rsq) ? 1u : 0u;
-            }
+    return total;
+}
+
+/* calcSolidFraction (src/ibProcessors3D.hh:116-158) for one query per lane,
+ * called by all 64 lanes together: the two distance shortcuts per lane, then
+ * every lane that needs a count gets it from wave_count in turn. */
+template <int NT>
+__device__ __forceinline__ double coop_solid_fraction(bool active, double dx, double dy, double dz,
+                                                      double rp2, double rm2, double rsq, int Nrt,
+                                                      double sw, double frac, int& n, int& br) {
+    const int N = NT > 0 ? NT : Nrt;
+    double sf = 0.0;
+    n = 0;
+    br = 0;
+    bool need = false;
+    if (active) {
+        const double dist = dx * dx + dy * dy + dz * dz;
+        if (dist > rp2) {
+            br = 0;
+        } else if (dist < rm2) {
+            br = 1; n = N * N * N; sf = 1.0;
+        } else {
+            br = 2; need = true;
+        }
     }
-    n = (int)cnt;
-    br = 2;
-    return frac * (double)cnt;
+    unsigned long long pend = __ballot(need);
+    const int lane = (int)__lane_id();
+    while (pend) {
+        const int L = __ffsll(pend) - 1;
+        pend &= pend - 1;
+        const double bx = __shfl(dx, L), by = __shfl(dy, L), bz = __shfl(dz, L), b2 = __shfl(rsq, L);
+        const unsigned cnt = wave_count<NT>(bx, by, bz, b2, N, sw);
+        if (lane == L) { n = (int)cnt; sf = frac * (double)cnt; }
+    }
+    return sf;
 }
 
 enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
@@ -347,20 +403,27 @@ __global__ __launch_bounds__(256) void k_fold(FoldArgs a) {
             cd.e = e;
         }
         __syncthreads();
+        for (int j = 0; j < nchunk; ++j) {
+            const Cand& cd = cs[j];
+            double sfv[2], dxv[2], dyv[2], dzv[2];
+            bool inb[2];
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if (!valid[k]) continue;
-            CellState& s = st[k];
-            for (int j = 0; j < nchunk; ++j) {
-                const Cand& cd = cs[j];
-                if (X[k] < cd.lo[0] || X[k] > cd.hi[0] || Y[k] < cd.lo[1] || Y[k] > cd.hi[1] ||
-                    Z[k] < cd.lo[2] || Z[k] > cd.hi[2])
-                    continue;
-                const double dx = (double)X[k] - cd.x[0];
-                const double dy = (double)Y[k] - cd.x[1];
-                const double dz = (double)Z[k] - cd.x[2];
+            for (int k = 0; k < 2; ++k) {
+                inb[k] = valid[k] && X[k] >= cd.lo[0] && X[k] <= cd.hi[0] && Y[k] >= cd.lo[1] &&
+                         Y[k] <= cd.hi[1] && Z[k] >= cd.lo[2] && Z[k] <= cd.hi[2];
+                dxv[k] = (double)X[k] - cd.x[0];
+                dyv[k] = (double)Y[k] - cd.x[1];
+                dzv[k] = (double)Z[k] - cd.x[2];
                 int nn, br;
-                const double sf = solid_fraction<NT>(dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, a.N, a.sw, a.frac, nn, br);
+                sfv[k] = coop_solid_fraction<NT>(inb[k], dxv[k], dyv[k], dzv[k], cd.rp2, cd.rm2, cd.rsq,
+                                                 a.N, a.sw, a.frac, nn, br);
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                if (!inb[k]) continue;
+                CellState& s = st[k];
+                const double sf = sfv[k];
+                const double dx = dxv[k], dy = dyv[k], dz = dzv[k];
                 const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (s.sf > SOLFRAC_MIN ? 1 : 0);
                 bool set = false, zero = false;
                 if (flag == 0) zero = true;
@@ -481,13 +544,17 @@ __global__ __launch_bounds__(256) void k_subcell(int64_t n, const double* __rest
                                                  double frac, double h, double* __restrict__ sf,
                                                  int32_t* __restrict__ cnt, int32_t* __restrict__ br) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const double rr = r[k];
+    const bool act = k < n;
+    const double rr = act ? r[k] : 1.0;
     const double r_p = rr + h, r_m = rr - h;
+    const double dx = act ? d3[3 * k] : 0.0, dy = act ? d3[3 * k + 1] : 0.0, dz = act ? d3[3 * k + 2] : 0.0;
     int nn, bb;
-    sf[k] = solid_fraction<NT>(d3[3 * k], d3[3 * k + 1], d3[3 * k + 2], r_p * r_p, r_m * r_m, rr * rr, N, sw, frac, nn, bb);
-    cnt[k] = nn;
-    br[k] = bb;
+    const double v = coop_solid_fraction<NT>(act, dx, dy, dz, r_p * r_p, r_m * r_m, rr * rr, N, sw, frac, nn, bb);
+    if (act) {
+        sf[k] = v;
+        cnt[k] = nn;
+        br[k] = bb;
+    }
 }
 
 /* Prescribed kinematics: x += v, wrapped into [-1/2, n - 1/2) on periodic axes. */
@@ -528,7 +595,7 @@ int run_voxelize(lbdem_psm* h) {
     PSM_HIP(h, hipMemsetAsync(h->counters + 2, 0, 6 * sizeof(int64_t), h->stream));
     PSM_HIP(h, hipMemsetAsync(h->bin_cand_count, 0, nbins * sizeof(int32_t), h->stream));
     PSM_HIP(h, hipMemsetAsync(h->bin_fill, 0, nbins * sizeof(int32_t), h->stream));
-    const unsigned pg = (unsigned)std::max<int64_t>(1, (h->np + 255) / 256);
+    const unsigned pg = (unsigned)std::max<int64_t>(1, (h->np + 3) / 4);   /* a wave per particle */
     if (h->np > 0)
         hipLaunchKernelGGL(k_bin_count, dim3(pg), dim3(256), 0, h->stream, va, h->bin_cand_count, h->p_nent);
     size_t tb = h->scan_tmp_bytes;
