@@ -137,6 +137,22 @@ def cpu_baseline(wl, spheres, nsteps):
                       f"sum single-threaded as in the reference; {dt:.1f} s"}
 
 
+def kernel_probe(wl, sph):
+    """Counter-calibration copy of known size, then three coupled steps per
+    precision.  The copy is the FP64 population upload: k_pop_convert<FP64>
+    reads 19*Nc doubles and writes 19*Nc doubles."""
+    n = wl["n"]
+    nc = int(np.prod(n))
+    lat = L.Lattice(*n, periodic=wl["periodic"], tau=wl["tau"], precision="fp64", slices_per_dim=8)
+    lat.set_populations(np.zeros((19, nc)))
+    lat.synchronize()
+    lat.close()
+    log(f"probe copy: {2 * 19 * nc * 8} bytes")
+    for prec in ("fp64", "fp32_dev", "fp16_store"):
+        run_gpu(wl, prec, 3, 1, sph)
+        log(f"probe {prec} done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +161,9 @@ def main():
     ap.add_argument("--workload", default="dense256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-extra", action="store_true", help="skip the 512^3-class FP32 line")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--kernel-probe", action="store_true",
+                    help="short run for hardware-counter profiles: a 1 GiB device copy (calibration) "
+                         "then 3 coupled steps per precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -156,6 +175,9 @@ def main():
     nc = int(np.prod(n))
     log(f"workload {args.workload}: generating {wl['count']} spheres")
     sph = make_spheres(wl)
+    if args.kernel_probe:
+        kernel_probe(wl, sph)
+        return
     res = {}
     for prec in ("fp64", "fp32_dev", "fp16_store"):
         log(f"{prec}: {args.warmup} warmup + {args.steps} timed steps")

@@ -147,7 +147,7 @@ static int ensure_entries(lbdem_psm* h, int64_t need) {
 }
 
 static int read_counters(lbdem_psm* h) {
-    PSM_HIP(h, hipMemcpyAsync(h->h_counters, h->counters, 8 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+    PSM_HIP(h, hipMemcpyAsync(h->h_counters, h->counters, 72 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
     PSM_HIP(h, hipStreamSynchronize(h->stream));
     return LBDEM_PSM_OK;
 }
@@ -274,6 +274,7 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
         h->g.nb[d] = (nn[d] + BIN - 1) / BIN;
     }
     h->g.nc = (int64_t)nn[0] * nn[1] * nn[2];
+    h->g.nzp = ((int64_t)nn[2] + 63) / 64 * 64;
     h->nbins = (int64_t)h->g.nb[0] * h->g.nb[1] * h->g.nb[2];
     h->omega = 1.0 / desc->tau;
     h->ooo = 1.0 / h->omega - 0.5;          /* src/ibCompositeDynamics3D.hh:178 */
@@ -281,10 +282,12 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
     h->has_force = desc->body_accel[0] != 0.0 || desc->body_accel[1] != 0.0 || desc->body_accel[2] != 0.0;
     h->precision = desc->precision;
     const size_t S = h->precision == LBDEM_PSM_FP64 ? 8 : (h->precision == LBDEM_PSM_FP32_DEV ? 4 : 2);
-    h->pop_bytes = (size_t)Q * h->g.nc * S;
+    h->pop_bytes = (size_t)Q * (size_t)nn[0] * (size_t)nn[1] * (size_t)h->g.nzp * S;
     h->rec_cap = desc->max_solid_cells > 0
                      ? desc->max_solid_cells
                      : std::min<int64_t>(h->g.nc, std::max<int64_t>(h->g.nc / 2, 1 << 20));
+    /* records are allocated per workgroup from nseg segments of the buffer */
+    h->nseg = (int)std::max<int64_t>(1, std::min<int64_t>(64, h->rec_cap / 8192));
 
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { rc = hip_check(nullptr, e, "hipStreamCreate"); delete h; return rc; }
@@ -310,9 +313,9 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
     CREATE_CHECK(dalloc(&h->bin_fill, h->nbins * sizeof(int32_t)));
     CREATE_CHECK(hipMemset(h->bin_cand_count, 0, h->nbins * sizeof(int32_t)));
     CREATE_CHECK(hipMemset(h->bin_cand_start, 0, h->nbins * sizeof(int32_t)));
-    CREATE_CHECK(dalloc(&h->counters, 8 * sizeof(int64_t)));
-    CREATE_CHECK(hipMemset(h->counters, 0, 8 * sizeof(int64_t)));
-    CREATE_CHECK(hipHostMalloc((void**)&h->h_counters, 8 * sizeof(int64_t), hipHostMallocDefault));
+    CREATE_CHECK(dalloc(&h->counters, 72 * sizeof(int64_t)));
+    CREATE_CHECK(hipMemset(h->counters, 0, 72 * sizeof(int64_t)));
+    CREATE_CHECK(hipHostMalloc((void**)&h->h_counters, 72 * sizeof(int64_t), hipHostMallocDefault));
     CREATE_CHECK(dalloc(&h->red_tmp, (6 * 4096 + 8) * sizeof(double)));
     for (int k = 0; k < 2; ++k) {
         rc = alloc_records(h, h->rec[k]);
@@ -378,10 +381,6 @@ int lbdem_psm_set_populations(lbdem_psm* h, const double* f) {
     if (!h || !f) return LBDEM_PSM_ERR_INVALID_ARG;
     const size_t bytes = (size_t)Q * h->g.nc * sizeof(double);
     PSM_HIP(h, hipStreamSynchronize(h->stream));
-    if (h->precision == LBDEM_PSM_FP64) {
-        PSM_HIP(h, hipMemcpy(h->f[h->cur_f], f, bytes, hipMemcpyHostToDevice));
-        return LBDEM_PSM_OK;
-    }
     double* tmp = nullptr;
     PSM_HIP(h, dalloc(&tmp, bytes));
     PSM_HIP(h, hipMemcpy(tmp, f, bytes, hipMemcpyHostToDevice));
@@ -397,10 +396,6 @@ int lbdem_psm_get_populations(lbdem_psm* h, double* f) {
     if (!h || !f) return LBDEM_PSM_ERR_INVALID_ARG;
     const size_t bytes = (size_t)Q * h->g.nc * sizeof(double);
     PSM_HIP(h, hipStreamSynchronize(h->stream));
-    if (h->precision == LBDEM_PSM_FP64) {
-        PSM_HIP(h, hipMemcpy(f, h->f[h->cur_f], bytes, hipMemcpyDeviceToHost));
-        return LBDEM_PSM_OK;
-    }
     double* tmp = nullptr;
     PSM_HIP(h, dalloc(&tmp, bytes));
     int rc = launch_pop_convert(h, tmp, false);
@@ -619,6 +614,7 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
     if (h->p_cap > 0) PSM_HIP(h, hipMemset(h->p_nent, 0, h->p_cap * sizeof(int32_t)));
     int64_t hc[8] = {m, 0, m, 0, 0, 0, 0, 0};
     PSM_HIP(h, hipMemcpy(h->counters, hc, sizeof(hc), hipMemcpyHostToDevice));
+    h->records_from_host = true;
     return LBDEM_PSM_OK;
 }
 
@@ -673,7 +669,9 @@ int lbdem_psm_stats(lbdem_psm* h, int64_t* out4) {
         PSM_HIP(h, hipMemcpy(&last_start, h->bin_cand_start + h->nbins - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
         PSM_HIP(h, hipMemcpy(&last_count, h->bin_cand_count + h->nbins - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
     }
-    out4[0] = h->h_counters[0];
+    int64_t nrec = 0;
+    for (int k = 0; k < h->nseg; ++k) nrec += h->h_counters[8 + k];
+    out4[0] = h->records_from_host ? h->h_counters[0] : nrec;
     out4[1] = (int64_t)last_start + last_count;
     out4[2] = h->h_counters[4];
     out4[3] = h->h_counters[2];

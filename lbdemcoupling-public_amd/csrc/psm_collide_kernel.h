@@ -1,0 +1,327 @@
+/*
+ * Fused PSM collide + stream kernel template, shared by the FP64 translation
+ * unit (psm_collide.hip, compiled without contraction: bit-exact with the
+ * FP64 restatement) and the FP32 / FP16-storage unit (psm_collide_lp.hip,
+ * compiled with FMA contraction and the hardware reciprocal).
+ */
+#ifndef LBDEM_PSM_COLLIDE_KERNEL_H
+#define LBDEM_PSM_COLLIDE_KERNEL_H
+
+#include "psm_common.h"
+
+namespace psm {
+
+/* 1/x: exact division in FP64, the hardware reciprocal (1 ulp) in FP32. */
+template <typename T> __device__ __forceinline__ T recip(T x);
+template <> __device__ __forceinline__ double recip<double>(double x) { return 1.0 / x; }
+template <> __device__ __forceinline__ float recip<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+
+struct CollideArgs {
+    int nx, ny, nz;
+    int px, py, pz;
+    int64_t nc;
+    int64_t nzp;
+    double omega, ooo;
+    double acc[3];
+    const int32_t* slot;
+    const double* rsf;
+    const double* rus;
+    double* rF;
+    int64_t rcap;
+};
+
+/* Equilibria of a population pair (i, i+9): c_{i+9} = -c_i, so the pair
+ * shares c.j, (3 c.j)(c.j) - |j|^2 and its product with 1.5/rho; each value
+ * is bit-identical to equilibrium(i, ...) / equilibrium(i+9, ...). */
+template <typename T>
+__device__ __forceinline__ void eq_pair(int i, T rhoBar, T p, const T* j, T jSqr, T& ei, T& eo) {
+    const T cj = cdot<T>(i, j);
+    const T a = T(3) * cj;
+    const T b = a * cj;
+    const T q = b - jSqr;
+    const T pq = p * q;
+    ei = wgt<T>(i) * ((rhoBar + a) + pq);
+    eo = wgt<T>(i) * ((rhoBar - a) + pq);
+}
+
+template <typename T>
+__device__ __forceinline__ T eq_rest(T rhoBar, T p, T jSqr) {
+    return equilibrium<T>(0, rhoBar, p, nullptr, jSqr);
+}
+
+/* BGK (with Guo forcing when FORCE) of all 19 populations of one cell, in
+ * the operation order of the base dynamics. */
+template <typename T, bool FORCE>
+__device__ __forceinline__ void bgk_all(const T* f, T* fb, T omega, const T* acc) {
+    T rhoBar, j[3];
+    moments<T>(f, rhoBar, j);
+    const T rho = rhoBar + T(1);
+    const T invRho = recip<T>(rho);
+    T jEq[3] = {j[0], j[1], j[2]};
+    T u[3] = {T(0), T(0), T(0)};
+    if (FORCE) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            u[d] = j[d] * invRho + T(0.5) * acc[d];
+            jEq[d] = rho * u[d];
+        }
+    }
+    const T jSqr = norm_sqr<T>(jEq);
+    const T p = T(1.5) * invRho;
+    const T omm = T(1) - omega;
+    fb[0] = f[0] * omm + omega * eq_rest<T>(rhoBar, p, jSqr);
+#pragma unroll
+    for (int i = 1; i <= 9; ++i) {
+        T ei, eo;
+        eq_pair<T>(i, rhoBar, p, jEq, jSqr, ei, eo);
+        fb[i] = f[i] * omm + omega * ei;
+        fb[i + 9] = f[i + 9] * omm + omega * eo;
+    }
+    if (FORCE) {
+        const T pref = (T(1) - T(0.5) * omega) * rho;
+        const T ua = (u[0] * acc[0] + u[1] * acc[1]) + u[2] * acc[2];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const T ca = cdot<T>(i, acc);
+            const T cu = cdot<T>(i, u);
+            const T g = T(3) * (ca - ua) + T(9) * (cu * ca);
+            fb[i] = fb[i] + (pref * wgt<T>(i)) * g;
+        }
+    }
+}
+
+/* One workgroup per z-row (x, y uniform): per direction the destination
+ * row block and the x/y wall test are uniform, only z varies per lane. */
+template <typename S, bool PZ>
+struct Pusher {
+    S* fout;
+    int64_t nzp;
+    int x, y, nx, ny, nz, px, py;
+    __device__ __forceinline__ void put(int i, int z, S v) const {
+        int xd = x + cx(i), yd = y + cy(i);
+        bool wall = false;
+        if (xd < 0 || xd >= nx) { if (px) xd = xd < 0 ? xd + nx : xd - nx; else wall = true; }
+        if (yd < 0 || yd >= ny) { if (py) yd = yd < 0 ? yd + ny : yd - ny; else wall = true; }
+        int zd = z + cz(i);
+        bool zwall = false;
+        if (cz(i) != 0) {
+            if (PZ) zd = zd < 0 ? zd + nz : (zd >= nz ? zd - nz : zd);
+            else zwall = zd < 0 || zd >= nz;
+        }
+        if (wall || zwall) {
+            char* own = (char*)(fout + pop_at((int64_t)x * ny + y, opp(i), 0, nzp));
+            *(S*)(own + (uint32_t)z * (uint32_t)sizeof(S)) = v;
+        } else {
+            char* dst = (char*)(fout + pop_at((int64_t)xd * ny + yd, i, 0, nzp));
+            *(S*)(dst + (uint32_t)zd * (uint32_t)sizeof(S)) = v;
+        }
+    }
+};
+
+/* FP32 arithmetic of the same collision, restructured for FMAs: with
+ * K1 = 3 W, K2 = 3 p W and base = W (rhoBar - p |j|^2) per weight class,
+ * eq_i = base + cj (K1 + K2 cj) and eq_{i+9} = base + cj (K2 cj - K1). */
+struct EqCoef {
+    float base1, k1a, k1b, base2, k2a, k2b, base0;
+    __device__ __forceinline__ void init(float scale, float rhoBar, float p, float jSqr) {
+        const float w0 = 1.0f / 3.0f, w1 = 1.0f / 18.0f, w2 = 1.0f / 36.0f;
+        const float r0 = rhoBar - p * jSqr;
+        base0 = scale * w0 * r0;
+        base1 = scale * w1 * r0;
+        base2 = scale * w2 * r0;
+        k1a = scale * 3.0f * w1;
+        k2a = scale * 3.0f * w2;
+        k1b = k1a * p;
+        k2b = k2a * p;
+    }
+    __device__ __forceinline__ void pair(int i, float cj, float& ei, float& eo) const {
+        const bool c1 = (i <= 3);
+        const float base = c1 ? base1 : base2, ka = c1 ? k1a : k2a, kb = c1 ? k1b : k2b;
+        ei = fmaf(cj, fmaf(kb, cj, ka), base);
+        eo = fmaf(cj, fmaf(kb, cj, -ka), base);
+    }
+};
+
+template <bool FORCE>
+__device__ __forceinline__ void bgk_all_fast(const float* f, float* fb, float omega, const float* acc) {
+    float rhoBar, j[3];
+    moments<float>(f, rhoBar, j);
+    const float rho = rhoBar + 1.0f;
+    const float invRho = recip<float>(rho);
+    float jEq[3] = {j[0], j[1], j[2]};
+    float u[3] = {0.0f, 0.0f, 0.0f};
+    if (FORCE) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            u[d] = j[d] * invRho + 0.5f * acc[d];
+            jEq[d] = rho * u[d];
+        }
+    }
+    const float jSqr = norm_sqr<float>(jEq);
+    const float omm = 1.0f - omega;
+    EqCoef ec;
+    ec.init(omega, rhoBar, 1.5f * invRho, jSqr);
+    fb[0] = fmaf(f[0], omm, ec.base0);
+#pragma unroll
+    for (int i = 1; i <= 9; ++i) {
+        float ei, eo;
+        ec.pair(i, cdot<float>(i, jEq), ei, eo);
+        fb[i] = fmaf(f[i], omm, ei);
+        fb[i + 9] = fmaf(f[i + 9], omm, eo);
+    }
+    if (FORCE) {
+        const float pref = (1.0f - 0.5f * omega) * rho;
+        const float ua = (u[0] * acc[0] + u[1] * acc[1]) + u[2] * acc[2];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const float ca = cdot<float>(i, acc);
+            const float cu = cdot<float>(i, u);
+            const float g = 3.0f * (ca - ua) + 9.0f * (cu * ca);
+            fb[i] = fb[i] + (pref * wgt<float>(i)) * g;
+        }
+    }
+}
+
+/* The solid blend in FP32 (same structure as the FP64 path). */
+__device__ __forceinline__ void psm_blend_fast(const float* fp, float* fo, bool full, float B,
+                                               const float* us, float* F) {
+    const float oneMinB = 1.0f - B;
+    float post[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) post[i] = full ? fp[i] : fo[i];
+    float rhoB, jB[3];
+    moments<float>(post, rhoB, jB);
+    const float invRho = recip<float>(rhoB + 1.0f);
+    const float jSqr = norm_sqr<float>(jB);
+    const float p = 1.5f * invRho;
+    const float onePlus = 1.0f + rhoB;
+    const float uP[3] = {us[0] * onePlus, us[1] * onePlus, us[2] * onePlus};
+    const float uPSqr = norm_sqr<float>(uP);
+    EqCoef ce, cs;
+    ce.init(1.0f, rhoB, p, jSqr);
+    cs.init(1.0f, rhoB, p, uPSqr);
+    F[0] = F[1] = F[2] = 0.0f;
+    fo[0] = fmaf(oneMinB, post[0] - fp[0], fp[0]) + B * (ce.base0 - cs.base0);
+#pragma unroll
+    for (int i = 1; i <= 9; ++i) {
+        const int o = i + 9;
+        float Ei, Eo, Si, So;
+        ce.pair(i, cdot<float>(i, jB), Ei, Eo);
+        cs.pair(i, cdot<float>(i, uP), Si, So);
+        const float bb = B * ((fp[o] - Eo) - (fp[i] - Si));
+        const float bbo = B * ((fp[i] - Ei) - (fp[o] - So));
+        fo[i] = fmaf(oneMinB, post[i] - fp[i], fp[i]) + bb;
+        fo[o] = fmaf(oneMinB, post[o] - fp[o], fp[o]) + bbo;
+        const float diff = bb - bbo;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (cc(i, d) == 1) F[d] = F[d] - diff;
+            else if (cc(i, d) == -1) F[d] = F[d] + diff;
+        }
+    }
+}
+
+/* One lane per cell: blockIdx.x = row (x*ny + y), blockIdx.y = z chunk. */
+template <int PREC, bool FORCE, bool PZ>
+__global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PREC>::S* __restrict__ fin,
+                                                        typename Store<PREC>::S* __restrict__ fout,
+                                                        CollideArgs a) {
+    typedef typename Store<PREC>::T T;
+    typedef typename Store<PREC>::S S;
+    const int row = blockIdx.x;
+    const int z = blockIdx.y * blockDim.x + threadIdx.x;
+    if (z >= a.nz) return;
+    /* x, y are wave-uniform: keep them (and the row-block addresses derived
+     * from them) in scalar registers */
+    const int x = __builtin_amdgcn_readfirstlane(row / a.ny);
+    const int y = __builtin_amdgcn_readfirstlane(row - x * a.ny);
+    const T omega = T(a.omega);
+    const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
+    Pusher<S, PZ> push{fout, a.nzp, x, y, a.nx, a.ny, a.nz, a.px, a.py};
+    const char* frow = (const char*)(fin + pop_at(row, 0, 0, a.nzp));
+    const size_t qstride = (size_t)a.nzp * sizeof(S);
+
+    const uint32_t c = (uint32_t)row * (uint32_t)a.nz + (uint32_t)z;
+    const int s = a.slot[c];
+    const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(S);
+    T fp[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fp[i] = (T)(*(const S*)(frow + i * qstride + zoff));
+    T fo[Q];
+    if constexpr (sizeof(T) == 4) {
+        bgk_all_fast<FORCE>(fp, fo, omega, acc);
+        if (s >= 0) {
+            const float sf = (float)a.rsf[s];
+            const bool full = a.rsf[s] > SOLFRAC_MAX;
+            const float ooo = (float)a.ooo;
+            const float B = full ? 1.0f : (sf * ooo) / ((1.0f - sf) + ooo);
+            const float us[3] = {(float)a.rus[s], (float)a.rus[a.rcap + s], (float)a.rus[2 * a.rcap + s]};
+            float F[3];
+            psm_blend_fast(fp, fo, full, B, us, F);
+            a.rF[s] = double(F[0]);
+            a.rF[a.rcap + s] = double(F[1]);
+            a.rF[2 * a.rcap + s] = double(F[2]);
+        }
+    } else {
+        bgk_all<T, FORCE>(fp, fo, omega, acc);
+        if (s >= 0) {
+            /* IBcompositeDynamics::collide (src/ibCompositeDynamics3D.hh:130-203).
+             * sf > 0.999 runs as B = 1 on the pre-collision cell: with B = 1
+             * and fB = fP the blend (fP + (1-B)(fB - fP)) + B*Omega reduces
+             * exactly to fP + Omega, the reference's full-solid branch. */
+            const T sf = T(a.rsf[s]);
+            const bool full = a.rsf[s] > SOLFRAC_MAX;
+            const T ooo = T(a.ooo);
+            const T B = full ? T(1) : (sf * ooo) / ((T(1) - sf) + ooo);
+            const T oneMinB = T(1) - B;
+            const T us[3] = {T(a.rus[s]), T(a.rus[a.rcap + s]), T(a.rus[2 * a.rcap + s])};
+            T post[Q];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) post[i] = full ? fp[i] : fo[i];
+            T rhoB, jB[3];
+            moments<T>(post, rhoB, jB);
+            const T invRho = recip<T>(rhoB + T(1));
+            const T jSqr = norm_sqr<T>(jB);
+            const T p = T(1.5) * invRho;
+            const T onePlus = T(1) + rhoB;
+            T uP[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) uP[d] = us[d] * onePlus;
+            const T uPSqr = norm_sqr<T>(uP);
+            T F[3] = {T(0), T(0), T(0)};
+            {
+                const T bb0 = eq_rest<T>(rhoB, p, jSqr) - eq_rest<T>(rhoB, p, uPSqr);
+                fo[0] = (fp[0] + oneMinB * (post[0] - fp[0])) + B * bb0;
+            }
+#pragma unroll
+            for (int i = 1; i <= 9; ++i) {
+                const int o = i + 9;
+                T Ei, Eo, Si, So;
+                eq_pair<T>(i, rhoB, p, jB, jSqr, Ei, Eo);
+                eq_pair<T>(i, rhoB, p, uP, uPSqr, Si, So);
+                const T bb = B * ((fp[o] - Eo) - (fp[i] - Si));
+                const T bbo = B * ((fp[i] - Ei) - (fp[o] - So));
+                fo[i] = (fp[i] + oneMinB * (post[i] - fp[i])) + bb;
+                fo[o] = (fp[o] + oneMinB * (post[o] - fp[o])) + bbo;
+                const T diff = bb - bbo;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (cc(i, d) == 1) F[d] = F[d] - diff;
+                    else if (cc(i, d) == -1) F[d] = F[d] + diff;
+                }
+            }
+            a.rF[s] = double(F[0]);
+            a.rF[a.rcap + s] = double(F[1]);
+            a.rF[2 * a.rcap + s] = double(F[2]);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) push.put(i, z, (S)fo[i]);
+}
+
+template <int PREC>
+void launch_cs_prec(lbdem_psm* h, const CollideArgs& a);
+
+}  // namespace psm
+
+#endif

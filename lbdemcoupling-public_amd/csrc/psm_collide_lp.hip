@@ -1,0 +1,31 @@
+/*
+ * FP32-deviation and FP16-storage instantiations of the fused PSM
+ * collide + stream kernel.  This unit is compiled with FMA contraction
+ * (-ffp-contract=fast) and uses the hardware reciprocal: these variants are
+ * checked against FP64 by error bounds, not bit for bit.
+ */
+#include "psm_collide_kernel.h"
+
+namespace psm {
+
+template <int PREC>
+static void launch_lp(lbdem_psm* h, const CollideArgs& a) {
+    typedef typename Store<PREC>::S S;
+    const S* fin = (const S*)h->f[h->cur_f];
+    S* fout = (S*)h->f[1 - h->cur_f];
+    dim3 grid((unsigned)(a.nx * a.ny), (unsigned)((a.nz + 255) / 256)), block(256);
+    if (h->has_force) {
+        if (a.pz) hipLaunchKernelGGL((k_collide_stream<PREC, true, true>), grid, block, 0, h->stream, fin, fout, a);
+        else hipLaunchKernelGGL((k_collide_stream<PREC, true, false>), grid, block, 0, h->stream, fin, fout, a);
+    } else {
+        if (a.pz) hipLaunchKernelGGL((k_collide_stream<PREC, false, true>), grid, block, 0, h->stream, fin, fout, a);
+        else hipLaunchKernelGGL((k_collide_stream<PREC, false, false>), grid, block, 0, h->stream, fin, fout, a);
+    }
+}
+
+template <>
+void launch_cs_prec<LBDEM_PSM_FP32_DEV>(lbdem_psm* h, const CollideArgs& a) { launch_lp<LBDEM_PSM_FP32_DEV>(h, a); }
+template <>
+void launch_cs_prec<LBDEM_PSM_FP16_STORE>(lbdem_psm* h, const CollideArgs& a) { launch_lp<LBDEM_PSM_FP16_STORE>(h, a); }
+
+}  // namespace psm

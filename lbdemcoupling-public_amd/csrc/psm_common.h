@@ -70,6 +70,7 @@ __device__ __forceinline__ void moments(const T* f, T& rhoBar, T* j) {
 
 template <typename T>
 __device__ __forceinline__ T cdot(int i, const T* v) {
+    if (i == 0) return T(0);
     T s = T(0);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -228,7 +229,16 @@ struct Geom {
     int periodic[3];
     int nb[3];       /* bins per axis */
     int64_t nc;
+    int64_t nzp;     /* z stride of the population store (nz padded to 64) */
 };
+
+/* Population store layout, row-blocked structure of arrays: for row
+ * r = x*ny + y the 19 populations are consecutive z-lines,
+ * f[(r*19 + q)*nzp + z].  A workgroup working on one row touches only its
+ * own row block and the 8 neighbouring ones. */
+__host__ __device__ inline size_t pop_at(int64_t row, int q, int z, int64_t nzp) {
+    return ((size_t)row * Q + (size_t)q) * (size_t)nzp + (size_t)z;
+}
 
 }  // namespace psm
 
@@ -239,7 +249,7 @@ struct lbdem_psm {
     double omega, ooo, sqrt3half;
     bool has_force;
     int precision;
-    size_t pop_bytes;          /* bytes of one population buffer */
+    size_t pop_bytes;          /* bytes of one population buffer (row-blocked layout) */
     hipStream_t stream = nullptr;
     void* f[2] = {nullptr, nullptr};
     int cur_f = 0;
@@ -260,6 +270,7 @@ struct lbdem_psm {
     int32_t flags = 0;
     bool have_particles = false;
     bool voxelized = false;
+    bool records_from_host = false;   /* records last set by set_cell_records */
 
     /* binning */
     int32_t *bin_cand_count = nullptr, *bin_cand_start = nullptr, *bin_fill = nullptr;
@@ -273,8 +284,10 @@ struct lbdem_psm {
     double* psum_x = nullptr;        /* positions for torque arms */
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
-    int64_t* counters = nullptr;     /* device: [0] records, [1] overflow flags, [2] orphans,
-                                        [3] entries, [4] images, [5] defineVelocity cells */
+    int64_t* counters = nullptr;     /* device: [0] unused, [1] overflow flags, [2] orphans,
+                                        [3] unused, [4] images, [5] defineVelocity cells,
+                                        [8 .. 8+nseg) record-allocation segment counters */
+    int nseg = 1;                    /* record-allocation segments */
     int64_t* h_counters = nullptr;   /* pinned host mirror */
     double* red_tmp = nullptr;       /* reduction scratch */
     int32_t* dv_cell = nullptr;      /* defineVelocity list (initVel), lazily allocated */

@@ -240,6 +240,9 @@ struct FoldArgs {
     int32_t* orphan;
     int32_t* dv_cell;
     double* dv_us;
+    int64_t* seg_count;   /* nseg record-allocation counters */
+    int nseg;
+    int64_t seg;          /* records per segment */
 };
 
 /* Sub-cell count of one (offset, radius) query by the whole wavefront:
@@ -315,61 +318,42 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
 
 enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
 
-struct CellState {
-    int kind;
-    double sf;
-    int id;
-    double us[3];
-    int e;
-    int oldslot;
-    bool dv;
-    double dvus[3];
-};
-
-template <int NT>
-__global__ __launch_bounds__(256) void k_fold(FoldArgs a) {
+/* One workgroup of 512 lanes per 8^3 bin, one cell per lane (a wavefront
+ * is one x-slice of the bin).  The cell's record of the previous step is
+ * folded with the bin's candidates in key order through the four-case rule
+ * of SetSingleSphere3D::process (src/ibProcessors3D.hh:85-105). */
+template <int NT, bool INITVEL>
+__global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     __shared__ Cand cs[64];
-    __shared__ int scan[256];
+    __shared__ int wave_tot[8];
     __shared__ long long s_base;
 
     const int64_t b = blockIdx.x;
-    const int bz = (int)(b % a.nb[2]);
-    const int by = (int)((b / a.nb[2]) % a.nb[1]);
-    const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
     const int ncand = a.bin_cand_count[b];
-    const int cstart = a.bin_cand_start[b];
     const int old_cnt = a.oldr.bin_count[b];
     if (ncand == 0 && old_cnt == 0) {
         if (threadIdx.x == 0) { a.newr.bin_count[b] = 0; a.newr.bin_start[b] = 0; }
         return;
     }
+    const int cstart = a.bin_cand_start[b];
     if ((int64_t)cstart + ncand > a.ent_cap) return;   /* overflow already flagged */
+    const int bz = (int)(b % a.nb[2]);
+    const int by = (int)((b / a.nb[2]) % a.nb[1]);
+    const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
+    const int l = threadIdx.x;
+    const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
+    const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
+    const int64_t cidx = valid ? ((int64_t)X * a.n[1] + Y) * a.n[2] + Z : 0;
 
-    CellState st[2];
-    int X[2], Y[2], Z[2];
-    bool valid[2];
-    int64_t cidx[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        const int l = 2 * threadIdx.x + k;
-        X[k] = bx * BIN + (l >> 6);
-        Y[k] = by * BIN + ((l >> 3) & 7);
-        Z[k] = bz * BIN + (l & 7);
-        valid[k] = X[k] < a.n[0] && Y[k] < a.n[1] && Z[k] < a.n[2];
-        cidx[k] = valid[k] ? ((int64_t)X[k] * a.n[1] + Y[k]) * a.n[2] + Z[k] : 0;
-        st[k].dv = false;
-        st[k].e = -1;
-        st[k].oldslot = valid[k] ? a.slot[cidx[k]] : -1;
-        if (st[k].oldslot >= 0) {
-            st[k].kind = ST_OLD;
-            st[k].sf = a.oldr.sf[st[k].oldslot];
-            st[k].id = a.oldr.id[st[k].oldslot];
-        } else {
-            st[k].kind = ST_ZERO;
-            st[k].sf = 0.0;
-            st[k].id = 0;
-        }
-        st[k].us[0] = st[k].us[1] = st[k].us[2] = 0.0;
+    const int oldslot = valid ? a.slot[cidx] : -1;
+    int kind = ST_ZERO, sid = 0, se = -1;
+    double ssf = 0.0, sus[3] = {0.0, 0.0, 0.0};
+    bool dv = false;
+    double dvus[3] = {0.0, 0.0, 0.0};
+    if (oldslot >= 0) {
+        kind = ST_OLD;
+        ssf = a.oldr.sf[oldslot];
+        sid = a.oldr.id[oldslot];
     }
 
     for (int c0 = 0; c0 < ncand; c0 += 64) {
@@ -388,7 +372,10 @@ __global__ __launch_bounds__(256) void k_fold(FoldArgs a) {
             va.auto_images = a.auto_images;
             double xi[3];
             int lo[3], hi[3];
-            if (!image_box(x, r, rank, va, lo, hi, xi)) { lo[0] = 1; hi[0] = 0; lo[1] = lo[2] = 1; hi[1] = hi[2] = 0; xi[0] = xi[1] = xi[2] = 0.0; }
+            if (!image_box(x, r, rank, va, lo, hi, xi)) {
+                lo[0] = lo[1] = lo[2] = 1; hi[0] = hi[1] = hi[2] = 0;
+                xi[0] = xi[1] = xi[2] = 0.0;
+            }
             for (int d = 0; d < 3; ++d) {
                 cd.x[d] = xi[d]; cd.lo[d] = lo[d]; cd.hi[d] = hi[d];
                 cd.v[d] = a.pv[3 * q + d]; cd.w[d] = a.pw[3 * q + d];
@@ -405,81 +392,73 @@ __global__ __launch_bounds__(256) void k_fold(FoldArgs a) {
         __syncthreads();
         for (int j = 0; j < nchunk; ++j) {
             const Cand& cd = cs[j];
-            double sfv[2], dxv[2], dyv[2], dzv[2];
-            bool inb[2];
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                inb[k] = valid[k] && X[k] >= cd.lo[0] && X[k] <= cd.hi[0] && Y[k] >= cd.lo[1] &&
-                         Y[k] <= cd.hi[1] && Z[k] >= cd.lo[2] && Z[k] <= cd.hi[2];
-                dxv[k] = (double)X[k] - cd.x[0];
-                dyv[k] = (double)Y[k] - cd.x[1];
-                dzv[k] = (double)Z[k] - cd.x[2];
-                int nn, br;
-                sfv[k] = coop_solid_fraction<NT>(inb[k], dxv[k], dyv[k], dzv[k], cd.rp2, cd.rm2, cd.rsq,
-                                                 a.N, a.sw, a.frac, nn, br);
+            const bool inb = valid && X >= cd.lo[0] && X <= cd.hi[0] && Y >= cd.lo[1] && Y <= cd.hi[1] &&
+                             Z >= cd.lo[2] && Z <= cd.hi[2];
+            const double dx = (double)X - cd.x[0];
+            const double dy = (double)Y - cd.x[1];
+            const double dz = (double)Z - cd.x[2];
+            int nn, br;
+            const double sf = coop_solid_fraction<NT>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, a.N, a.sw,
+                                                      a.frac, nn, br);
+            if (!inb) continue;
+            const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (ssf > SOLFRAC_MIN ? 1 : 0);
+            bool set = false, zero = false;
+            if (flag == 0) zero = true;
+            else if (flag == 1) set = true;
+            else if (flag == 2) zero = (sid == cd.id);
+            else set = (sf > ssf || sid == cd.id);
+            if (zero) {
+                kind = ST_ZERO; ssf = 0.0; sid = 0; se = -1;
+                sus[0] = sus[1] = sus[2] = 0.0;
             }
-#pragma unroll
-            for (int k = 0; k < 2; ++k) {
-                if (!inb[k]) continue;
-                CellState& s = st[k];
-                const double sf = sfv[k];
-                const double dx = dxv[k], dy = dyv[k], dz = dzv[k];
-                const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (s.sf > SOLFRAC_MIN ? 1 : 0);
-                bool set = false, zero = false;
-                if (flag == 0) zero = true;
-                else if (flag == 1) set = true;
-                else if (flag == 2) zero = (s.id == cd.id);
-                else set = (sf > s.sf || s.id == cd.id);
-                if (zero) {
-                    s.kind = ST_ZERO; s.sf = 0.0; s.id = 0; s.e = -1;
-                    s.us[0] = s.us[1] = s.us[2] = 0.0;
-                }
-                if (set) {   /* setValues, src/ibProcessors3D.hh:162-174 */
-                    s.kind = ST_SET;
-                    s.sf = sf;
-                    s.id = cd.id;
-                    s.e = cd.e;
-                    s.us[0] = cd.v[0] + (cd.w[1] * dz - cd.w[2] * dy);
-                    s.us[1] = cd.v[1] + (-cd.w[0] * dz + cd.w[2] * dx);
-                    s.us[2] = cd.v[2] + (cd.w[0] * dy - cd.w[1] * dx);
-                }
-                if (a.init_vel && sf > SOLFRAC_MAX) {
-                    s.dv = true;
-                    if (s.kind == ST_OLD) {
-                        const int os = s.oldslot;
-                        s.dvus[0] = a.oldr.us[os];
-                        s.dvus[1] = a.oldr.us[a.rcap + os];
-                        s.dvus[2] = a.oldr.us[2 * a.rcap + os];
-                    } else {
-                        s.dvus[0] = s.us[0]; s.dvus[1] = s.us[1]; s.dvus[2] = s.us[2];
-                    }
+            if (set) {   /* setValues, src/ibProcessors3D.hh:162-174 */
+                kind = ST_SET;
+                ssf = sf;
+                sid = cd.id;
+                se = cd.e;
+                sus[0] = cd.v[0] + (cd.w[1] * dz - cd.w[2] * dy);
+                sus[1] = cd.v[1] + (-cd.w[0] * dz + cd.w[2] * dx);
+                sus[2] = cd.v[2] + (cd.w[0] * dy - cd.w[1] * dx);
+            }
+            if (INITVEL && sf > SOLFRAC_MAX) {
+                dv = true;
+                if (kind == ST_OLD) {
+                    dvus[0] = a.oldr.us[oldslot];
+                    dvus[1] = a.oldr.us[a.rcap + oldslot];
+                    dvus[2] = a.oldr.us[2 * a.rcap + oldslot];
+                } else {
+                    dvus[0] = sus[0]; dvus[1] = sus[1]; dvus[2] = sus[2];
                 }
             }
         }
     }
 
-    /* compaction in cell order inside the bin */
-    int fl[2];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) fl[k] = (valid[k] && st[k].kind != ST_ZERO) ? 1 : 0;
-    const int mine = fl[0] + fl[1];
+    /* compaction in cell order inside the bin: wave ballots + 8 wave totals */
+    const bool keep = valid && kind != ST_ZERO;
+    const unsigned long long m = __ballot(keep);
+    const int wave = threadIdx.x >> 6;
+    const int lane = (int)__lane_id();
+    const int before = (int)__popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = (int)__popcll(m);
     __syncthreads();
-    scan[threadIdx.x] = mine;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const int v = (int)threadIdx.x >= off ? scan[threadIdx.x - off] : 0;
-        __syncthreads();
-        scan[threadIdx.x] += v;
-        __syncthreads();
+    int off = before, total = 0;
+    for (int w = 0; w < 8; ++w) {
+        const int t = wave_tot[w];
+        if (w < wave) off += t;
+        total += t;
     }
-    const int total = scan[255];
-    const int excl = scan[threadIdx.x] - mine;
     if (threadIdx.x == 0) {
+        /* records are allocated from one of nseg segment counters, so that
+         * the workgroups do not all serialize on one address */
         long long base = 0;
-        if (total > 0) base = (long long)atomicAdd((unsigned long long*)&a.counters[0], (unsigned long long)total);
-        if (base + total > a.rcap) {
-            atomicOr((unsigned long long*)&a.counters[1], 1ull);
-            base = -1;
+        if (total > 0) {
+            const int sg = (int)(b % a.nseg);
+            const long long o = (long long)atomicAdd((unsigned long long*)&a.seg_count[sg], (unsigned long long)total);
+            base = (long long)sg * a.seg + o;
+            if (o + total > a.seg) {
+                atomicOr((unsigned long long*)&a.counters[1], 1ull);
+                base = -1;
+            }
         }
         s_base = base;
         a.newr.bin_start[b] = base < 0 ? 0 : (int32_t)base;
@@ -487,53 +466,34 @@ __global__ __launch_bounds__(256) void k_fold(FoldArgs a) {
     }
     __syncthreads();
     const long long base = s_base;
-    int off = excl;
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        if (!valid[k]) continue;
-        CellState& s = st[k];
-        if (!fl[k] || base < 0) {
-            a.slot[cidx[k]] = -1;
-            continue;
-        }
+    if (!valid) return;
+    if (!keep || base < 0) {
+        a.slot[cidx] = -1;
+    } else {
         const int64_t idx = base + off;
-        ++off;
-        if (s.kind == ST_OLD) {
-            const int os = s.oldslot;
-            s.us[0] = a.oldr.us[os];
-            s.us[1] = a.oldr.us[a.rcap + os];
-            s.us[2] = a.oldr.us[2 * a.rcap + os];
+        if (kind == ST_OLD) {
+            sus[0] = a.oldr.us[oldslot];
+            sus[1] = a.oldr.us[a.rcap + oldslot];
+            sus[2] = a.oldr.us[2 * a.rcap + oldslot];
             const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[2], 1ull);
             if ((int64_t)p < a.rcap) a.orphan[p] = (int32_t)idx;
         }
-        a.newr.sf[idx] = s.sf;
-        a.newr.id[idx] = s.id;
-        a.newr.us[idx] = s.us[0];
-        a.newr.us[a.rcap + idx] = s.us[1];
-        a.newr.us[2 * a.rcap + idx] = s.us[2];
-        a.newr.cell[idx] = (int32_t)cidx[k];
-        a.newr.cand[idx] = s.e;
-        a.slot[cidx[k]] = (int32_t)idx;
-        if (s.dv && a.dv_cell) {
-            const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[5], 1ull);
-            if ((int64_t)p < a.rcap) {
-                a.dv_cell[p] = (int32_t)cidx[k];
-                a.dv_us[3 * p] = s.dvus[0];
-                a.dv_us[3 * p + 1] = s.dvus[1];
-                a.dv_us[3 * p + 2] = s.dvus[2];
-            }
-        }
+        a.newr.sf[idx] = ssf;
+        a.newr.id[idx] = sid;
+        a.newr.us[idx] = sus[0];
+        a.newr.us[a.rcap + idx] = sus[1];
+        a.newr.us[2 * a.rcap + idx] = sus[2];
+        a.newr.cell[idx] = (int32_t)cidx;
+        a.newr.cand[idx] = se;
+        a.slot[cidx] = (int32_t)idx;
     }
-    /* cells that end empty but requested defineVelocity keep it too */
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-        if (!valid[k] || fl[k] || !st[k].dv || !a.dv_cell) continue;
+    if (INITVEL && dv && a.dv_cell) {
         const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[5], 1ull);
         if ((int64_t)p < a.rcap) {
-            a.dv_cell[p] = (int32_t)cidx[k];
-            a.dv_us[3 * p] = st[k].dvus[0];
-            a.dv_us[3 * p + 1] = st[k].dvus[1];
-            a.dv_us[3 * p + 2] = st[k].dvus[2];
+            a.dv_cell[p] = (int32_t)cidx;
+            a.dv_us[3 * p] = dvus[0];
+            a.dv_us[3 * p + 1] = dvus[1];
+            a.dv_us[3 * p + 2] = dvus[2];
         }
     }
 }
@@ -592,7 +552,7 @@ int run_voxelize(lbdem_psm* h) {
     VoxArgs va = vox_args(h);
     /* counters[1] (overflow flags) is cleared by the public entry points only */
     PSM_HIP(h, hipMemsetAsync(h->counters, 0, sizeof(int64_t), h->stream));
-    PSM_HIP(h, hipMemsetAsync(h->counters + 2, 0, 6 * sizeof(int64_t), h->stream));
+    PSM_HIP(h, hipMemsetAsync(h->counters + 2, 0, (6 + 64) * sizeof(int64_t), h->stream));
     PSM_HIP(h, hipMemsetAsync(h->bin_cand_count, 0, nbins * sizeof(int32_t), h->stream));
     PSM_HIP(h, hipMemsetAsync(h->bin_fill, 0, nbins * sizeof(int32_t), h->stream));
     const unsigned pg = (unsigned)std::max<int64_t>(1, (h->np + 3) / 4);   /* a wave per particle */
@@ -630,12 +590,22 @@ int run_voxelize(lbdem_psm* h) {
     fa.orphan = h->orphan;
     fa.dv_cell = fa.init_vel ? h->dv_cell : nullptr;
     fa.dv_us = h->dv_us;
-    dim3 grid((unsigned)nbins), block(256);
-    if (fa.N == 5) hipLaunchKernelGGL(k_fold<5>, grid, block, 0, h->stream, fa);
-    else if (fa.N == 8) hipLaunchKernelGGL(k_fold<8>, grid, block, 0, h->stream, fa);
-    else hipLaunchKernelGGL(k_fold<0>, grid, block, 0, h->stream, fa);
+    fa.seg_count = h->counters + 8;
+    fa.nseg = h->nseg;
+    fa.seg = h->rec_cap / h->nseg;
+    dim3 grid((unsigned)nbins), block(512);
+    if (fa.init_vel) {
+        if (fa.N == 5) hipLaunchKernelGGL((k_fold<5, true>), grid, block, 0, h->stream, fa);
+        else if (fa.N == 8) hipLaunchKernelGGL((k_fold<8, true>), grid, block, 0, h->stream, fa);
+        else hipLaunchKernelGGL((k_fold<0, true>), grid, block, 0, h->stream, fa);
+    } else {
+        if (fa.N == 5) hipLaunchKernelGGL((k_fold<5, false>), grid, block, 0, h->stream, fa);
+        else if (fa.N == 8) hipLaunchKernelGGL((k_fold<8, false>), grid, block, 0, h->stream, fa);
+        else hipLaunchKernelGGL((k_fold<0, false>), grid, block, 0, h->stream, fa);
+    }
     PSM_HIP(h, hipGetLastError());
     h->cur_rec = 1 - h->cur_rec;
+    h->records_from_host = false;
     prof_end(h, 1, ev);
     if (fa.init_vel) {
         int rc = launch_define_velocity(h);
