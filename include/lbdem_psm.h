@@ -61,7 +61,7 @@ typedef struct lbdem_psm_desc {
     double body_accel[3];        /* Guo body-force acceleration, lattice units */
     int32_t device;              /* HIP device ordinal */
     int32_t reserved;
-    int64_t max_solid_cells;     /* coupling-record capacity, 0 = automatic */
+    int64_t max_solid_cells;     /* optional limit on live coupling records, 0 = none */
 } lbdem_psm_desc;
 
 /* Defaults: 1x1x1, periodic, tau 1, FP64, 5 slices, no force, device 0. */

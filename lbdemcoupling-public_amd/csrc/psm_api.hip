@@ -73,18 +73,12 @@ static int alloc_records(lbdem_psm* h, Records& r) {
     PSM_HIP(h, dalloc(&r.id, cap * sizeof(int32_t)));
     PSM_HIP(h, dalloc(&r.us, 3 * cap * sizeof(double)));
     PSM_HIP(h, dalloc(&r.F, 3 * cap * sizeof(double)));
-    PSM_HIP(h, dalloc(&r.cell, cap * sizeof(int32_t)));
     PSM_HIP(h, dalloc(&r.cand, cap * sizeof(int32_t)));
-    PSM_HIP(h, dalloc(&r.bin_start, h->nbins * sizeof(int32_t)));
-    PSM_HIP(h, dalloc(&r.bin_count, h->nbins * sizeof(int32_t)));
-    PSM_HIP(h, hipMemset(r.bin_start, 0, h->nbins * sizeof(int32_t)));
-    PSM_HIP(h, hipMemset(r.bin_count, 0, h->nbins * sizeof(int32_t)));
     return LBDEM_PSM_OK;
 }
 
 static void free_records(Records& r) {
-    dfree(r.sf); dfree(r.id); dfree(r.us); dfree(r.F); dfree(r.cell); dfree(r.cand);
-    dfree(r.bin_start); dfree(r.bin_count);
+    dfree(r.sf); dfree(r.id); dfree(r.us); dfree(r.F); dfree(r.cand);
     memset(&r, 0, sizeof(r));
 }
 
@@ -137,7 +131,8 @@ static int ensure_entries(lbdem_psm* h, int64_t need) {
     if (need > (int64_t)INT32_MAX)
         return set_err(h, LBDEM_PSM_ERR_CAPACITY, "candidate entries exceed 2^31");
     if (need <= h->ent_cap && h->cand_key) return LBDEM_PSM_OK;
-    dfree(h->cand_key); dfree(h->cand_bin); dfree(h->owner_ent); dfree(h->partial);
+    dfree(h->cand_key); dfree(h->cand_bin); dfree(h->owner_ent); dfree(h->partial); dfree(h->cand_data);
+    PSM_HIP(h, dalloc(&h->cand_data, need * sizeof(Cand)));
     PSM_HIP(h, dalloc(&h->cand_key, need * sizeof(uint32_t)));
     PSM_HIP(h, dalloc(&h->cand_bin, need * sizeof(int32_t)));
     PSM_HIP(h, dalloc(&h->owner_ent, need * sizeof(int32_t)));
@@ -147,19 +142,26 @@ static int ensure_entries(lbdem_psm* h, int64_t need) {
 }
 
 static int read_counters(lbdem_psm* h) {
-    PSM_HIP(h, hipMemcpyAsync(h->h_counters, h->counters, 72 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+    PSM_HIP(h, hipMemcpyAsync(h->h_counters, h->counters, 8 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
     PSM_HIP(h, hipStreamSynchronize(h->stream));
     return LBDEM_PSM_OK;
 }
 
 static int check_overflow(lbdem_psm* h) {
+    if (h->max_records > 0) {
+        int rc0 = count_records(h);
+        if (rc0) return rc0;
+    }
     int rc = read_counters(h);
     if (rc) return rc;
-    if (h->h_counters[1] & 1)
-        return set_err(h, LBDEM_PSM_ERR_CAPACITY,
-                       "coupling-record capacity exceeded (raise max_solid_cells); lattice state is invalid");
     if (h->h_counters[1] & 2)
-        return set_err(h, LBDEM_PSM_ERR_CAPACITY, "candidate-entry capacity exceeded");
+        return set_err(h, LBDEM_PSM_ERR_CAPACITY, "candidate-entry capacity exceeded; lattice state is invalid");
+    if (h->h_counters[1] & 4)
+        return set_err(h, LBDEM_PSM_ERR_CAPACITY, "too many stale records; forces incomplete");
+    if (h->h_counters[1] & 8)
+        return set_err(h, LBDEM_PSM_ERR_CAPACITY, "too many initVel cells");
+    if (h->max_records > 0 && h->h_counters[0] > h->max_records)
+        return set_err(h, LBDEM_PSM_ERR_CAPACITY, "coupling records exceed max_solid_cells");
     return LBDEM_PSM_OK;
 }
 
@@ -283,12 +285,9 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
     h->precision = desc->precision;
     const size_t S = h->precision == LBDEM_PSM_FP64 ? 8 : (h->precision == LBDEM_PSM_FP32_DEV ? 4 : 2);
     h->pop_bytes = (size_t)Q * (size_t)nn[0] * (size_t)nn[1] * (size_t)h->g.nzp * S;
-    h->rec_cap = desc->max_solid_cells > 0
-                     ? desc->max_solid_cells
-                     : std::min<int64_t>(h->g.nc, std::max<int64_t>(h->g.nc / 2, 1 << 20));
-    /* records are allocated per workgroup from nseg segments of the buffer */
-    h->nseg = (int)std::max<int64_t>(1, std::min<int64_t>(64, h->rec_cap / 8192));
-
+    h->rec_cap = h->nbins * 512;                 /* one record slot per cell of the bin grid */
+    h->max_records = desc->max_solid_cells;
+    h->orphan_cap = std::min<int64_t>(h->rec_cap, (int64_t)1 << 24);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { rc = hip_check(nullptr, e, "hipStreamCreate"); delete h; return rc; }
     std::string msg;
@@ -305,22 +304,24 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
     } while (0)
     CREATE_CHECK(dalloc(&h->f[0], h->pop_bytes));
     CREATE_CHECK(dalloc(&h->f[1], h->pop_bytes));
-    CREATE_CHECK(dalloc(&h->slot, h->g.nc * sizeof(int32_t)));
-    CREATE_CHECK(hipMemset(h->slot, 0xff, h->g.nc * sizeof(int32_t)));
-    CREATE_CHECK(dalloc(&h->orphan, h->rec_cap * sizeof(int32_t)));
+    CREATE_CHECK(dalloc(&h->sflag, h->g.nc * sizeof(uint8_t)));
+    CREATE_CHECK(hipMemset(h->sflag, 0, h->g.nc * sizeof(uint8_t)));
+    for (int k = 0; k < 2; ++k) {
+        CREATE_CHECK(dalloc(&h->bin_any[k], h->nbins * sizeof(uint32_t)));
+        CREATE_CHECK(hipMemset(h->bin_any[k], 0, h->nbins * sizeof(uint32_t)));
+    }
+    CREATE_CHECK(dalloc(&h->orphan, h->orphan_cap * sizeof(int32_t)));
     CREATE_CHECK(dalloc(&h->bin_cand_count, h->nbins * sizeof(int32_t)));
     CREATE_CHECK(dalloc(&h->bin_cand_start, h->nbins * sizeof(int32_t)));
     CREATE_CHECK(dalloc(&h->bin_fill, h->nbins * sizeof(int32_t)));
     CREATE_CHECK(hipMemset(h->bin_cand_count, 0, h->nbins * sizeof(int32_t)));
     CREATE_CHECK(hipMemset(h->bin_cand_start, 0, h->nbins * sizeof(int32_t)));
-    CREATE_CHECK(dalloc(&h->counters, 72 * sizeof(int64_t)));
-    CREATE_CHECK(hipMemset(h->counters, 0, 72 * sizeof(int64_t)));
-    CREATE_CHECK(hipHostMalloc((void**)&h->h_counters, 72 * sizeof(int64_t), hipHostMallocDefault));
+    CREATE_CHECK(dalloc(&h->counters, 8 * sizeof(int64_t)));
+    CREATE_CHECK(hipMemset(h->counters, 0, 8 * sizeof(int64_t)));
+    CREATE_CHECK(hipHostMalloc((void**)&h->h_counters, 8 * sizeof(int64_t), hipHostMallocDefault));
     CREATE_CHECK(dalloc(&h->red_tmp, (6 * 4096 + 8) * sizeof(double)));
-    for (int k = 0; k < 2; ++k) {
-        rc = alloc_records(h, h->rec[k]);
-        if (rc) { msg = h->err; lbdem_psm_destroy(h); g_create_err = msg; return rc; }
-    }
+    rc = alloc_records(h, h->rec);
+    if (rc) { msg = h->err; lbdem_psm_destroy(h); g_create_err = msg; return rc; }
     rc = ensure_particles(h, 256);
     if (rc) { msg = h->err; lbdem_psm_destroy(h); g_create_err = msg; return rc; }
     rc = ensure_entries(h, 1024);
@@ -336,8 +337,9 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
 int lbdem_psm_destroy(lbdem_psm* h) {
     if (!h) return LBDEM_PSM_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    dfree(h->f[0]); dfree(h->f[1]); dfree(h->slot);
-    for (int k = 0; k < 2; ++k) free_records(h->rec[k]);
+    dfree(h->f[0]); dfree(h->f[1]); dfree(h->sflag);
+    dfree(h->bin_any[0]); dfree(h->bin_any[1]); dfree(h->cand_data);
+    free_records(h->rec);
     dfree(h->px); dfree(h->pv); dfree(h->pw); dfree(h->pr); dfree(h->pid); dfree(h->pexcl);
     dfree(h->powner); dfree(h->id_map); dfree(h->bin_cand_count); dfree(h->bin_cand_start);
     dfree(h->bin_fill); dfree(h->cand_key); dfree(h->cand_bin); dfree(h->p_nent);
@@ -458,8 +460,8 @@ int lbdem_psm_set_spheres(lbdem_psm* h, int64_t n, const double* x3, const doubl
     h->flags = flags;
     h->have_particles = true;
     if ((flags & LBDEM_PSM_INIT_VEL) && !h->dv_cell) {
-        PSM_HIP(h, dalloc(&h->dv_cell, h->rec_cap * sizeof(int32_t)));
-        PSM_HIP(h, dalloc(&h->dv_us, 3 * h->rec_cap * sizeof(double)));
+        PSM_HIP(h, dalloc(&h->dv_cell, h->orphan_cap * sizeof(int32_t)));
+        PSM_HIP(h, dalloc(&h->dv_us, 3 * h->orphan_cap * sizeof(double)));
     }
     PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
     rc = run_voxelize(h);
@@ -532,31 +534,29 @@ int lbdem_psm_get_particles(lbdem_psm* h, int64_t n, double* x3) {
 int lbdem_psm_get_cell_records(lbdem_psm* h, double* sf, int64_t* id, double* us, double* F) {
     if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
     const int64_t nc = h->g.nc, cap = h->rec_cap;
+    const int n0 = h->g.n[0], n1 = h->g.n[1], n2 = h->g.n[2];
     PSM_HIP(h, hipStreamSynchronize(h->stream));
-    std::vector<int32_t> slot(nc);
-    PSM_HIP(h, hipMemcpy(slot.data(), h->slot, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
-    int64_t m = 0;
-    for (int64_t c = 0; c < nc; ++c) m = std::max<int64_t>(m, (int64_t)slot[c] + 1);
-    const Records& r = h->rec[h->cur_rec];
-    std::vector<double> rsf(m), rus(3 * m), rF(3 * m);
-    std::vector<int32_t> rid(m);
-    if (m > 0) {
-        PSM_HIP(h, hipMemcpy(rsf.data(), r.sf, m * sizeof(double), hipMemcpyDeviceToHost));
-        PSM_HIP(h, hipMemcpy(rid.data(), r.id, m * sizeof(int32_t), hipMemcpyDeviceToHost));
-        for (int d = 0; d < 3; ++d) {
-            PSM_HIP(h, hipMemcpy(&rus[d * m], r.us + d * cap, m * sizeof(double), hipMemcpyDeviceToHost));
-            PSM_HIP(h, hipMemcpy(&rF[d * m], r.F + d * cap, m * sizeof(double), hipMemcpyDeviceToHost));
-        }
-    }
-    for (int64_t c = 0; c < nc; ++c) {
-        const int32_t s = slot[c];
-        if (sf) sf[c] = s >= 0 ? rsf[s] : 0.0;
-        if (id) id[c] = s >= 0 ? rid[s] : 0;
-        for (int d = 0; d < 3; ++d) {
-            if (us) us[d * nc + c] = s >= 0 ? rus[d * m + s] : 0.0;
-            if (F) F[d * nc + c] = s >= 0 ? rF[d * m + s] : 0.0;
-        }
-    }
+    std::vector<uint8_t> flag(nc);
+    PSM_HIP(h, hipMemcpy(flag.data(), h->sflag, nc, hipMemcpyDeviceToHost));
+    std::vector<double> rsf(cap), rus(3 * cap), rF(3 * cap);
+    std::vector<int32_t> rid(cap);
+    PSM_HIP(h, hipMemcpy(rsf.data(), h->rec.sf, cap * sizeof(double), hipMemcpyDeviceToHost));
+    PSM_HIP(h, hipMemcpy(rid.data(), h->rec.id, cap * sizeof(int32_t), hipMemcpyDeviceToHost));
+    PSM_HIP(h, hipMemcpy(rus.data(), h->rec.us, 3 * cap * sizeof(double), hipMemcpyDeviceToHost));
+    PSM_HIP(h, hipMemcpy(rF.data(), h->rec.F, 3 * cap * sizeof(double), hipMemcpyDeviceToHost));
+    for (int X = 0; X < n0; ++X)
+        for (int Y = 0; Y < n1; ++Y)
+            for (int Z = 0; Z < n2; ++Z) {
+                const int64_t c = ((int64_t)X * n1 + Y) * n2 + Z;
+                const bool live = flag[c] != 0;
+                const int64_t r = rec_index(X, Y, Z, h->g.nb[1], h->g.nb[2]);
+                if (sf) sf[c] = live ? rsf[r] : 0.0;
+                if (id) id[c] = live ? rid[r] : 0;
+                for (int d = 0; d < 3; ++d) {
+                    if (us) us[d * nc + c] = live ? rus[d * cap + r] : 0.0;
+                    if (F) F[d * nc + c] = live ? rF[d * cap + r] : 0.0;
+                }
+            }
     return LBDEM_PSM_OK;
 }
 
@@ -564,50 +564,38 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
     if (!h || !sf) return LBDEM_PSM_ERR_INVALID_ARG;
     const int64_t nc = h->g.nc, cap = h->rec_cap, nbins = h->nbins;
     const int n0 = h->g.n[0], n1 = h->g.n[1], n2 = h->g.n[2];
-    std::vector<int32_t> slot(nc, -1), bstart(nbins, 0), bcount(nbins, 0);
-    std::vector<double> rsf, rus0, rus1, rus2;
-    std::vector<int32_t> rid, rcell;
-    int64_t m = 0;
-    for (int64_t b = 0; b < nbins; ++b) {
-        const int bz = (int)(b % h->g.nb[2]), by = (int)((b / h->g.nb[2]) % h->g.nb[1]);
-        const int bx = (int)(b / ((int64_t)h->g.nb[2] * h->g.nb[1]));
-        bstart[b] = (int32_t)m;
-        for (int l = 0; l < BIN_CELLS; ++l) {
-            const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
-            if (X >= n0 || Y >= n1 || Z >= n2) continue;
-            const int64_t c = ((int64_t)X * n1 + Y) * n2 + Z;
-            if (!(sf[c] > SOLFRAC_MIN)) continue;
-            if (m >= cap) return set_err(h, LBDEM_PSM_ERR_CAPACITY, "records exceed max_solid_cells");
-            const int64_t pid = id ? id[c] : 0;
-            if (pid < 0 || pid >= (int64_t)INT32_MAX) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "bad record id");
-            slot[c] = (int32_t)m;
-            rsf.push_back(sf[c]);
-            rid.push_back((int32_t)pid);
-            rus0.push_back(us ? us[c] : 0.0);
-            rus1.push_back(us ? us[nc + c] : 0.0);
-            rus2.push_back(us ? us[2 * nc + c] : 0.0);
-            rcell.push_back((int32_t)c);
-            ++m;
-        }
-        bcount[b] = (int32_t)(m - bstart[b]);
-    }
+    std::vector<uint8_t> flag(nc, 0);
+    std::vector<double> rsf(cap, 0.0), rus(3 * cap, 0.0);
+    std::vector<int32_t> rid(cap, 0), cand(cap, -1), orph;
+    std::vector<uint32_t> any(nbins, 0u);   /* live records per bin */
+    for (int X = 0; X < n0; ++X)
+        for (int Y = 0; Y < n1; ++Y)
+            for (int Z = 0; Z < n2; ++Z) {
+                const int64_t c = ((int64_t)X * n1 + Y) * n2 + Z;
+                /* only records with sf > SOLFRAC_MIN take part in collision and ownership */
+                if (!(sf[c] > SOLFRAC_MIN)) continue;
+                const int64_t pid = id ? id[c] : 0;
+                if (pid < 0 || pid >= (int64_t)INT32_MAX) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "bad record id");
+                const int64_t r = rec_index(X, Y, Z, h->g.nb[1], h->g.nb[2]);
+                flag[c] = 1;
+                rsf[r] = sf[c];
+                rid[r] = (int32_t)pid;
+                for (int d = 0; d < 3; ++d) rus[d * cap + r] = us ? us[d * nc + c] : 0.0;
+                any[r >> 9] += 1u;
+                orph.push_back((int32_t)r);
+            }
+    const int64_t m = (int64_t)orph.size();
+    if (m > h->orphan_cap) return set_err(h, LBDEM_PSM_ERR_CAPACITY, "too many records");
+    if (h->max_records > 0 && m > h->max_records)
+        return set_err(h, LBDEM_PSM_ERR_CAPACITY, "records exceed max_solid_cells");
     PSM_HIP(h, hipStreamSynchronize(h->stream));
-    const Records& r = h->rec[h->cur_rec];
-    std::vector<int32_t> cand(m, -1), orph(m);
-    for (int64_t k = 0; k < m; ++k) orph[k] = (int32_t)k;
-    if (m > 0) {
-        PSM_HIP(h, hipMemcpy(r.sf, rsf.data(), m * sizeof(double), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(r.id, rid.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(r.us, rus0.data(), m * sizeof(double), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(r.us + cap, rus1.data(), m * sizeof(double), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(r.us + 2 * cap, rus2.data(), m * sizeof(double), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(r.cell, rcell.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(r.cand, cand.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(h->orphan, orph.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
-    }
-    PSM_HIP(h, hipMemcpy(r.bin_start, bstart.data(), nbins * sizeof(int32_t), hipMemcpyHostToDevice));
-    PSM_HIP(h, hipMemcpy(r.bin_count, bcount.data(), nbins * sizeof(int32_t), hipMemcpyHostToDevice));
-    PSM_HIP(h, hipMemcpy(h->slot, slot.data(), nc * sizeof(int32_t), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(h->sflag, flag.data(), nc, hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(h->rec.sf, rsf.data(), cap * sizeof(double), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(h->rec.id, rid.data(), cap * sizeof(int32_t), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(h->rec.us, rus.data(), 3 * cap * sizeof(double), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(h->rec.cand, cand.data(), cap * sizeof(int32_t), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(h->bin_any[h->cur_any], any.data(), nbins * sizeof(uint32_t), hipMemcpyHostToDevice));
+    if (m > 0) PSM_HIP(h, hipMemcpy(h->orphan, orph.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
     /* Until the next voxelization every record is summed on the orphan
      * path: no candidate lists describe these records. */
     PSM_HIP(h, hipMemset(h->bin_cand_count, 0, nbins * sizeof(int32_t)));
@@ -662,16 +650,16 @@ int lbdem_psm_check_health(lbdem_psm* h, double* max_u, int64_t* n_nonfinite) {
 
 int lbdem_psm_stats(lbdem_psm* h, int64_t* out4) {
     if (!h || !out4) return LBDEM_PSM_ERR_INVALID_ARG;
-    int rc = read_counters(h);
+    int rc = count_records(h);
+    if (rc) return rc;
+    rc = read_counters(h);
     if (rc) return rc;
     int32_t last_start = 0, last_count = 0;
     if (h->nbins > 0) {
         PSM_HIP(h, hipMemcpy(&last_start, h->bin_cand_start + h->nbins - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
         PSM_HIP(h, hipMemcpy(&last_count, h->bin_cand_count + h->nbins - 1, sizeof(int32_t), hipMemcpyDeviceToHost));
     }
-    int64_t nrec = 0;
-    for (int k = 0; k < h->nseg; ++k) nrec += h->h_counters[8 + k];
-    out4[0] = h->records_from_host ? h->h_counters[0] : nrec;
+    out4[0] = h->h_counters[0];
     out4[1] = (int64_t)last_start + last_count;
     out4[2] = h->h_counters[4];
     out4[3] = h->h_counters[2];

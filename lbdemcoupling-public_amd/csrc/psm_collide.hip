@@ -8,14 +8,13 @@
  * Layout: populations are stored as f - t in FP64, FP32 or FP16, row-blocked
  * structure of arrays: row r = x*ny + y holds the 19 z-lines of its
  * populations back to back, f[(r*19 + q)*nzp + z].  One workgroup owns one
- * z-row: every lane reads its cell's 19 populations with unit stride (fully
- * coalesced, all inside one row block), collides, and pushes each
- * population to its neighbour x + c_q, i.e. into one of 9 row blocks.  A
- * destination z-line is written only by one workgroup, so every cache line
- * is filled by one workgroup.
- * Cells with a coupling record are found through slot[c] (int32, -1 for
- * plain fluid); the record (solid fraction, particle velocity) is read and
- * the per-cell hydrodynamic force written only for those cells.
+ * z-row chunk of 256 cells: every lane reads its cell's 19 populations with
+ * unit stride (fully coalesced, all inside one row block), collides, and
+ * pushes each population to its neighbour x + c_q, i.e. into one of 9 row
+ * blocks.
+ * Cells with a coupling record are marked by a 1-byte flag; only for those
+ * is the record (solid fraction, particle velocity) read and the per-cell
+ * hydrodynamic force written.
  * Non-periodic axes use half-way bounce-back.
  */
 #include "psm_collide_kernel.h"
@@ -164,8 +163,10 @@ static CollideArgs make_args(lbdem_psm* h) {
     a.nzp = h->g.nzp;
     a.omega = h->omega; a.ooo = h->ooo;
     for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
-    const Records& r = h->rec[h->cur_rec];
-    a.slot = h->slot; a.rsf = r.sf; a.rus = r.us; a.rF = r.F; a.rcap = h->rec_cap;
+    a.sflag = h->sflag;
+    a.nb1 = h->g.nb[1];
+    a.nb2 = h->g.nb[2];
+    a.rsf = h->rec.sf; a.rus = h->rec.us; a.rF = h->rec.F; a.rcap = h->rec_cap;
     return a;
 }
 
@@ -279,13 +280,13 @@ int launch_define_velocity(lbdem_psm* h) {
     dim3 grid(1024), block(256);
     switch (h->precision) {
         case LBDEM_PSM_FP64:
-            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->rec_cap, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.nzp);
             break;
         case LBDEM_PSM_FP32_DEV:
-            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->rec_cap, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.nzp);
             break;
         default:
-            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->rec_cap, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.nzp);
             break;
     }
     PSM_HIP(h, hipGetLastError());

@@ -23,7 +23,8 @@ struct CollideArgs {
     int64_t nzp;
     double omega, ooo;
     double acc[3];
-    const int32_t* slot;
+    const uint8_t* sflag;
+    int nb1, nb2;            /* bins along y and z (record index) */
     const double* rsf;
     const double* rus;
     double* rF;
@@ -242,7 +243,8 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     const size_t qstride = (size_t)a.nzp * sizeof(S);
 
     const uint32_t c = (uint32_t)row * (uint32_t)a.nz + (uint32_t)z;
-    const int s = a.slot[c];
+    const bool solid = a.sflag[c] != 0;
+    const int64_t s = solid ? rec_index(x, y, z, a.nb1, a.nb2) : 0;
     const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(S);
     T fp[Q];
 #pragma unroll
@@ -250,7 +252,7 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     T fo[Q];
     if constexpr (sizeof(T) == 4) {
         bgk_all_fast<FORCE>(fp, fo, omega, acc);
-        if (s >= 0) {
+        if (solid) {
             const float sf = (float)a.rsf[s];
             const bool full = a.rsf[s] > SOLFRAC_MAX;
             const float ooo = (float)a.ooo;
@@ -264,7 +266,7 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
         }
     } else {
         bgk_all<T, FORCE>(fp, fo, omega, acc);
-        if (s >= 0) {
+        if (solid) {
             /* IBcompositeDynamics::collide (src/ibCompositeDynamics3D.hh:130-203).
              * sf > 0.999 runs as B = 1 on the pre-collision cell: with B = 1
              * and fB = fP the blend (fP + (1-B)(fB - fP)) + B*Omega reduces

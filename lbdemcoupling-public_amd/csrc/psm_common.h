@@ -212,17 +212,34 @@ template <> struct Store<LBDEM_PSM_FP16_STORE> {
     static __device__ __forceinline__ void store(S* p, T v) { *p = (_Float16)v; }
 };
 
-/* Coupling records, structure of arrays, one buffer of the double buffer. */
+/* Coupling records (IBdynamicsParticleData), structure of arrays in the
+ * bin-permuted dense layout: the cell (x, y, z) of bin b at local index
+ * l = ((x&7)*8 + (y&7))*8 + (z&7) owns entry b*512 + l.  A record is
+ * live when the cell's solid flag is set. */
 struct Records {
     double* sf;      /* solid fraction */
     int32_t* id;     /* particle tag */
     double* us;      /* 3*cap: particle velocity at the cell, component-major */
     double* F;       /* 3*cap: hydrodynamic force of the last collision */
-    int32_t* cell;   /* cell index */
-    int32_t* cand;   /* candidate entry of the setter / owner, -1 = none */
-    int32_t* bin_start;  /* per bin: first record */
-    int32_t* bin_count;  /* per bin: number of records */
+    int32_t* cand;   /* candidate entry of the setter this step, -1 = none */
 };
+
+/* A candidate particle image of one bin, gathered in key order. */
+struct Cand {
+    double x[3];     /* image centre */
+    double r;
+    double v[3];
+    double w[3];
+    double rp2, rm2, rsq;
+    int lo[3], hi[3];   /* clipped bounding box (cells) */
+    int id;
+    int e;              /* entry index */
+};
+
+__host__ __device__ inline int64_t rec_index(int x, int y, int z, int nb1, int nb2) {
+    const int64_t bin = ((int64_t)(x >> 3) * nb1 + (y >> 3)) * nb2 + (z >> 3);
+    return bin * 512 + ((x & 7) * 8 + (y & 7)) * 8 + (z & 7);
+}
 
 struct Geom {
     int n[3];
@@ -253,11 +270,13 @@ struct lbdem_psm {
     hipStream_t stream = nullptr;
     void* f[2] = {nullptr, nullptr};
     int cur_f = 0;
-    int32_t* slot = nullptr;
-    psm::Records rec[2];
-    int cur_rec = 0;
-    int64_t rec_cap = 0;
+    uint8_t* sflag = nullptr;        /* per cell: 1 when the cell holds a coupling record */
+    psm::Records rec;                /* dense, bin-permuted, updated in place */
+    int64_t rec_cap = 0;             /* nbins * 512 */
     int64_t nbins = 0;
+    uint32_t* bin_any[2] = {nullptr, nullptr};   /* per bin: live records (this / next step) */
+    int cur_any = 0;
+    int64_t max_records = 0;         /* optional limit on live records, 0 = none */
 
     /* particles (device) */
     int64_t np = 0, p_cap = 0;
@@ -276,18 +295,18 @@ struct lbdem_psm {
     int32_t *bin_cand_count = nullptr, *bin_cand_start = nullptr, *bin_fill = nullptr;
     uint32_t* cand_key = nullptr;
     int32_t* cand_bin = nullptr;
+    psm::Cand* cand_data = nullptr;  /* candidates in key order, gathered per step */
     int64_t ent_cap = 0;
     int32_t *p_nent = nullptr, *p_ent_start = nullptr, *owner_ent = nullptr;
     int32_t* orphan = nullptr;       /* record indices without a candidate this step */
+    int64_t orphan_cap = 0;
     double* partial = nullptr;       /* 6 per candidate entry */
     double* pft = nullptr;           /* 6 per particle: F[3], T[3] */
     double* psum_x = nullptr;        /* positions for torque arms */
     void* scan_tmp = nullptr;
     size_t scan_tmp_bytes = 0;
-    int64_t* counters = nullptr;     /* device: [0] unused, [1] overflow flags, [2] orphans,
-                                        [3] unused, [4] images, [5] defineVelocity cells,
-                                        [8 .. 8+nseg) record-allocation segment counters */
-    int nseg = 1;                    /* record-allocation segments */
+    int64_t* counters = nullptr;     /* device: [0] live records, [1] overflow flags, [2] orphans,
+                                        [3] unused, [4] images, [5] defineVelocity cells */
     int64_t* h_counters = nullptr;   /* pinned host mirror */
     double* red_tmp = nullptr;       /* reduction scratch */
     int32_t* dv_cell = nullptr;      /* defineVelocity list (initVel), lazily allocated */
@@ -316,6 +335,7 @@ int run_voxelize(lbdem_psm* h);
 int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_r, double* d_sf,
                    int32_t* d_cnt, int32_t* d_br);
 int run_force_sum(lbdem_psm* h);
+int count_records(lbdem_psm* h);
 int launch_advance(lbdem_psm* h);
 int launch_define_velocity(lbdem_psm* h);
 int set_err(lbdem_psm* h, int code, const std::string& msg);

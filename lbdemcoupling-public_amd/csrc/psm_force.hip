@@ -6,7 +6,7 @@
  * (d = cell - x[map(id)] with the minimum image over the block size,
  * :235-251) into the particle's slot.
  *
- * Here the sum runs over the compact record list only, in two
+ * Here the sum runs over the bins that hold candidates, in two
  * deterministic stages:
  *  1. per bin (one workgroup): the bin's records are staged in LDS; each
  *     wavefront takes some of the bin's candidate entries and sums, for
@@ -24,21 +24,28 @@ namespace psm {
 struct ForceArgs {
     int n[3];
     int nb[3];
-    const int32_t* bin_rec_start;
-    const int32_t* bin_rec_count;
     const int32_t* bin_cand_start;
     const int32_t* bin_cand_count;
     const uint32_t* key;
-    const int32_t* rec_cell;
-    const int32_t* rec_cand;
-    const int32_t* rec_id;
-    const double* rF;
+    const uint8_t* sflag;
+    Records rec;
     int64_t rcap;
     const int32_t* powner;
     const double* xs;      /* positions for the torque arms */
     double* partial;
     int64_t ent_cap;
 };
+
+__device__ __forceinline__ void rec_cell(int64_t ridx, const int* nb, int& X, int& Y, int& Z) {
+    const int64_t b = ridx >> 9;
+    const int l = (int)(ridx & 511);
+    const int bz = (int)(b % nb[2]);
+    const int by = (int)((b / nb[2]) % nb[1]);
+    const int bx = (int)(b / ((int64_t)nb[2] * nb[1]));
+    X = bx * BIN + (l >> 6);
+    Y = by * BIN + ((l >> 3) & 7);
+    Z = bz * BIN + (l & 7);
+}
 
 __device__ __forceinline__ double min_image(double d, int n) {
     /* src/ibProcessors3D.hh:238-243, integer halves of the block size */
@@ -55,27 +62,27 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
     __shared__ double sF[BIN_CELLS][3];
-    __shared__ int sX[BIN_CELLS][3];
     __shared__ int sE[BIN_CELLS];
     const int64_t b = blockIdx.x;
     const int ncand = a.bin_cand_count[b];
     if (ncand == 0) return;
     const int cstart = a.bin_cand_start[b];
     if ((int64_t)cstart + ncand > a.ent_cap) return;
-    const int cnt = a.bin_rec_count[b];
-    const int rstart = a.bin_rec_start[b];
-    for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
-        const int r = rstart + k;
-        sF[k][0] = a.rF[r];
-        sF[k][1] = a.rF[a.rcap + r];
-        sF[k][2] = a.rF[2 * a.rcap + r];
-        const int c = a.rec_cell[r];
-        const int z = c % a.n[2];
-        const int t = c / a.n[2];
-        sX[k][0] = t / a.n[1];
-        sX[k][1] = t % a.n[1];
-        sX[k][2] = z;
-        sE[k] = a.rec_cand[r];
+    const int bz = (int)(b % a.nb[2]);
+    const int by = (int)((b / a.nb[2]) % a.nb[1]);
+    const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
+    for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) {
+        const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
+        const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
+        int e = -1;
+        if (valid && a.sflag[((int64_t)X * a.n[1] + Y) * a.n[2] + Z]) {
+            const int64_t r = b * 512 + l;
+            e = a.rec.cand[r];
+            sF[l][0] = a.rec.F[r];
+            sF[l][1] = a.rec.F[a.rcap + r];
+            sF[l][2] = a.rec.F[2 * a.rcap + r];
+        }
+        sE[l] = e;
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -85,12 +92,12 @@ __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
         const int o = a.powner[q];
         const double xo = a.xs[3 * o], yo = a.xs[3 * o + 1], zo = a.xs[3 * o + 2];
         double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int k = lane; k < cnt; k += 64) {
-            if (sE[k] != e) continue;
-            const double dx = min_image((double)sX[k][0] - xo, a.n[0]);
-            const double dy = min_image((double)sX[k][1] - yo, a.n[1]);
-            const double dz = min_image((double)sX[k][2] - zo, a.n[2]);
-            const double fx = sF[k][0], fy = sF[k][1], fz = sF[k][2];
+        for (int l = lane; l < BIN_CELLS; l += 64) {
+            if (sE[l] != e) continue;
+            const double dx = min_image((double)(bx * BIN + (l >> 6)) - xo, a.n[0]);
+            const double dy = min_image((double)(by * BIN + ((l >> 3) & 7)) - yo, a.n[1]);
+            const double dz = min_image((double)(bz * BIN + (l & 7)) - zo, a.n[2]);
+            const double fx = sF[l][0], fy = sF[l][1], fz = sF[l][2];
             acc[0] += fx;
             acc[1] += fy;
             acc[2] += fz;
@@ -137,24 +144,23 @@ __global__ __launch_bounds__(256) void k_force_particle(int64_t np, const int32_
 }
 
 __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_t* __restrict__ orphan,
-                                                       const int64_t* __restrict__ counters,
+                                                       const int64_t* __restrict__ counters, int64_t ocap,
                                                        const int32_t* __restrict__ id_map,
                                                        int64_t id_map_size, double* __restrict__ pft) {
-    const int64_t n = counters[2] < a.rcap ? counters[2] : a.rcap;
+    const int64_t n = counters[2] < ocap ? counters[2] : ocap;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * blockDim.x) {
-        const int r = orphan[k];
-        const int id = a.rec_id[r];
+        const int64_t r = orphan[k];
+        const int id = a.rec.id[r];
         if (id < 1 || id >= id_map_size) continue;
         const int o = id_map[id];
         if (o < 0) continue;
-        const int c = a.rec_cell[r];
-        const int z = c % a.n[2];
-        const int t = c / a.n[2];
-        const double dx = min_image((double)(t / a.n[1]) - a.xs[3 * o], a.n[0]);
-        const double dy = min_image((double)(t % a.n[1]) - a.xs[3 * o + 1], a.n[1]);
-        const double dz = min_image((double)z - a.xs[3 * o + 2], a.n[2]);
-        const double fx = a.rF[r], fy = a.rF[a.rcap + r], fz = a.rF[2 * a.rcap + r];
+        int X, Y, Z;
+        rec_cell(r, a.nb, X, Y, Z);
+        const double dx = min_image((double)X - a.xs[3 * o], a.n[0]);
+        const double dy = min_image((double)Y - a.xs[3 * o + 1], a.n[1]);
+        const double dz = min_image((double)Z - a.xs[3 * o + 2], a.n[2]);
+        const double fx = a.rec.F[r], fy = a.rec.F[a.rcap + r], fz = a.rec.F[2 * a.rcap + r];
         atomicAdd(&pft[6 * o + 0], fx);
         atomicAdd(&pft[6 * o + 1], fy);
         atomicAdd(&pft[6 * o + 2], fz);
@@ -167,18 +173,13 @@ __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_
 int run_force_sum(lbdem_psm* h) {
     hipEvent_t ev = nullptr;
     prof_begin(h, 2, &ev);
-    const Records& r = h->rec[h->cur_rec];
     ForceArgs a;
     for (int d = 0; d < 3; ++d) { a.n[d] = h->g.n[d]; a.nb[d] = h->g.nb[d]; }
-    a.bin_rec_start = r.bin_start;
-    a.bin_rec_count = r.bin_count;
     a.bin_cand_start = h->bin_cand_start;
     a.bin_cand_count = h->bin_cand_count;
     a.key = h->cand_key;
-    a.rec_cell = r.cell;
-    a.rec_cand = r.cand;
-    a.rec_id = r.id;
-    a.rF = r.F;
+    a.sflag = h->sflag;
+    a.rec = h->rec;
     a.rcap = h->rec_cap;
     a.powner = h->powner;
     a.xs = h->psum_x;
@@ -189,7 +190,7 @@ int run_force_sum(lbdem_psm* h) {
         hipLaunchKernelGGL(k_force_particle, dim3((unsigned)((h->np + 3) / 4)), dim3(256), 0, h->stream,
                            h->np, h->p_ent_start, h->p_nent, h->owner_ent, h->partial, h->pft, h->ent_cap);
         hipLaunchKernelGGL(k_force_orphans, dim3(64), dim3(256), 0, h->stream, a, h->orphan, h->counters,
-                           h->id_map, h->id_map_size, h->pft);
+                           h->orphan_cap, h->id_map, h->id_map_size, h->pft);
     }
     PSM_HIP(h, hipGetLastError());
     prof_end(h, 2, ev);

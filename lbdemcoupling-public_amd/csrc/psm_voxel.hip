@@ -15,8 +15,9 @@
  * the given particles by index, then periodic images by shift rank), each
  * bin's candidate list is sorted, and one workgroup per bin stages the
  * candidates in LDS and folds them, per cell and in key order, through the
- * reference's four-case rule.  The surviving records are compacted per bin
- * (cell order inside the bin) into the next record buffer.
+ * reference's four-case rule.  Each cell's record lives at a fixed,
+ * bin-permuted index and is updated in place (the fold of a cell reads
+ * and writes only that cell's record).
  *
  * The sub-cell count is evaluated in FP64 with the reference's expression
  * order and without contraction (this file is compiled with
@@ -203,46 +204,66 @@ __global__ __launch_bounds__(256) void k_owner_entries(VoxArgs a, const int32_t*
 
 /* ---- the fold ---- */
 
-struct Cand {
-    double x[3];
-    double r;
-    double v[3];
-    double w[3];
-    double rp2, rm2, rsq;
-    int lo[3], hi[3];
-    int id;
-    int e;
-};
+/* Candidate data in key order, one entry per thread: the image box and the
+ * squared radii used by the sub-cell test. */
+__global__ __launch_bounds__(256) void k_cand_gather(VoxArgs a, const uint32_t* __restrict__ key,
+                                                     const double* __restrict__ pv,
+                                                     const double* __restrict__ pw,
+                                                     const int32_t* __restrict__ pid, double h,
+                                                     const int32_t* __restrict__ bin_start,
+                                                     const int32_t* __restrict__ bin_count,
+                                                     int64_t nbins, Cand* __restrict__ out, int64_t cap) {
+    const int64_t nent = nbins > 0 ? (int64_t)bin_start[nbins - 1] + bin_count[nbins - 1] : 0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nent && e < cap;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t kk = key[e];
+        const int q = (int)(kk & QMASK);
+        const int rank = (int)(kk >> QBITS);
+        const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
+        const double r = a.pr[q];
+        double xi[3];
+        int lo[3], hi[3];
+        Cand cd;
+        if (!image_box(x, r, rank, a, lo, hi, xi)) {
+            lo[0] = lo[1] = lo[2] = 1;
+            hi[0] = hi[1] = hi[2] = 0;
+            xi[0] = xi[1] = xi[2] = 0.0;
+        }
+        for (int d = 0; d < 3; ++d) {
+            cd.x[d] = xi[d]; cd.lo[d] = lo[d]; cd.hi[d] = hi[d];
+            cd.v[d] = pv[3 * q + d]; cd.w[d] = pw[3 * q + d];
+        }
+        cd.r = r;
+        const double r_p = r + h;
+        const double r_m = r - h;
+        cd.rp2 = r_p * r_p;
+        cd.rm2 = r_m * r_m;
+        cd.rsq = r * r;
+        cd.id = pid[q];
+        cd.e = (int)e;
+        out[e] = cd;
+    }
+}
 
 struct FoldArgs {
     int n[3];
     int nb[3];
     int N;
     double sw, frac;
-    double sqrt3half;
-    int init_vel;
-    int auto_images;
-    int per[3];
-    const double* px;
-    const double* pv;
-    const double* pw;
-    const double* pr;
-    const int32_t* pid;
     const int32_t* bin_cand_start;
     const int32_t* bin_cand_count;
-    const uint32_t* key;
+    const Cand* cand;
     int64_t ent_cap;
-    Records oldr;
-    Records newr;
+    Records rec;
     int64_t rcap;
-    int32_t* slot;
+    uint8_t* sflag;
+    const uint32_t* cnt_old;   /* per bin: live records of the previous step */
+    uint32_t* cnt_new;         /* per bin: live records after this fold */
     int64_t* counters;
     int32_t* orphan;
+    int64_t orphan_cap;
     int32_t* dv_cell;
     double* dv_us;
-    int64_t* seg_count;   /* nseg record-allocation counters */
-    int nseg;
-    int64_t seg;          /* records per segment */
 };
 
 /* Sub-cell count of one (offset, radius) query by the whole wavefront:
@@ -321,20 +342,15 @@ enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
 /* One workgroup of 512 lanes per 8^3 bin, one cell per lane (a wavefront
  * is one x-slice of the bin).  The cell's record of the previous step is
  * folded with the bin's candidates in key order through the four-case rule
- * of SetSingleSphere3D::process (src/ibProcessors3D.hh:85-105). */
+ * of SetSingleSphere3D::process (src/ibProcessors3D.hh:85-105) and written
+ * back in place. */
 template <int NT, bool INITVEL>
 __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     __shared__ Cand cs[64];
-    __shared__ int wave_tot[8];
-    __shared__ long long s_base;
 
     const int64_t b = blockIdx.x;
     const int ncand = a.bin_cand_count[b];
-    const int old_cnt = a.oldr.bin_count[b];
-    if (ncand == 0 && old_cnt == 0) {
-        if (threadIdx.x == 0) { a.newr.bin_count[b] = 0; a.newr.bin_start[b] = 0; }
-        return;
-    }
+    if (ncand == 0 && a.cnt_old[b] == 0u) return;   /* nothing here, nothing was here */
     const int cstart = a.bin_cand_start[b];
     if ((int64_t)cstart + ncand > a.ent_cap) return;   /* overflow already flagged */
     const int bz = (int)(b % a.nb[2]);
@@ -344,51 +360,22 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
     const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
     const int64_t cidx = valid ? ((int64_t)X * a.n[1] + Y) * a.n[2] + Z : 0;
+    const int64_t ridx = b * 512 + l;
 
-    const int oldslot = valid ? a.slot[cidx] : -1;
-    int kind = ST_ZERO, sid = 0, se = -1;
+    const bool had = valid && a.sflag[cidx] != 0;
+    int kind = had ? ST_OLD : ST_ZERO, sid = 0, se = -1;
     double ssf = 0.0, sus[3] = {0.0, 0.0, 0.0};
     bool dv = false;
     double dvus[3] = {0.0, 0.0, 0.0};
-    if (oldslot >= 0) {
-        kind = ST_OLD;
-        ssf = a.oldr.sf[oldslot];
-        sid = a.oldr.id[oldslot];
+    if (had) {
+        ssf = a.rec.sf[ridx];
+        sid = a.rec.id[ridx];
     }
 
     for (int c0 = 0; c0 < ncand; c0 += 64) {
         const int nchunk = min(64, ncand - c0);
         __syncthreads();
-        if ((int)threadIdx.x < nchunk) {
-            const int e = cstart + c0 + threadIdx.x;
-            const uint32_t kk = a.key[e];
-            const int q = (int)(kk & QMASK);
-            const int rank = (int)(kk >> QBITS);
-            Cand& cd = cs[threadIdx.x];
-            const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
-            const double r = a.pr[q];
-            VoxArgs va;
-            for (int d = 0; d < 3; ++d) { va.n[d] = a.n[d]; va.per[d] = a.per[d]; va.nb[d] = a.nb[d]; }
-            va.auto_images = a.auto_images;
-            double xi[3];
-            int lo[3], hi[3];
-            if (!image_box(x, r, rank, va, lo, hi, xi)) {
-                lo[0] = lo[1] = lo[2] = 1; hi[0] = hi[1] = hi[2] = 0;
-                xi[0] = xi[1] = xi[2] = 0.0;
-            }
-            for (int d = 0; d < 3; ++d) {
-                cd.x[d] = xi[d]; cd.lo[d] = lo[d]; cd.hi[d] = hi[d];
-                cd.v[d] = a.pv[3 * q + d]; cd.w[d] = a.pw[3 * q + d];
-            }
-            cd.r = r;
-            const double r_p = r + a.sqrt3half;
-            const double r_m = r - a.sqrt3half;
-            cd.rp2 = r_p * r_p;
-            cd.rm2 = r_m * r_m;
-            cd.rsq = r * r;
-            cd.id = a.pid[q];
-            cd.e = e;
-        }
+        if ((int)threadIdx.x < nchunk) cs[threadIdx.x] = a.cand[cstart + c0 + threadIdx.x];
         __syncthreads();
         for (int j = 0; j < nchunk; ++j) {
             const Cand& cd = cs[j];
@@ -423,9 +410,9 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
             if (INITVEL && sf > SOLFRAC_MAX) {
                 dv = true;
                 if (kind == ST_OLD) {
-                    dvus[0] = a.oldr.us[oldslot];
-                    dvus[1] = a.oldr.us[a.rcap + oldslot];
-                    dvus[2] = a.oldr.us[2 * a.rcap + oldslot];
+                    dvus[0] = a.rec.us[ridx];
+                    dvus[1] = a.rec.us[a.rcap + ridx];
+                    dvus[2] = a.rec.us[2 * a.rcap + ridx];
                 } else {
                     dvus[0] = sus[0]; dvus[1] = sus[1]; dvus[2] = sus[2];
                 }
@@ -433,67 +420,36 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
         }
     }
 
-    /* compaction in cell order inside the bin: wave ballots + 8 wave totals */
     const bool keep = valid && kind != ST_ZERO;
     const unsigned long long m = __ballot(keep);
-    const int wave = threadIdx.x >> 6;
-    const int lane = (int)__lane_id();
-    const int before = (int)__popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) wave_tot[wave] = (int)__popcll(m);
-    __syncthreads();
-    int off = before, total = 0;
-    for (int w = 0; w < 8; ++w) {
-        const int t = wave_tot[w];
-        if (w < wave) off += t;
-        total += t;
-    }
-    if (threadIdx.x == 0) {
-        /* records are allocated from one of nseg segment counters, so that
-         * the workgroups do not all serialize on one address */
-        long long base = 0;
-        if (total > 0) {
-            const int sg = (int)(b % a.nseg);
-            const long long o = (long long)atomicAdd((unsigned long long*)&a.seg_count[sg], (unsigned long long)total);
-            base = (long long)sg * a.seg + o;
-            if (o + total > a.seg) {
-                atomicOr((unsigned long long*)&a.counters[1], 1ull);
-                base = -1;
-            }
-        }
-        s_base = base;
-        a.newr.bin_start[b] = base < 0 ? 0 : (int32_t)base;
-        a.newr.bin_count[b] = base < 0 ? 0 : total;
-    }
-    __syncthreads();
-    const long long base = s_base;
+    /* one atomic per wavefront on the bin's own counter (distinct addresses
+     * across bins; a single global counter would serialize every wave) */
+    if (__lane_id() == 0 && m) atomicAdd(&a.cnt_new[b], (uint32_t)__popcll(m));
     if (!valid) return;
-    if (!keep || base < 0) {
-        a.slot[cidx] = -1;
-    } else {
-        const int64_t idx = base + off;
-        if (kind == ST_OLD) {
-            sus[0] = a.oldr.us[oldslot];
-            sus[1] = a.oldr.us[a.rcap + oldslot];
-            sus[2] = a.oldr.us[2 * a.rcap + oldslot];
-            const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[2], 1ull);
-            if ((int64_t)p < a.rcap) a.orphan[p] = (int32_t)idx;
-        }
-        a.newr.sf[idx] = ssf;
-        a.newr.id[idx] = sid;
-        a.newr.us[idx] = sus[0];
-        a.newr.us[a.rcap + idx] = sus[1];
-        a.newr.us[2 * a.rcap + idx] = sus[2];
-        a.newr.cell[idx] = (int32_t)cidx;
-        a.newr.cand[idx] = se;
-        a.slot[cidx] = (int32_t)idx;
+    if (kind == ST_SET) {
+        a.rec.sf[ridx] = ssf;
+        a.rec.id[ridx] = sid;
+        a.rec.us[ridx] = sus[0];
+        a.rec.us[a.rcap + ridx] = sus[1];
+        a.rec.us[2 * a.rcap + ridx] = sus[2];
+        a.rec.cand[ridx] = se;
+    } else if (kind == ST_OLD) {
+        /* untouched this step: values stay, summed on the orphan path */
+        a.rec.cand[ridx] = -1;
+        const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[2], 1ull);
+        if ((int64_t)p < a.orphan_cap) a.orphan[p] = (int32_t)ridx;
+        else atomicOr((unsigned long long*)&a.counters[1], 4ull);
     }
+    if (keep != had) a.sflag[cidx] = keep ? 1 : 0;
     if (INITVEL && dv && a.dv_cell) {
         const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[5], 1ull);
-        if ((int64_t)p < a.rcap) {
+        if ((int64_t)p < a.orphan_cap) {
             a.dv_cell[p] = (int32_t)cidx;
             a.dv_us[3 * p] = dvus[0];
             a.dv_us[3 * p + 1] = dvus[1];
             a.dv_us[3 * p + 2] = dvus[2];
+        } else {
+            atomicOr((unsigned long long*)&a.counters[1], 8ull);
         }
     }
 }
@@ -552,7 +508,7 @@ int run_voxelize(lbdem_psm* h) {
     VoxArgs va = vox_args(h);
     /* counters[1] (overflow flags) is cleared by the public entry points only */
     PSM_HIP(h, hipMemsetAsync(h->counters, 0, sizeof(int64_t), h->stream));
-    PSM_HIP(h, hipMemsetAsync(h->counters + 2, 0, (6 + 64) * sizeof(int64_t), h->stream));
+    PSM_HIP(h, hipMemsetAsync(h->counters + 2, 0, 6 * sizeof(int64_t), h->stream));
     PSM_HIP(h, hipMemsetAsync(h->bin_cand_count, 0, nbins * sizeof(int32_t), h->stream));
     PSM_HIP(h, hipMemsetAsync(h->bin_fill, 0, nbins * sizeof(int32_t), h->stream));
     const unsigned pg = (unsigned)std::max<int64_t>(1, (h->np + 3) / 4);   /* a wave per particle */
@@ -569,32 +525,34 @@ int run_voxelize(lbdem_psm* h) {
     if (h->np > 0)
         hipLaunchKernelGGL(k_owner_entries, dim3(pg), dim3(256), 0, h->stream, va, h->bin_cand_start, h->bin_cand_count, h->cand_key, h->p_ent_start, h->owner_ent, h->ent_cap);
 
+    hipLaunchKernelGGL(k_cand_gather, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, (h->ent_cap + 255) / 256))),
+                       dim3(256), 0, h->stream, va, h->cand_key, h->pv, h->pw, h->pid, h->sqrt3half,
+                       h->bin_cand_start, h->bin_cand_count, nbins, h->cand_data, h->ent_cap);
+    uint32_t* cnt_new = h->bin_any[1 - h->cur_any];
+    PSM_HIP(h, hipMemsetAsync(cnt_new, 0, nbins * sizeof(uint32_t), h->stream));
+
     FoldArgs fa;
-    for (int d = 0; d < 3; ++d) { fa.n[d] = h->g.n[d]; fa.nb[d] = h->g.nb[d]; fa.per[d] = h->g.periodic[d]; }
+    for (int d = 0; d < 3; ++d) { fa.n[d] = h->g.n[d]; fa.nb[d] = h->g.nb[d]; }
     fa.N = h->desc.slices_per_dim;
     fa.sw = 1.0 / (double)fa.N;
     fa.frac = 1.0 / (double)(fa.N * fa.N * fa.N);
-    fa.sqrt3half = h->sqrt3half;
-    fa.init_vel = (h->flags & LBDEM_PSM_INIT_VEL) ? 1 : 0;
-    fa.auto_images = va.auto_images;
-    fa.px = h->px; fa.pv = h->pv; fa.pw = h->pw; fa.pr = h->pr; fa.pid = h->pid;
+    const bool init_vel = (h->flags & LBDEM_PSM_INIT_VEL) != 0;
     fa.bin_cand_start = h->bin_cand_start;
     fa.bin_cand_count = h->bin_cand_count;
-    fa.key = h->cand_key;
+    fa.cand = h->cand_data;
     fa.ent_cap = h->ent_cap;
-    fa.oldr = h->rec[h->cur_rec];
-    fa.newr = h->rec[1 - h->cur_rec];
+    fa.rec = h->rec;
     fa.rcap = h->rec_cap;
-    fa.slot = h->slot;
+    fa.sflag = h->sflag;
+    fa.cnt_old = h->bin_any[h->cur_any];
+    fa.cnt_new = cnt_new;
     fa.counters = h->counters;
     fa.orphan = h->orphan;
-    fa.dv_cell = fa.init_vel ? h->dv_cell : nullptr;
+    fa.orphan_cap = h->orphan_cap;
+    fa.dv_cell = init_vel ? h->dv_cell : nullptr;
     fa.dv_us = h->dv_us;
-    fa.seg_count = h->counters + 8;
-    fa.nseg = h->nseg;
-    fa.seg = h->rec_cap / h->nseg;
     dim3 grid((unsigned)nbins), block(512);
-    if (fa.init_vel) {
+    if (init_vel) {
         if (fa.N == 5) hipLaunchKernelGGL((k_fold<5, true>), grid, block, 0, h->stream, fa);
         else if (fa.N == 8) hipLaunchKernelGGL((k_fold<8, true>), grid, block, 0, h->stream, fa);
         else hipLaunchKernelGGL((k_fold<0, true>), grid, block, 0, h->stream, fa);
@@ -604,13 +562,34 @@ int run_voxelize(lbdem_psm* h) {
         else hipLaunchKernelGGL((k_fold<0, false>), grid, block, 0, h->stream, fa);
     }
     PSM_HIP(h, hipGetLastError());
-    h->cur_rec = 1 - h->cur_rec;
+    h->cur_any = 1 - h->cur_any;
     h->records_from_host = false;
     prof_end(h, 1, ev);
-    if (fa.init_vel) {
+    if (init_vel) {
         int rc = launch_define_velocity(h);
         if (rc) return rc;
     }
+    return LBDEM_PSM_OK;
+}
+
+/* Total live records: sum of the per-bin counts into counters[0]. */
+__global__ __launch_bounds__(256) void k_sum_bins(const uint32_t* __restrict__ cnt, int64_t nbins,
+                                                  int64_t* __restrict__ out) {
+    __shared__ unsigned long long sh[256];
+    unsigned long long v = 0;
+    for (int64_t b = threadIdx.x; b < nbins; b += blockDim.x) v += cnt[b];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (int64_t)sh[0];
+}
+
+int count_records(lbdem_psm* h) {
+    hipLaunchKernelGGL(k_sum_bins, dim3(1), dim3(256), 0, h->stream, h->bin_any[h->cur_any], h->nbins, h->counters);
+    PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }
 
