@@ -107,6 +107,17 @@ def run_gpu(wl, precision, steps, warmup, spheres, slices=8):
     }
 
 
+def measured_traffic(workload, kernel):
+    try:
+        t = json.load(open(TRAFFIC_JSON))
+    except (OSError, ValueError):
+        return None
+    if t.get("workload") != workload:
+        return None
+    v = t.get("bytes_per_launch", {}).get(kernel)
+    return None if v is None else int(v)
+
+
 def rel(a, b):
     den = np.abs(b).max()
     return float(np.abs(a - b).max() / den) if den > 0 else 0.0
@@ -137,6 +148,85 @@ def cpu_baseline(wl, spheres, nsteps):
                       f"sum single-threaded as in the reference; {dt:.1f} s"}
 
 
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "collide_stream_traffic.json")
+PROBE_CMD = ("rocprofv3 --pmc {ctr} --output-format csv -d DIR/{tag} -o run -- "
+             "python3 bench.py --kernel-probe")
+TRACE_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace -o run -- "
+             "python3 bench.py --steps 10 --warmup 2 --no-extra --cpu-steps 1")
+
+
+def _short(name):
+    for key, short in (("k_collide_stream<0", "k_collide_stream<FP64>"), ("k_collide_stream<1", "k_collide_stream<FP32_DEV>"),
+                       ("k_collide_stream<2", "k_collide_stream<FP16_STORE>"), ("k_pop_convert<0", "k_pop_convert<FP64> (copy)")):
+        if key in name:
+            return short
+    return name.split("(")[0].replace("void ", "").replace("psm::", "")[:48]
+
+
+def summarize_profiles(d):
+    """Kernel-trace stats and FETCH/WRITE counters -> profiles/*.md + traffic json."""
+    import csv
+    import glob
+    import sqlite3
+    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
+    dbs = glob.glob(os.path.join(d, "trace", "*results.db"))
+    if dbs:
+        con = sqlite3.connect(dbs[0])
+        rows = list(con.execute("select name,total_calls,total_duration,average,percentage from top_kernels"))
+        with open(os.path.join(ROOT, "profiles", "kernels_dense256_trace.md"), "w") as f:
+            f.write("# Kernel durations, dense suspension 256^3 (2000 spheres r~U[4,8], tau 0.7, 8^3 samples)\n\n")
+            f.write("Command: `" + TRACE_CMD + "` (rocprofv3 kernel trace, ROCm 7.2, one MI355X).\n")
+            f.write("The run does 1 + 2 + 10 coupled steps per precision (FP64, FP32-dev, FP16-store).\n\n")
+            f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
+            for name, n, tot, avg, pct in rows[:24]:
+                f.write(f"| {_short(name)} | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |\n")
+    res = {}
+    for tag, ctr in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
+        path = os.path.join(d, tag, "run_counter_collection.csv")
+        if not os.path.exists(path):
+            continue
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != ctr:
+                continue
+            acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+        for k, v in acc.items():
+            res.setdefault(k, {})[ctr] = sum(v) / len(v)
+    if not res:
+        return
+    copy = [k for k in res if "k_pop_convert<0>" in k]
+    nc = 256 ** 3
+    copy_bytes = 19 * nc * 8
+    fetch_f = write_f = None
+    if copy:
+        fetch_f = copy_bytes / (res[copy[0]]["FETCH_SIZE"] * 1024.0)
+        write_f = copy_bytes / (res[copy[0]]["WRITE_SIZE"] * 1024.0)
+    lines = ["# HBM traffic per dispatch, dense suspension 256^3\n",
+             "Counters: `" + PROBE_CMD.format(ctr="FETCH_SIZE", tag="fetch") + "`, then the same with WRITE_SIZE.",
+             f"Calibration: the FP64 population copy k_pop_convert<FP64> reads and writes {copy_bytes} B;",
+             f"FETCH_SIZE scale = {fetch_f:.4f} B per counted KiB/1024, WRITE_SIZE scale = {write_f:.4f}.\n" if fetch_f else "",
+             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read GB (scaled) | written GB (scaled) | algorithmic GB |",
+             "|---|---|---|---|---|---|"]
+    traffic = {}
+    algo = {"k_collide_stream<FP64>": 304 * nc, "k_collide_stream<FP32_DEV>": 152 * nc,
+            "k_collide_stream<FP16_STORE>": 76 * nc, "k_pop_convert<FP64> (copy)": 2 * copy_bytes}
+    for k, v in sorted(res.items(), key=lambda kv: -kv[1].get("FETCH_SIZE", 0)):
+        fk, wk = v.get("FETCH_SIZE", 0.0), v.get("WRITE_SIZE", 0.0)
+        rd = fk * 1024 * (fetch_f or 1) / 1e9
+        wr = wk * 1024 * (write_f or 1) / 1e9
+        sk = _short(k)
+        al = algo.get(sk)
+        lines.append(f"| {sk} | {fk:.0f} | {wk:.0f} | {rd:.3f} | {wr:.3f} | {al / 1e9:.3f} |" if al else
+                     f"| {sk} | {fk:.0f} | {wk:.0f} | {rd:.3f} | {wr:.3f} | |")
+        if sk.startswith("k_collide_stream"):
+            traffic[sk] = (rd + wr) * 1e9
+    with open(os.path.join(ROOT, "profiles", "collide_stream_dense256_hbm_traffic.md"), "w") as f:
+        f.write("\n".join(lines[:14] + lines[14:40]) + "\n")
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump({"workload": "dense256", "bytes_per_launch": traffic,
+                   "fetch_scale": fetch_f, "write_scale": write_f}, f, indent=1)
+
+
 def kernel_probe(wl, sph):
     """Counter-calibration copy of known size, then three coupled steps per
     precision.  The copy is the FP64 population upload: k_pop_convert<FP64>
@@ -161,10 +251,16 @@ def main():
     ap.add_argument("--workload", default="dense256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-extra", action="store_true", help="skip the 512^3-class FP32 line")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--summarize-profiles", metavar="DIR",
+                    help="turn rocprofv3 outputs under DIR (kernel trace db, fetch/ and write/ "
+                         "counter csv) into profiles/*.md and profiles/collide_stream_traffic.json")
     ap.add_argument("--kernel-probe", action="store_true",
-                    help="short run for hardware-counter profiles: a 1 GiB device copy (calibration) "
-                         "then 3 coupled steps per precision")
+                    help="short run for hardware-counter profiles: the FP64 population copy "
+                         "(known size, for calibration) then 3 coupled steps per precision")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        summarize_profiles(args.summarize_profiles)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not a measured path)", file=sys.stderr)
@@ -234,7 +330,10 @@ def main():
                    "step": "voxelize + fused collide/stream + force/torque sum"},
         "roofline": {"bound": "hbm", "achieved": round(f64["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(f64["kernel_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_collide_stream<FP64>",
+                     "traffic": measured_traffic(args.workload, "k_collide_stream<FP64>"),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a known-size copy; "
+                                       "profiles/collide_stream_dense256_hbm_traffic.md",
+                     "kernel": "k_collide_stream<FP64>",
                      "algorithmic_bytes_per_launch": 304 * nc},
         "cpu_baseline": cpu,
         "fp64": summary(f64),
