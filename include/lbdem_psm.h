@@ -43,6 +43,10 @@ extern "C" {
 #define LBDEM_PSM_FP32_DEV 1    /* f - t in FP32, FP32 arithmetic */
 #define LBDEM_PSM_FP16_STORE 2  /* f - t stored in FP16, FP32 arithmetic */
 
+/* Voxelization rule (lbdem_psm_desc.voxel_mode) */
+#define LBDEM_PSM_VOXEL_SUBCELL 0  /* calcSolidFraction sub-cell count (the reference) */
+#define LBDEM_PSM_VOXEL_OVERLAP 1  /* analytic sphere-cell overlap, differentiable in x */
+
 /* lbdem_psm_set_spheres flags */
 #define LBDEM_PSM_INIT_VEL 1    /* initVelFlag of setSpheresOnLattice */
 #define LBDEM_PSM_AUTO_IMAGES 2 /* library appends periodic images (no ghosts given) */
@@ -60,11 +64,12 @@ typedef struct lbdem_psm_desc {
     int32_t slices_per_dim;      /* sub-cell samples per axis (reference: 5) */
     double body_accel[3];        /* Guo body-force acceleration, lattice units */
     int32_t device;              /* HIP device ordinal */
-    int32_t reserved;
+    int32_t voxel_mode;          /* LBDEM_PSM_VOXEL_SUBCELL / _OVERLAP */
     int64_t max_solid_cells;     /* optional limit on live coupling records, 0 = none */
 } lbdem_psm_desc;
 
-/* Defaults: 1x1x1, periodic, tau 1, FP64, 5 slices, no force, device 0. */
+/* Defaults: 1x1x1, periodic, tau 1, FP64, 5 slices, no force, device 0,
+ * sub-cell count voxelization. */
 int lbdem_psm_desc_init(lbdem_psm_desc* desc);
 
 /* Creates the lattice (populations at rest, rho = 1) -- the Palabos
@@ -147,6 +152,41 @@ int lbdem_psm_synchronize(lbdem_psm* h);
  * 3 advance.  read: total ms and launch count per class; reset clears. */
 int lbdem_psm_profile(lbdem_psm* h, int32_t enable);
 int lbdem_psm_profile_read(lbdem_psm* h, double* ms4, int64_t* count4, int32_t reset);
+
+/* ---- Differentiable window (no counterpart in the reference) ----
+ *
+ * A window of nsteps collide+stream+force-sum steps over frozen geometry:
+ * the coupling records of the last set_spheres stay fixed while the
+ * populations evolve (prescribed particle velocities act as moving-wall
+ * data).  Objective J = sum_t sum_p g_ft[t][p] . (F_p(t), T_p(t))
+ * + <g_fT, f(T)>.  The backward pass is the hand-written adjoint of the
+ * fused PSM collide-stream and of the force/torque sum; populations are
+ * checkpointed every checkpoint_every steps and recomputed per segment.
+ * Gradients: particle velocities v, angular velocities w, positions x
+ * (through the analytic overlap when voxel_mode = OVERLAP, plus the
+ * rigid-body lever arms and the torque arms in either mode), tau, and the
+ * per-cell solid fraction sf and weight B.  FP64 and FP32_DEV storage. */
+
+/* Changes tau (omega = 1/tau) of an existing lattice. */
+int lbdem_psm_set_tau(lbdem_psm* h, double tau);
+/* Removes every coupling record (fresh ownership history). */
+int lbdem_psm_clear_records(lbdem_psm* h);
+/* sf += dsf[c] on every cell with a live record (dsf: Nc, dense).  The
+ * result must stay in (0.001, 1]; dsf must be 0 on cells without a record. */
+int lbdem_psm_perturb_solid_fraction(lbdem_psm* h, const double* dsf);
+/* Runs the window from the current populations; ft (may be NULL):
+ * nsteps*n*6 doubles, (F, T) per particle per step, folded onto map(id)
+ * like sum_force_torque.  Leaves the lattice at step nsteps. */
+int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t checkpoint_every, double* ft);
+/* Adjoint of the last diff_forward.  g_ft: nsteps*n*6 (may be NULL = 0);
+ * g_fT: 19*Nc host populations layout (may be NULL = 0).  Outputs (each may
+ * be NULL): g_x3, g_v3, g_w3 (3*n, on map(id), ghost slots 0), g_tau,
+ * g_sf and g_B (Nc, dense, 0 where no partial record), g_f0 (19*Nc). */
+int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT, double* g_x3,
+                            double* g_v3, double* g_w3, double* g_tau, double* g_sf, double* g_B,
+                            double* g_f0);
+/* Frees the checkpoints and adjoint buffers. */
+int lbdem_psm_diff_release(lbdem_psm* h);
 
 /* Host-side argument check of set_spheres, no device needed. */
 int lbdem_psm_check_spheres(const lbdem_psm_desc* desc, int64_t n, const double* x3,

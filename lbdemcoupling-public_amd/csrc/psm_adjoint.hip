@@ -1,0 +1,892 @@
+/*
+ * Adjoint of the fused PSM collide+stream and of the force/torque sum on
+ * gfx950: the backward pass of a differentiable window over frozen
+ * geometry (the coupling records of the last voxelization stay fixed while
+ * the populations evolve).
+ *
+ * Forward step t (per cell, src/ibCompositeDynamics3D.hh:130-203 plus push
+ * streaming):  f(t+1)[x + c_i, i] = C_i(f(t)[x]; omega, B, us)
+ * and the per-cell force F(x) that SumForceTorque3D
+ * (src/ibProcessors3D.hh:202-275) adds into (F_p, T_p = sum d x F).
+ * Backward step t: lambda*(x, i) = lambda(t+1)[x + c_i, i] (the transpose of
+ * the push is a pull; at a wall the bounce-back is undone the same way),
+ * then lambda(t)[x] = (dC/df)^T lambda* + (dF/df)^T gF with
+ * gF = dJ/dF_p + dJ/dT_p x d, and the parameter adjoints are accumulated per
+ * record slot (B, us, F x dJ/dT_p) and per workgroup (omega).  The
+ * reverse-mode formulas are written out by hand below; nothing is taped.
+ *
+ * Memory: the window's populations are checkpointed every K steps in the
+ * forward pass; the backward pass recomputes one segment (K - 1 states)
+ * from its checkpoint and runs the adjoint steps of the segment backwards.
+ */
+#include "psm_collide_kernel.h"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace psm {
+
+/* ---- reverse-mode pieces of the cell update ---- */
+
+/* Adjoint accumulators of one set of equilibria
+ * eq_i = W_i ((rhoBar + 3 c.j) + 1.5 invRho (3 (c.j)^2 - |j|^2)):
+ * with e_i the adjoint of eq_i and w_i = W_i e_i,
+ *   d/d rhoBar = sum w_i,  d/d invRho = 1.5 sum w_i (3 (c.j)^2 - |j|^2),
+ *   d/d j = 3 sum w_i c_i + 9 invRho sum w_i (c.j) c_i - 3 invRho j sum w_i. */
+template <typename T>
+struct EqBar {
+    T sW, sQ, sC[3], sCJ[3];
+    __device__ __forceinline__ void zero() {
+        sW = T(0); sQ = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) { sC[d] = T(0); sCJ[d] = T(0); }
+    }
+    __device__ __forceinline__ void add(int i, T e, T cj, T jSqr) {
+        const T we = wgt<T>(i) * e;
+        sW += we;
+        sQ += we * (T(3) * cj * cj - jSqr);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (cc(i, d) == 1) { sC[d] += we; sCJ[d] += we * cj; }
+            else if (cc(i, d) == -1) { sC[d] -= we; sCJ[d] -= we * cj; }
+        }
+    }
+    __device__ __forceinline__ void j_bar(T invRho, const T* j, T* out) const {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[d] = T(3) * sC[d] + T(9) * invRho * sCJ[d] - T(3) * invRho * j[d] * sW;
+    }
+};
+
+/* Adjoint of the base dynamics fb = BGK(fP) (Guo forcing when FORCE):
+ * fPbar += (d fb / d fP)^T gb, omega_bar += (d fb / d omega) . gb. */
+template <typename T, bool FORCE>
+__device__ __forceinline__ void bgk_vjp(const T* fP, const T* gb, T omega, const T* acc, T* fPbar,
+                                        T& omega_bar) {
+    T rhoBar, j[3];
+    moments<T>(fP, rhoBar, j);
+    const T rho = rhoBar + T(1);
+    const T invRho = T(1) / rho;
+    T jE[3] = {j[0], j[1], j[2]};
+    T u[3] = {T(0), T(0), T(0)};
+    if (FORCE) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            u[d] = j[d] * invRho + T(0.5) * acc[d];
+            jE[d] = rho * u[d];
+        }
+    }
+    const T jSqr = norm_sqr<T>(jE);
+    const T p = T(1.5) * invRho;
+    const T omm = T(1) - omega;
+    EqBar<T> eb;
+    eb.zero();
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const T eq = equilibrium<T>(i, rhoBar, p, jE, jSqr);
+        fPbar[i] += omm * gb[i];
+        omega_bar += gb[i] * (eq - fP[i]);
+        eb.add(i, omega * gb[i], cdot<T>(i, jE), jSqr);
+    }
+    T rb_bar = eb.sW;
+    T ir_bar = T(1.5) * eb.sQ;
+    T jE_bar[3];
+    eb.j_bar(invRho, jE, jE_bar);
+    T j_bar[3];
+    if (FORCE) {
+        /* fb_i += pref W_i G_i, pref = (1 - omega/2) rho,
+         * G_i = 3 (c_i.a - u.a) + 9 (c_i.u)(c_i.a) */
+        const T pref = (T(1) - T(0.5) * omega) * rho;
+        const T ua = (u[0] * acc[0] + u[1] * acc[1]) + u[2] * acc[2];
+        T sg = T(0), sgw = T(0), sga[3] = {T(0), T(0), T(0)};
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const T ca = cdot<T>(i, acc);
+            const T cu = cdot<T>(i, u);
+            const T G = T(3) * (ca - ua) + T(9) * (cu * ca);
+            const T wg = wgt<T>(i) * gb[i];
+            sg += wg * G;
+            sgw += wg;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (cc(i, d) == 1) sga[d] += wg * ca;
+                else if (cc(i, d) == -1) sga[d] -= wg * ca;
+            }
+        }
+        omega_bar += T(-0.5) * rho * sg;
+        rb_bar += (T(1) - T(0.5) * omega) * sg;
+        T u_bar[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) u_bar[d] = pref * (T(-3) * acc[d] * sgw + T(9) * sga[d]);
+        /* jE = rho u */
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            rb_bar += jE_bar[d] * u[d];
+            u_bar[d] += rho * jE_bar[d];
+        }
+        /* u = j invRho + a/2 */
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            j_bar[d] = invRho * u_bar[d];
+            ir_bar += u_bar[d] * j[d];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) j_bar[d] = jE_bar[d];
+    }
+    rb_bar += -(invRho * invRho) * ir_bar;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fPbar[i] += rb_bar + cdot<T>(i, j_bar);
+}
+
+/* Adjoint of the PSM update of a cell with a record.  Forward
+ * (src/ibCompositeDynamics3D.hh:146-201), with fb = BGK(fP) for a partial
+ * cell and fb = fP, B = 1 for a full one:
+ *   out_0 = fP_0 + (1 - B)(fb_0 - fP_0) + B (E_0 - S_0)
+ *   out_i = fP_i + (1 - B)(fb_i - fP_i) + B D_i,  i = 1..18,
+ *   D_i = (fP_o - E_o) - (fP_i - S_i), o = opposite of i,
+ *   F = -sum_{i=1..9} c_i B (D_i - D_{i+9}),
+ * E = eq(rho(fb), j(fb)), S = eq(rho(fb), us (1 + rhoBar(fb))).
+ * lam: adjoint of out; gF: adjoint of F.  Accumulates fPbar, omega_bar,
+ * B_bar, us_bar; returns the recomputed F. */
+template <typename T, bool FORCE>
+__device__ __forceinline__ void psm_vjp(const T* fP, const T* lam, const T* gF, bool full, T B,
+                                        const T* us, T omega, const T* acc, T* fPbar, T& omega_bar,
+                                        T& B_bar, T* us_bar, T* F) {
+    T fb[Q];
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) fb[i] = fP[i];
+    } else {
+        bgk_all<T, FORCE>(fP, fb, omega, acc);
+    }
+    T rhoB, jB[3];
+    moments<T>(fb, rhoB, jB);
+    const T invRho = T(1) / (rhoB + T(1));
+    const T jSqr = norm_sqr<T>(jB);
+    const T p = T(1.5) * invRho;
+    const T onePlus = T(1) + rhoB;
+    T uP[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) uP[d] = us[d] * onePlus;
+    const T uPSqr = norm_sqr<T>(uP);
+    const T oneMinB = T(1) - B;
+
+    EqBar<T> e1, e2;
+    e1.zero();
+    e2.zero();
+    T fbbar[Q];
+    T Bb = T(0);
+    F[0] = F[1] = F[2] = T(0);
+    {
+        const T E0 = eq_rest<T>(rhoB, p, jSqr), S0 = eq_rest<T>(rhoB, p, uPSqr);
+        const T D0 = E0 - S0;
+        fPbar[0] += B * lam[0];
+        fbbar[0] = oneMinB * lam[0];
+        Bb += lam[0] * (fP[0] - fb[0]) + lam[0] * D0;
+        const T c0 = B * lam[0];
+        e1.add(0, c0, T(0), jSqr);
+        e2.add(0, -c0, T(0), uPSqr);
+    }
+#pragma unroll
+    for (int i = 1; i <= 9; ++i) {
+        const int o = i + 9;
+        T Ei, Eo, Si, So;
+        eq_pair<T>(i, rhoB, p, jB, jSqr, Ei, Eo);
+        eq_pair<T>(i, rhoB, p, uP, uPSqr, Si, So);
+        const T Di = (fP[o] - Eo) - (fP[i] - Si);
+        const T Do = (fP[i] - Ei) - (fP[o] - So);
+        const T diff = B * Di - B * Do;
+        T s = T(0);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            if (cc(i, d) == 1) { F[d] -= diff; s += gF[d]; }
+            else if (cc(i, d) == -1) { F[d] += diff; s -= gF[d]; }
+        }
+        const T Dbi = lam[i] - s, Dbo = lam[o] + s;
+        fPbar[i] += B * lam[i];
+        fPbar[o] += B * lam[o];
+        fbbar[i] = oneMinB * lam[i];
+        fbbar[o] = oneMinB * lam[o];
+        Bb += (lam[i] * (fP[i] - fb[i]) + lam[o] * (fP[o] - fb[o])) + (Dbi * Di + Dbo * Do);
+        const T ci = B * Dbi, co = B * Dbo;
+        fPbar[i] += co - ci;
+        fPbar[o] += ci - co;
+        const T cjB = cdot<T>(i, jB), cjS = cdot<T>(i, uP);
+        e1.add(i, -co, cjB, jSqr);
+        e1.add(o, -ci, -cjB, jSqr);
+        e2.add(i, ci, cjS, uPSqr);
+        e2.add(o, co, -cjS, uPSqr);
+    }
+    T rb_bar = e1.sW + e2.sW;
+    const T ir_bar = T(1.5) * (e1.sQ + e2.sQ);
+    T jB_bar[3], uP_bar[3];
+    e1.j_bar(invRho, jB, jB_bar);
+    e2.j_bar(invRho, uP, uP_bar);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        us_bar[d] += onePlus * uP_bar[d];
+        rb_bar += us[d] * uP_bar[d];
+    }
+    rb_bar += -(invRho * invRho) * ir_bar;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fbbar[i] += rb_bar + cdot<T>(i, jB_bar);
+    if (full) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) fPbar[i] += fbbar[i];
+    } else {
+        B_bar += Bb;
+        bgk_vjp<T, FORCE>(fP, fbbar, omega, acc, fPbar, omega_bar);
+    }
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+    return v;
+}
+
+/* Fixed-order sum over a 256-thread block; the result is valid in thread 0. */
+__device__ __forceinline__ double block_sum_256(double v, double* sh) {
+    v = wave_sum_d(v);
+    const int wave = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[wave] = v;
+    __syncthreads();
+    return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+struct AdjArgs {
+    int nx, ny, nz;
+    int px, py, pz;
+    int64_t nzp;
+    double omega, ooo;
+    double acc[3];
+    const uint8_t* sflag;
+    int nb1, nb2;
+    const double* rsf;
+    const double* rus;
+    const int32_t* rid;
+    int64_t rcap;
+    const double* gft;          /* 6 per particle for this step, or null */
+    const int32_t* id_map;
+    int64_t id_map_size;
+    const double* xs;           /* positions of the torque arms */
+    double* gacc;               /* 7 per record slot (SoA): gB, gus[3], F x gT[3] */
+    double* opart;              /* one omega-gradient partial per workgroup */
+};
+
+/* One backward step: lout = adjoint of f(t) from lin = adjoint of f(t+1).
+ * Same grid as the forward kernel: blockIdx.x = row, blockIdx.y = z chunk. */
+template <int PREC, bool FORCE>
+__global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* __restrict__ fin,
+                                                 const typename Store<PREC>::S* __restrict__ lin,
+                                                 typename Store<PREC>::S* __restrict__ lout, AdjArgs a) {
+    typedef typename Store<PREC>::T T;
+    __shared__ double sh[4];
+    const int row = blockIdx.x;
+    const int z = blockIdx.y * blockDim.x + threadIdx.x;
+    const bool active = z < a.nz;
+    const int x = row / a.ny;
+    const int y = row - x * a.ny;
+    double wbar = 0.0;
+    if (active) {
+        const T omega = T(a.omega);
+        const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
+        T lam[Q], fP[Q], fPbar[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            int xd = x + cx(i), yd = y + cy(i), zd = z + cz(i);
+            bool wall = false;
+            if (xd < 0 || xd >= a.nx) { if (a.px) xd = xd < 0 ? xd + a.nx : xd - a.nx; else wall = true; }
+            if (yd < 0 || yd >= a.ny) { if (a.py) yd = yd < 0 ? yd + a.ny : yd - a.ny; else wall = true; }
+            if (zd < 0 || zd >= a.nz) { if (a.pz) zd = zd < 0 ? zd + a.nz : zd - a.nz; else wall = true; }
+            if (wall) lam[i] = Store<PREC>::load(lin + pop_at(row, opp(i), z, a.nzp));
+            else lam[i] = Store<PREC>::load(lin + pop_at((int64_t)xd * a.ny + yd, i, zd, a.nzp));
+            fP[i] = Store<PREC>::load(fin + pop_at(row, i, z, a.nzp));
+            fPbar[i] = T(0);
+        }
+        T om = T(0);
+        const int64_t c = (int64_t)row * a.nz + z;
+        if (!a.sflag[c]) {
+            bgk_vjp<T, FORCE>(fP, lam, omega, acc, fPbar, om);
+        } else {
+            const int64_t r = rec_index(x, y, z, a.nb1, a.nb2);
+            const double sf = a.rsf[r];
+            const bool full = sf > SOLFRAC_MAX;
+            const T B = full ? T(1) : T((sf * a.ooo) / ((1.0 - sf) + a.ooo));
+            const T us[3] = {T(a.rus[r]), T(a.rus[a.rcap + r]), T(a.rus[2 * a.rcap + r])};
+            T gF[3] = {T(0), T(0), T(0)};
+            double gT[3] = {0.0, 0.0, 0.0};
+            bool obj = false;
+            if (a.gft) {
+                const int id = a.rid[r];
+                const int o = (id >= 1 && id < a.id_map_size) ? a.id_map[id] : -1;
+                if (o >= 0) {
+                    obj = true;
+                    const double d[3] = {min_image((double)x - a.xs[3 * o], a.nx),
+                                         min_image((double)y - a.xs[3 * o + 1], a.ny),
+                                         min_image((double)z - a.xs[3 * o + 2], a.nz)};
+                    const double* g = a.gft + 6 * (int64_t)o;
+                    gT[0] = g[3]; gT[1] = g[4]; gT[2] = g[5];
+                    /* d(gT . (d x F))/dF = gT x d */
+                    gF[0] = T(g[0] + (gT[1] * d[2] - gT[2] * d[1]));
+                    gF[1] = T(g[1] + (gT[2] * d[0] - gT[0] * d[2]));
+                    gF[2] = T(g[2] + (gT[0] * d[1] - gT[1] * d[0]));
+                }
+            }
+            T Bb = T(0), usb[3] = {T(0), T(0), T(0)}, F[3];
+            psm_vjp<T, FORCE>(fP, lam, gF, full, B, us, omega, acc, fPbar, om, Bb, usb, F);
+            double* g = a.gacc;
+            if (!full) g[r] += (double)Bb;
+            g[a.rcap + r] += (double)usb[0];
+            g[2 * a.rcap + r] += (double)usb[1];
+            g[3 * a.rcap + r] += (double)usb[2];
+            if (obj) {
+                const double Fx = F[0], Fy = F[1], Fz = F[2];
+                g[4 * a.rcap + r] += Fy * gT[2] - Fz * gT[1];
+                g[5 * a.rcap + r] += Fz * gT[0] - Fx * gT[2];
+                g[6 * a.rcap + r] += Fx * gT[1] - Fy * gT[0];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < Q; ++i) Store<PREC>::store(lout + pop_at(row, i, z, a.nzp), fPbar[i]);
+        wbar = (double)om;
+    }
+    const double s = block_sum_256(wbar, sh);
+    if (threadIdx.x == 0) a.opart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] += s;
+}
+
+/* ---- reductions of the accumulated adjoints ---- */
+
+__global__ __launch_bounds__(256) void k_sum_vec(const double* __restrict__ in, int64_t n,
+                                                 double* __restrict__ out) {
+    __shared__ double sh[4];
+    double v = 0.0;
+    for (int64_t k = threadIdx.x; k < n; k += blockDim.x) v += in[k];
+    const double s = block_sum_256(v, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
+/* Per record slot: dense dJ/dB and dJ/dsf (partial cells; B = sf o / ((1 -
+ * sf) + o), o = tau - 1/2) and the tau partial sum_r dJ/dB dB/do. */
+__global__ __launch_bounds__(256) void k_grad_cells(int n0, int n1, int n2, int nb1, int nb2,
+                                                    int64_t rcap, const uint8_t* __restrict__ sflag,
+                                                    const double* __restrict__ rsf,
+                                                    const double* __restrict__ gacc, double ooo,
+                                                    double* __restrict__ cell_out, int64_t nc,
+                                                    double* __restrict__ tau_part) {
+    __shared__ double sh[4];
+    double tp = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rcap;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = r >> 9;
+        const int l = (int)(r & 511);
+        const int X = (int)(b / ((int64_t)nb1 * nb2)) * BIN + (l >> 6);
+        const int Y = (int)((b / nb2) % nb1) * BIN + ((l >> 3) & 7);
+        const int Z = (int)(b % nb2) * BIN + (l & 7);
+        if (X >= n0 || Y >= n1 || Z >= n2) continue;
+        const int64_t c = ((int64_t)X * n1 + Y) * n2 + Z;
+        double gB = 0.0, gsf = 0.0;
+        if (sflag[c]) {
+            const double sf = rsf[r];
+            if (!(sf > SOLFRAC_MAX)) {
+                gB = gacc[r];
+                const double den = (1.0 - sf) + ooo;
+                gsf = gB * (ooo * (1.0 + ooo)) / (den * den);
+                tp += gB * (sf * (1.0 - sf)) / (den * den);
+            }
+        }
+        cell_out[c] = gsf;
+        cell_out[nc + c] = gB;
+    }
+    const double s = block_sum_256(tp, sh);
+    if (threadIdx.x == 0) tau_part[blockIdx.x] = s;
+}
+
+struct GradArgs {
+    int n[3];
+    int nb[3];
+    const uint8_t* sflag;
+    const double* rsf;
+    const int32_t* rcand;
+    const int32_t* rid;
+    int64_t rcap;
+    const double* gacc;
+    const double* xs;
+    const double* pw;
+    const double* pr;
+    const int32_t* powner;
+    const int32_t* p_ent_start;
+    const int32_t* p_nent;
+    const int32_t* owner_ent;
+    const int32_t* cand_bin;
+    int64_t ent_cap;
+    double ooo;
+    double rc, vc;
+    int overlap;
+    double* gpart;              /* 9 per particle: g_v[3], g_w[3], g_x[3] */
+};
+
+/* Contribution of record r of the cell (X, Y, Z) owned by particle o. */
+__device__ __forceinline__ void record_grad(const GradArgs& a, int64_t r, int X, int Y, int Z, int o,
+                                            double* acc) {
+    const double d[3] = {min_image((double)X - a.xs[3 * o], a.n[0]),
+                         min_image((double)Y - a.xs[3 * o + 1], a.n[1]),
+                         min_image((double)Z - a.xs[3 * o + 2], a.n[2])};
+    const double g[3] = {a.gacc[a.rcap + r], a.gacc[2 * a.rcap + r], a.gacc[3 * a.rcap + r]};
+    const double fxt[3] = {a.gacc[4 * a.rcap + r], a.gacc[5 * a.rcap + r], a.gacc[6 * a.rcap + r]};
+    const double w[3] = {a.pw[3 * o], a.pw[3 * o + 1], a.pw[3 * o + 2]};
+    /* us = v + w x d, d = cell - x */
+    acc[0] += g[0];
+    acc[1] += g[1];
+    acc[2] += g[2];
+    acc[3] += d[1] * g[2] - d[2] * g[1];
+    acc[4] += d[2] * g[0] - d[0] * g[2];
+    acc[5] += d[0] * g[1] - d[1] * g[0];
+    /* d/dx: w x g (lever arm) - F x gT (torque arm) */
+    double gx[3] = {(w[1] * g[2] - w[2] * g[1]) - fxt[0], (w[2] * g[0] - w[0] * g[2]) - fxt[1],
+                    (w[0] * g[1] - w[1] * g[0]) - fxt[2]};
+    if (a.overlap) {
+        const double sf = a.rsf[r];
+        if (!(sf > SOLFRAC_MAX)) {
+            const double den = (1.0 - sf) + a.ooo;
+            const double gsf = a.gacc[r] * (a.ooo * (1.0 + a.ooo)) / (den * den);
+            const double dist = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+            double der = 0.0;
+            int br;
+            (void)overlap_fraction(dist, a.pr[o], a.rc, a.vc, &der, &br);
+            if (dist > 0.0 && der != 0.0) {
+                /* d dist / d x = -d / dist */
+                const double k = -gsf * der / dist;
+                gx[0] += k * d[0];
+                gx[1] += k * d[1];
+                gx[2] += k * d[2];
+            }
+        }
+    }
+    acc[6] += gx[0];
+    acc[7] += gx[1];
+    acc[8] += gx[2];
+}
+
+/* One workgroup per particle: the records its candidate entries set, in
+ * entry order, then a fixed-order block sum. */
+__global__ __launch_bounds__(256) void k_grad_particle(GradArgs a) {
+    __shared__ double sh[4];
+    const int64_t q = blockIdx.x;
+    const int o = a.powner[q];
+    const int64_t s = a.p_ent_start[q];
+    const int ne = a.p_nent[q];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < ne; ++k) {
+        if (s + k >= a.ent_cap) break;
+        const int e = a.owner_ent[s + k];
+        if (e < 0) continue;
+        const int64_t b = a.cand_bin[e];
+        const int bz = (int)(b % a.nb[2]);
+        const int by = (int)((b / a.nb[2]) % a.nb[1]);
+        const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
+        for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) {
+            const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
+            if (X >= a.n[0] || Y >= a.n[1] || Z >= a.n[2]) continue;
+            if (!a.sflag[((int64_t)X * a.n[1] + Y) * a.n[2] + Z]) continue;
+            const int64_t r = b * 512 + l;
+            if (a.rcand[r] != e) continue;
+            record_grad(a, r, X, Y, Z, o, acc);
+        }
+    }
+    for (int m = 0; m < 9; ++m) {
+        const double v = block_sum_256(acc[m], sh);
+        if (threadIdx.x == 0) a.gpart[9 * q + m] = v;
+    }
+}
+
+/* Records no candidate set in the last voxelization (kept from earlier
+ * steps), added with FP64 atomics onto map(id). */
+__global__ __launch_bounds__(256) void k_grad_orphans(GradArgs a, const int32_t* __restrict__ orphan,
+                                                      const int64_t* __restrict__ counters, int64_t ocap,
+                                                      const int32_t* __restrict__ id_map,
+                                                      int64_t id_map_size) {
+    const int64_t n = counters[2] < ocap ? counters[2] : ocap;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = orphan[k];
+        const int id = a.rid[r];
+        if (id < 1 || id >= id_map_size) continue;
+        const int o = id_map[id];
+        if (o < 0) continue;
+        const int64_t b = r >> 9;
+        const int l = (int)(r & 511);
+        const int bz = (int)(b % a.nb[2]);
+        const int by = (int)((b / a.nb[2]) % a.nb[1]);
+        const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
+        const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
+        if (X >= a.n[0] || Y >= a.n[1] || Z >= a.n[2]) continue;
+        if (!a.sflag[((int64_t)X * a.n[1] + Y) * a.n[2] + Z]) continue;
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        record_grad(a, r, X, Y, Z, o, acc);
+        for (int m = 0; m < 9; ++m) atomicAdd(&a.gpart[9 * (int64_t)o + m], acc[m]);
+    }
+}
+
+/* sf += dsf on live records; flags counters[1] bit 16 when a result leaves
+ * (SOLFRAC_MIN, 1] or dsf is non-zero on a cell without a record. */
+__global__ __launch_bounds__(256) void k_perturb_sf(int n0, int n1, int n2, int nb1, int nb2,
+                                                    const uint8_t* __restrict__ sflag,
+                                                    double* __restrict__ rsf,
+                                                    const double* __restrict__ dsf,
+                                                    int64_t* __restrict__ counters) {
+    const int64_t nc = (int64_t)n0 * n1 * n2;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const double d = dsf[c];
+        if (d == 0.0) continue;
+        if (!sflag[c]) {
+            atomicOr((unsigned long long*)&counters[1], 16ull);
+            continue;
+        }
+        const int Z = (int)(c % n2);
+        const int Y = (int)((c / n2) % n1);
+        const int X = (int)(c / ((int64_t)n2 * n1));
+        const int64_t r = rec_index(X, Y, Z, nb1, nb2);
+        const double v = rsf[r] + d;
+        if (!(v > SOLFRAC_MIN && v <= 1.0)) atomicOr((unsigned long long*)&counters[1], 16ull);
+        else rsf[r] = v;
+    }
+}
+
+/* ---- host side ---- */
+
+template <typename P>
+static hipError_t dalloc_d(P** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    return hipMalloc((void**)p, bytes);
+}
+
+static void dfree_d(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+static void free_states(std::vector<void*>& v) {
+    for (void* p : v) dfree_d(p);
+    v.clear();
+}
+
+int diff_free(lbdem_psm* h) {
+    auto& D = h->diff;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    free_states(D.ckpt);
+    free_states(D.seg);
+    dfree_d(D.lam[0]); dfree_d(D.lam[1]);
+    D.lam[0] = D.lam[1] = nullptr;
+    dfree_d(D.d_ft); dfree_d(D.gacc); dfree_d(D.Fscr); dfree_d(D.opart); dfree_d(D.gpart);
+    dfree_d(D.cell_out); dfree_d(D.tau_part);
+    D.d_ft = D.gacc = D.Fscr = D.opart = D.gpart = D.cell_out = D.tau_part = nullptr;
+    D.opart_n = D.tau_part_n = 0;
+    D.ready = false;
+    return LBDEM_PSM_OK;
+}
+
+static int ensure_states(lbdem_psm* h, std::vector<void*>& v, size_t n) {
+    while (v.size() > n) { dfree_d(v.back()); v.pop_back(); }
+    while (v.size() < n) {
+        void* p = nullptr;
+        hipError_t e = dalloc_d(&p, h->pop_bytes);
+        if (e != hipSuccess) return hip_check(h, e, "hipMalloc (window states)");
+        v.push_back(p);
+    }
+    return LBDEM_PSM_OK;
+}
+
+static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lout, const double* gft) {
+    AdjArgs a;
+    a.nx = h->g.n[0]; a.ny = h->g.n[1]; a.nz = h->g.n[2];
+    a.px = h->g.periodic[0]; a.py = h->g.periodic[1]; a.pz = h->g.periodic[2];
+    a.nzp = h->g.nzp;
+    a.omega = h->omega; a.ooo = h->ooo;
+    for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
+    a.sflag = h->sflag;
+    a.nb1 = h->g.nb[1]; a.nb2 = h->g.nb[2];
+    a.rsf = h->rec.sf; a.rus = h->rec.us; a.rid = h->rec.id; a.rcap = h->rec_cap;
+    a.gft = gft;
+    a.id_map = h->id_map; a.id_map_size = h->id_map_size;
+    a.xs = h->psum_x;
+    a.gacc = h->diff.gacc;
+    a.opart = h->diff.opart;
+    dim3 grid((unsigned)(a.nx * a.ny), (unsigned)((a.nz + 255) / 256)), block(256);
+    if (h->precision == LBDEM_PSM_FP64) {
+        typedef double S;
+        if (h->has_force) hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+        else hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+    } else {
+        typedef float S;
+        if (h->has_force) hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+        else hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+    }
+    PSM_HIP(h, hipGetLastError());
+    return LBDEM_PSM_OK;
+}
+
+}  // namespace psm
+
+using namespace psm;
+
+extern "C" {
+
+int lbdem_psm_set_tau(lbdem_psm* h, double tau) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    if (!(tau > 0.5) || !std::isfinite(tau))
+        return set_err(h, LBDEM_PSM_ERR_NUMERIC, "tau must be finite and > 0.5 (omega < 2)");
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    h->desc.tau = tau;
+    h->omega = 1.0 / tau;
+    h->ooo = 1.0 / h->omega - 0.5;
+    h->geom_gen += 1;
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_clear_records(lbdem_psm* h) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    PSM_HIP(h, hipMemsetAsync(h->sflag, 0, h->g.nc * sizeof(uint8_t), h->stream));
+    for (int k = 0; k < 2; ++k) PSM_HIP(h, hipMemsetAsync(h->bin_any[k], 0, h->nbins * sizeof(uint32_t), h->stream));
+    PSM_HIP(h, hipMemsetAsync(h->bin_cand_count, 0, h->nbins * sizeof(int32_t), h->stream));
+    if (h->p_cap > 0) PSM_HIP(h, hipMemsetAsync(h->p_nent, 0, h->p_cap * sizeof(int32_t), h->stream));
+    PSM_HIP(h, hipMemsetAsync(h->counters, 0, 8 * sizeof(int64_t), h->stream));
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    h->geom_gen += 1;
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_perturb_solid_fraction(lbdem_psm* h, const double* dsf) {
+    if (!h || !dsf) return LBDEM_PSM_ERR_INVALID_ARG;
+    const int64_t nc = h->g.nc;
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    double* d = nullptr;
+    PSM_HIP(h, dalloc_d(&d, nc * sizeof(double)));
+    hipError_t e = hipMemcpy(d, dsf, nc * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_perturb_sf, dim3((unsigned)std::min<int64_t>(8192, (nc + 255) / 256)), dim3(256), 0,
+                           h->stream, h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2], h->sflag, h->rec.sf,
+                           d, h->counters);
+        e = hipGetLastError();
+    }
+    int64_t flags = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&flags, h->counters + 1, sizeof(int64_t), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    dfree_d(d);
+    PSM_HIP(h, e);
+    h->geom_gen += 1;
+    if (flags & 16)
+        return set_err(h, LBDEM_PSM_ERR_INVALID_ARG,
+                       "solid-fraction perturbation leaves (0.001, 1] or touches a cell without a record "
+                       "(the valid entries were applied)");
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) {
+    if (!h || nsteps < 1 || K < 1) return LBDEM_PSM_ERR_INVALID_ARG;
+    if (h->precision == LBDEM_PSM_FP16_STORE)
+        return set_err(h, LBDEM_PSM_ERR_STATE, "the differentiable window supports FP64 and FP32_DEV storage");
+    if (!h->have_particles) return set_err(h, LBDEM_PSM_ERR_STATE, "diff_forward before set_spheres");
+    auto& D = h->diff;
+    D.ready = false;
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    const int64_t np = h->np;
+    const int ncp = (nsteps + K - 1) / K;
+    int rc = ensure_states(h, D.ckpt, (size_t)ncp);
+    if (rc) return rc;
+    dfree_d(D.d_ft);
+    D.d_ft = nullptr;
+    PSM_HIP(h, dalloc_d(&D.d_ft, (size_t)nsteps * (size_t)std::max<int64_t>(np, 1) * 6 * sizeof(double)));
+    if (np > 0) PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->px, 3 * np * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
+    for (int t = 0; t < nsteps; ++t) {
+        if (t % K == 0)
+            PSM_HIP(h, hipMemcpyAsync(D.ckpt[t / K], h->f[h->cur_f], h->pop_bytes, hipMemcpyDeviceToDevice, h->stream));
+        rc = launch_collide_stream(h);
+        if (rc) return rc;
+        if (np > 0) {
+            rc = run_force_sum(h);
+            if (rc) return rc;
+            PSM_HIP(h, hipMemcpyAsync(D.d_ft + (size_t)t * np * 6, h->pft, np * 6 * sizeof(double),
+                                      hipMemcpyDeviceToDevice, h->stream));
+        }
+    }
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    if (ft && np > 0) {
+        std::vector<double> tmp((size_t)nsteps * np * 6);
+        PSM_HIP(h, hipMemcpy(tmp.data(), D.d_ft, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int t = 0; t < nsteps; ++t) {
+            double* out = ft + (size_t)t * np * 6;
+            const double* in = tmp.data() + (size_t)t * np * 6;
+            for (int64_t k = 0; k < 6 * np; ++k) out[k] = 0.0;
+            /* getForcesFromLattice fold onto map(tag), as sum_force_torque */
+            for (int64_t q = 0; q < np; ++q) {
+                const int64_t o = h->h_owner[q];
+                for (int m = 0; m < 6; ++m) out[6 * o + m] += in[6 * q + m];
+            }
+        }
+    }
+    D.T = nsteps;
+    D.K = K;
+    D.np = np;
+    D.gen = h->geom_gen;
+    D.ready = true;
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT, double* g_x3, double* g_v3,
+                            double* g_w3, double* g_tau, double* g_sf, double* g_B, double* g_f0) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    auto& D = h->diff;
+    if (!D.ready) return set_err(h, LBDEM_PSM_ERR_STATE, "diff_backward without a diff_forward");
+    if (D.gen != h->geom_gen || D.np != h->np)
+        return set_err(h, LBDEM_PSM_ERR_STATE, "records, particles or tau changed since diff_forward");
+    const int T = D.T, K = D.K;
+    const int64_t np = D.np, nc = h->g.nc, rcap = h->rec_cap;
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    int rc = ensure_states(h, D.seg, (size_t)std::max(std::min(K, T) - 1, 0));
+    if (rc) return rc;
+    for (int k = 0; k < 2; ++k)
+        if (!D.lam[k]) PSM_HIP(h, dalloc_d(&D.lam[k], h->pop_bytes));
+    if (!D.gacc) PSM_HIP(h, dalloc_d(&D.gacc, 7 * rcap * sizeof(double)));
+    if (!D.Fscr) PSM_HIP(h, dalloc_d(&D.Fscr, 3 * rcap * sizeof(double)));
+    const int64_t nblk = (int64_t)h->g.n[0] * h->g.n[1] * ((h->g.n[2] + 255) / 256);
+    if (D.opart_n < nblk + 8) {
+        dfree_d(D.opart);
+        PSM_HIP(h, dalloc_d(&D.opart, (nblk + 8) * sizeof(double)));
+        D.opart_n = nblk + 8;
+    }
+    dfree_d(D.gpart);
+    D.gpart = nullptr;
+    PSM_HIP(h, dalloc_d(&D.gpart, 9 * std::max<int64_t>(np, 1) * sizeof(double)));
+    if (!D.cell_out) PSM_HIP(h, dalloc_d(&D.cell_out, 2 * nc * sizeof(double)));
+    const int gblocks = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (rcap + 255) / 256));
+    if (D.tau_part_n < gblocks + 8) {
+        dfree_d(D.tau_part);
+        PSM_HIP(h, dalloc_d(&D.tau_part, (gblocks + 8) * sizeof(double)));
+        D.tau_part_n = gblocks + 8;
+    }
+
+    /* objective weights (the forward outputs were already returned) */
+    const bool have_gft = g_ft && np > 0;
+    if (have_gft) PSM_HIP(h, hipMemcpy(D.d_ft, g_ft, (size_t)T * np * 6 * sizeof(double), hipMemcpyHostToDevice));
+    int cur = 0;
+    PSM_HIP(h, hipMemsetAsync(D.lam[0], 0, h->pop_bytes, h->stream));
+    PSM_HIP(h, hipMemsetAsync(D.lam[1], 0, h->pop_bytes, h->stream));
+    double* tmp64 = nullptr;
+    if (g_fT || g_f0) PSM_HIP(h, dalloc_d(&tmp64, (size_t)Q * nc * sizeof(double)));
+    if (g_fT) {
+        hipError_t e = hipMemcpy(tmp64, g_fT, (size_t)Q * nc * sizeof(double), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { dfree_d(tmp64); return hip_check(h, e, "hipMemcpy g_fT"); }
+        rc = launch_pop_convert_buf(h, D.lam[cur], tmp64, true);
+        if (rc) { dfree_d(tmp64); return rc; }
+    }
+    PSM_HIP(h, hipMemsetAsync(D.gacc, 0, 7 * rcap * sizeof(double), h->stream));
+    PSM_HIP(h, hipMemsetAsync(D.opart, 0, D.opart_n * sizeof(double), h->stream));
+
+    const int nseg = (T + K - 1) / K;
+    std::vector<const void*> st((size_t)K);
+    for (int s = nseg - 1; s >= 0; --s) {
+        const int t0 = s * K;
+        const int len = std::min(K, T - t0);
+        st[0] = D.ckpt[s];
+        for (int j = 1; j < len; ++j) {
+            rc = launch_collide_stream_buf(h, st[j - 1], D.seg[j - 1], D.Fscr);
+            if (rc) { dfree_d(tmp64); return rc; }
+            st[j] = D.seg[j - 1];
+        }
+        for (int j = len - 1; j >= 0; --j) {
+            const int t = t0 + j;
+            rc = adjoint_step(h, st[j], D.lam[cur], D.lam[1 - cur],
+                              have_gft ? D.d_ft + (size_t)t * np * 6 : nullptr);
+            if (rc) { dfree_d(tmp64); return rc; }
+            cur = 1 - cur;
+        }
+    }
+
+    /* reductions */
+    double* red = h->red_tmp;   /* [0] omega adjoint, [1] tau adjoint through B */
+    hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, h->stream, D.opart, nblk, red);
+    hipLaunchKernelGGL(k_grad_cells, dim3((unsigned)gblocks), dim3(256), 0, h->stream, h->g.n[0], h->g.n[1],
+                       h->g.n[2], h->g.nb[1], h->g.nb[2], rcap, h->sflag, h->rec.sf, D.gacc, h->ooo, D.cell_out, nc,
+                       D.tau_part);
+    hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, h->stream, D.tau_part, (int64_t)gblocks, red + 1);
+    if (np > 0) {
+        GradArgs ga;
+        for (int d = 0; d < 3; ++d) { ga.n[d] = h->g.n[d]; ga.nb[d] = h->g.nb[d]; }
+        ga.sflag = h->sflag;
+        ga.rsf = h->rec.sf;
+        ga.rcand = h->rec.cand;
+        ga.rid = h->rec.id;
+        ga.rcap = rcap;
+        ga.gacc = D.gacc;
+        ga.xs = h->psum_x;
+        ga.pw = h->pw;
+        ga.pr = h->pr;
+        ga.powner = h->powner;
+        ga.p_ent_start = h->p_ent_start;
+        ga.p_nent = h->p_nent;
+        ga.owner_ent = h->owner_ent;
+        ga.cand_bin = h->cand_bin;
+        ga.ent_cap = h->ent_cap;
+        ga.ooo = h->ooo;
+        ga.rc = h->cell_rc;
+        ga.vc = h->cell_vc;
+        ga.overlap = h->desc.voxel_mode == LBDEM_PSM_VOXEL_OVERLAP ? 1 : 0;
+        ga.gpart = D.gpart;
+        hipLaunchKernelGGL(k_grad_particle, dim3((unsigned)np), dim3(256), 0, h->stream, ga);
+        hipLaunchKernelGGL(k_grad_orphans, dim3(64), dim3(256), 0, h->stream, ga, h->orphan, h->counters,
+                           h->orphan_cap, h->id_map, h->id_map_size);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { dfree_d(tmp64); return hip_check(h, e, "adjoint reductions"); }
+    if (g_f0) {
+        rc = launch_pop_convert_buf(h, D.lam[cur], tmp64, false);
+        if (rc) { dfree_d(tmp64); return rc; }
+    }
+    double hred[2] = {0.0, 0.0};
+    e = hipMemcpyAsync(hred, red, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess && g_f0) e = hipMemcpy(g_f0, tmp64, (size_t)Q * nc * sizeof(double), hipMemcpyDeviceToHost);
+    dfree_d(tmp64);
+    PSM_HIP(h, e);
+    if (g_tau) {
+        /* omega = 1/tau, B depends on tau - 1/2 */
+        *g_tau = -hred[0] * (h->omega * h->omega) + hred[1];
+    }
+    if (g_sf || g_B) {
+        std::vector<double> co(2 * (size_t)nc);
+        PSM_HIP(h, hipMemcpy(co.data(), D.cell_out, co.size() * sizeof(double), hipMemcpyDeviceToHost));
+        if (g_sf) std::copy(co.begin(), co.begin() + nc, g_sf);
+        if (g_B) std::copy(co.begin() + nc, co.end(), g_B);
+    }
+    if (g_x3 || g_v3 || g_w3) {
+        std::vector<double> gp(9 * (size_t)np);
+        if (np > 0) PSM_HIP(h, hipMemcpy(gp.data(), D.gpart, gp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < 3 * np; ++k) {
+            if (g_v3) g_v3[k] = 0.0;
+            if (g_w3) g_w3[k] = 0.0;
+            if (g_x3) g_x3[k] = 0.0;
+        }
+        for (int64_t q = 0; q < np; ++q) {
+            const int64_t o = h->h_owner[q];
+            for (int d = 0; d < 3; ++d) {
+                if (g_v3) g_v3[3 * o + d] += gp[9 * q + d];
+                if (g_w3) g_w3[3 * o + d] += gp[9 * q + 3 + d];
+                if (g_x3) g_x3[3 * o + d] += gp[9 * q + 6 + d];
+            }
+        }
+    }
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_diff_release(lbdem_psm* h) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    return diff_free(h);
+}
+
+}  // extern "C"

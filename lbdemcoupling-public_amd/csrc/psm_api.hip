@@ -222,6 +222,8 @@ int lbdem_psm_check_desc(const lbdem_psm_desc* d) {
     for (int k = 0; k < 3; ++k)
         if (!std::isfinite(d->body_accel[k]))
             return set_err(nullptr, LBDEM_PSM_ERR_NUMERIC, "body_accel must be finite");
+    if (d->voxel_mode != LBDEM_PSM_VOXEL_SUBCELL && d->voxel_mode != LBDEM_PSM_VOXEL_OVERLAP)
+        return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "unknown voxel_mode");
     if (d->max_solid_cells < 0 || d->max_solid_cells > nc)
         return set_err(nullptr, LBDEM_PSM_ERR_INVALID_ARG, "max_solid_cells must be in [0, cells]");
     return LBDEM_PSM_OK;
@@ -281,6 +283,11 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
     h->omega = 1.0 / desc->tau;
     h->ooo = 1.0 / h->omega - 0.5;          /* src/ibCompositeDynamics3D.hh:178 */
     h->sqrt3half = std::sqrt(3.1) / 2.0;    /* src/ibProcessors3D.hh:127 */
+    {
+        const double pi = 3.14159265358979323846;
+        h->cell_rc = std::cbrt(3.0 / (4.0 * pi));
+        h->cell_vc = ((4.0 / 3.0) * pi) * ((h->cell_rc * h->cell_rc) * h->cell_rc);
+    }
     h->has_force = desc->body_accel[0] != 0.0 || desc->body_accel[1] != 0.0 || desc->body_accel[2] != 0.0;
     h->precision = desc->precision;
     const size_t S = h->precision == LBDEM_PSM_FP64 ? 8 : (h->precision == LBDEM_PSM_FP32_DEV ? 4 : 2);
@@ -337,6 +344,7 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
 int lbdem_psm_destroy(lbdem_psm* h) {
     if (!h) return LBDEM_PSM_OK;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    diff_free(h);
     dfree(h->f[0]); dfree(h->f[1]); dfree(h->sflag);
     dfree(h->bin_any[0]); dfree(h->bin_any[1]); dfree(h->cand_data);
     free_records(h->rec);
@@ -459,6 +467,7 @@ int lbdem_psm_set_spheres(lbdem_psm* h, int64_t n, const double* x3, const doubl
     h->np = n;
     h->flags = flags;
     h->have_particles = true;
+    h->geom_gen += 1;
     if ((flags & LBDEM_PSM_INIT_VEL) && !h->dv_cell) {
         PSM_HIP(h, dalloc(&h->dv_cell, h->orphan_cap * sizeof(int32_t)));
         PSM_HIP(h, dalloc(&h->dv_us, 3 * h->orphan_cap * sizeof(double)));
@@ -498,6 +507,7 @@ int lbdem_psm_sum_force_torque(lbdem_psm* h, int64_t n, const double* x3, double
 int lbdem_psm_step(lbdem_psm* h, int32_t nsteps, int32_t flags) {
     if (!h || nsteps < 0) return LBDEM_PSM_ERR_INVALID_ARG;
     if (!h->have_particles) return set_err(h, LBDEM_PSM_ERR_STATE, "step before set_spheres");
+    h->geom_gen += 1;
     PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
     for (int s = 0; s < nsteps; ++s) {
         int rc;
@@ -603,6 +613,7 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
     int64_t hc[8] = {m, 0, m, 0, 0, 0, 0, 0};
     PSM_HIP(h, hipMemcpy(h->counters, hc, sizeof(hc), hipMemcpyHostToDevice));
     h->records_from_host = true;
+    h->geom_gen += 1;
     return LBDEM_PSM_OK;
 }
 

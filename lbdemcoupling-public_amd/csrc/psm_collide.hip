@@ -171,10 +171,10 @@ static CollideArgs make_args(lbdem_psm* h) {
 }
 
 template <>
-void launch_cs_prec<LBDEM_PSM_FP64>(lbdem_psm* h, const CollideArgs& a) {
+void launch_cs_prec<LBDEM_PSM_FP64>(lbdem_psm* h, const CollideArgs& a, const void* vin, void* vout) {
     typedef typename Store<LBDEM_PSM_FP64>::S S;
-    const S* fin = (const S*)h->f[h->cur_f];
-    S* fout = (S*)h->f[1 - h->cur_f];
+    const S* fin = (const S*)vin;
+    S* fout = (S*)vout;
     dim3 grid((unsigned)(a.nx * a.ny), (unsigned)((a.nz + 255) / 256)), block(256);
     if (h->has_force) {
         if (a.pz) hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, true>), grid, block, 0, h->stream, fin, fout, a);
@@ -185,18 +185,32 @@ void launch_cs_prec<LBDEM_PSM_FP64>(lbdem_psm* h, const CollideArgs& a) {
     }
 }
 
+static void launch_any(lbdem_psm* h, const CollideArgs& a, const void* fin, void* fout) {
+    switch (h->precision) {
+        case LBDEM_PSM_FP64: launch_cs_prec<LBDEM_PSM_FP64>(h, a, fin, fout); break;
+        case LBDEM_PSM_FP32_DEV: launch_cs_prec<LBDEM_PSM_FP32_DEV>(h, a, fin, fout); break;
+        default: launch_cs_prec<LBDEM_PSM_FP16_STORE>(h, a, fin, fout); break;
+    }
+}
+
 int launch_collide_stream(lbdem_psm* h) {
     CollideArgs a = make_args(h);
     hipEvent_t ev = nullptr;
     prof_begin(h, 0, &ev);
-    switch (h->precision) {
-        case LBDEM_PSM_FP64: launch_cs_prec<LBDEM_PSM_FP64>(h, a); break;
-        case LBDEM_PSM_FP32_DEV: launch_cs_prec<LBDEM_PSM_FP32_DEV>(h, a); break;
-        default: launch_cs_prec<LBDEM_PSM_FP16_STORE>(h, a); break;
-    }
+    launch_any(h, a, h->f[h->cur_f], h->f[1 - h->cur_f]);
     PSM_HIP(h, hipGetLastError());
     prof_end(h, 0, ev);
     h->cur_f = 1 - h->cur_f;
+    return LBDEM_PSM_OK;
+}
+
+/* One collide+stream between caller-chosen buffers; the per-cell forces go
+ * to rF (3 per record slot) instead of the records. */
+int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, double* rF) {
+    CollideArgs a = make_args(h);
+    a.rF = rF;
+    launch_any(h, a, fin, fout);
+    PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }
 
@@ -226,6 +240,10 @@ int launch_init_equilibrium(lbdem_psm* h, const double* d_rho, const double* d_u
 }
 
 int launch_pop_convert(lbdem_psm* h, double* d_f64, bool to_store) {
+    return launch_pop_convert_buf(h, h->f[h->cur_f], d_f64, to_store);
+}
+
+int launch_pop_convert_buf(lbdem_psm* h, void* buf, double* d_f64, bool to_store) {
     const int64_t nc = h->g.nc;
     const int64_t total = (int64_t)Q * nc;
     const int nz = h->g.n[2];
@@ -233,13 +251,13 @@ int launch_pop_convert(lbdem_psm* h, double* d_f64, bool to_store) {
     dim3 grid(grid_for(total)), block(256);
     switch (h->precision) {
         case LBDEM_PSM_FP64:
-            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)h->f[h->cur_f], d_f64, nc, nz, nzp, (int)to_store);
+            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)buf, d_f64, nc, nz, nzp, (int)to_store);
             break;
         case LBDEM_PSM_FP32_DEV:
-            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)h->f[h->cur_f], d_f64, nc, nz, nzp, (int)to_store);
+            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)buf, d_f64, nc, nz, nzp, (int)to_store);
             break;
         default:
-            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)h->f[h->cur_f], d_f64, nc, nz, nzp, (int)to_store);
+            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)buf, d_f64, nc, nz, nzp, (int)to_store);
             break;
     }
     PSM_HIP(h, hipGetLastError());

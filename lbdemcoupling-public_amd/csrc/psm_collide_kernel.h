@@ -321,8 +321,10 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     for (int i = 0; i < Q; ++i) push.put(i, z, (S)fo[i]);
 }
 
+/* Launches one collide+stream from fin into fout (row-blocked stores of the
+ * handle's precision). */
 template <int PREC>
-void launch_cs_prec(lbdem_psm* h, const CollideArgs& a);
+void launch_cs_prec(lbdem_psm* h, const CollideArgs& a, const void* fin, void* fout);
 
 }  // namespace psm
 

@@ -9,10 +9,10 @@
 namespace psm {
 
 template <int PREC>
-static void launch_lp(lbdem_psm* h, const CollideArgs& a) {
+static void launch_lp(lbdem_psm* h, const CollideArgs& a, const void* vin, void* vout) {
     typedef typename Store<PREC>::S S;
-    const S* fin = (const S*)h->f[h->cur_f];
-    S* fout = (S*)h->f[1 - h->cur_f];
+    const S* fin = (const S*)vin;
+    S* fout = (S*)vout;
     dim3 grid((unsigned)(a.nx * a.ny), (unsigned)((a.nz + 255) / 256)), block(256);
     if (h->has_force) {
         if (a.pz) hipLaunchKernelGGL((k_collide_stream<PREC, true, true>), grid, block, 0, h->stream, fin, fout, a);
@@ -24,8 +24,12 @@ static void launch_lp(lbdem_psm* h, const CollideArgs& a) {
 }
 
 template <>
-void launch_cs_prec<LBDEM_PSM_FP32_DEV>(lbdem_psm* h, const CollideArgs& a) { launch_lp<LBDEM_PSM_FP32_DEV>(h, a); }
+void launch_cs_prec<LBDEM_PSM_FP32_DEV>(lbdem_psm* h, const CollideArgs& a, const void* fin, void* fout) {
+    launch_lp<LBDEM_PSM_FP32_DEV>(h, a, fin, fout);
+}
 template <>
-void launch_cs_prec<LBDEM_PSM_FP16_STORE>(lbdem_psm* h, const CollideArgs& a) { launch_lp<LBDEM_PSM_FP16_STORE>(h, a); }
+void launch_cs_prec<LBDEM_PSM_FP16_STORE>(lbdem_psm* h, const CollideArgs& a, const void* fin, void* fout) {
+    launch_lp<LBDEM_PSM_FP16_STORE>(h, a, fin, fout);
+}
 
 }  // namespace psm

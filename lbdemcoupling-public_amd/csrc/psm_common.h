@@ -194,6 +194,48 @@ __device__ __forceinline__ void psm_solid(const T* fPre, T* f, T omega, bool ful
     }
 }
 
+/* Minimum image of a torque arm component over the lattice size n
+ * (src/ibProcessors3D.hh:238-243, integer halves of the block size). */
+__device__ __forceinline__ double min_image(double d, int n) {
+    if (d > (double)(n / 2)) d -= (double)n;
+    else if (d < (double)(-n / 2)) d += (double)n;
+    return d;
+}
+
+/* Analytic sphere-cell overlap, the differentiable voxelization: the
+ * fraction of a unit-volume cell covered by a sphere of radius r at centre
+ * distance d, the cell taken as the sphere of equal volume (radius rc =
+ * (3/(4 pi))^(1/3), volume vc) at its centre.  Lens volume of two spheres,
+ * V = pi t^2 (d^2 + 2 d (r + rc) - 3 (r - rc)^2) / (12 d), t = r + rc - d;
+ * its derivative is dV/dd = -pi a^2, a the radius of the intersection
+ * circle.  br: 0 outside, 1 cell inside the sphere, 3 lens, 4 sphere inside
+ * the cell sphere. */
+__host__ __device__ inline double overlap_fraction(double d, double r, double rc, double vc,
+                                                   double* dsf_dd, int* br) {
+    const double pi = 3.14159265358979323846;
+    double sf, der = 0.0;
+    int b;
+    if (d >= r + rc) {
+        sf = 0.0; b = 0;
+    } else if (d <= fabs(r - rc)) {
+        if (r >= rc) { sf = 1.0; b = 1; }
+        else { sf = (((4.0 / 3.0) * pi) * ((r * r) * r)) / vc; b = 4; }
+    } else {
+        const double t = (r + rc) - d;
+        const double rm = r - rc;
+        const double poly = ((d * d) + (2.0 * d) * (r + rc)) - (3.0 * rm) * rm;
+        sf = (((pi * t) * t) * poly) / (12.0 * d) / vc;
+        const double s = ((d * d) - (rc * rc)) + (r * r);
+        double a2 = (((4.0 * d) * d) * (r * r) - s * s) / ((4.0 * d) * d);
+        if (a2 < 0.0) a2 = 0.0;
+        der = -(pi * a2) / vc;
+        b = 3;
+    }
+    if (dsf_dd) *dsf_dd = der;
+    if (br) *br = b;
+    return sf;
+}
+
 /* ---- storage conversions ---- */
 template <int PREC> struct Store;
 template <> struct Store<LBDEM_PSM_FP64> {
@@ -264,6 +306,7 @@ struct lbdem_psm {
     lbdem_psm_desc desc;
     psm::Geom g;
     double omega, ooo, sqrt3half;
+    double cell_rc, cell_vc;   /* radius and volume of the equal-volume cell sphere */
     bool has_force;
     int precision;
     size_t pop_bytes;          /* bytes of one population buffer (row-blocked layout) */
@@ -313,6 +356,27 @@ struct lbdem_psm {
     double* dv_us = nullptr;
     int64_t red_blocks = 0;
 
+    /* differentiable window */
+    int64_t geom_gen = 0;            /* bumped by every change of records or particles */
+    struct Diff {
+        bool ready = false;
+        int64_t gen = -1;
+        int32_t T = 0, K = 0;
+        int64_t np = 0;
+        std::vector<void*> ckpt;     /* population state at t = 0, K, 2K, ... */
+        std::vector<void*> seg;      /* recomputed states of one segment */
+        void* lam[2] = {nullptr, nullptr};
+        double* d_ft = nullptr;      /* T*np*6, per-particle (F, T) of every step */
+        double* gacc = nullptr;      /* 7 per record slot: gB, gus[3], F x gT[3] */
+        double* Fscr = nullptr;      /* 3 per record slot: forces of recomputed steps */
+        double* opart = nullptr;     /* omega-gradient partial per workgroup */
+        int64_t opart_n = 0;
+        double* gpart = nullptr;     /* 9 per particle */
+        double* cell_out = nullptr;  /* 2*Nc dense gsf, gB */
+        double* tau_part = nullptr;
+        int64_t tau_part_n = 0;
+    } diff;
+
     /* profiling */
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -329,6 +393,9 @@ namespace psm {
 int launch_collide_stream(lbdem_psm* h);
 int launch_init_equilibrium(lbdem_psm* h, const double* d_rho, const double* d_u);
 int launch_pop_convert(lbdem_psm* h, double* d_f64, bool to_device_store);
+int launch_pop_convert_buf(lbdem_psm* h, void* buf, double* d_f64, bool to_device_store);
+int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, double* rF);
+int diff_free(lbdem_psm* h);
 int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
                         int64_t* n_nonfinite);
 int run_voxelize(lbdem_psm* h);

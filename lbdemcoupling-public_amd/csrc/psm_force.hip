@@ -47,13 +47,6 @@ __device__ __forceinline__ void rec_cell(int64_t ridx, const int* nb, int& X, in
     Z = bz * BIN + (l & 7);
 }
 
-__device__ __forceinline__ double min_image(double d, int n) {
-    /* src/ibProcessors3D.hh:238-243, integer halves of the block size */
-    if (d > (double)(n / 2)) d -= (double)n;
-    else if (d < (double)(-n / 2)) d += (double)n;
-    return d;
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);

@@ -250,6 +250,7 @@ struct FoldArgs {
     int nb[3];
     int N;
     double sw, frac;
+    double rc, vc;             /* cell sphere of the analytic overlap */
     const int32_t* bin_cand_start;
     const int32_t* bin_cand_count;
     const Cand* cand;
@@ -337,6 +338,25 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
     return sf;
 }
 
+/* The voxelization rule: NT >= 0 the reference's sub-cell count (NT = N
+ * when specialized, 0 = runtime N), NT < 0 the analytic overlap of the
+ * differentiable mode (per lane, no wavefront cooperation).  Called by all
+ * 64 lanes together. */
+template <int NT>
+__device__ __forceinline__ double solid_fraction(bool active, double dx, double dy, double dz,
+                                                 double rp2, double rm2, double rsq, double r, int Nrt,
+                                                 double sw, double frac, double rc, double vc, int& n,
+                                                 int& br) {
+    if constexpr (NT < 0) {
+        n = 0;
+        br = 0;
+        if (!active) return 0.0;
+        return overlap_fraction(sqrt((dx * dx + dy * dy) + dz * dz), r, rc, vc, nullptr, &br);
+    } else {
+        return coop_solid_fraction<NT>(active, dx, dy, dz, rp2, rm2, rsq, Nrt, sw, frac, n, br);
+    }
+}
+
 enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
 
 /* One workgroup of 512 lanes per 8^3 bin, one cell per lane (a wavefront
@@ -385,8 +405,8 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
             const double dy = (double)Y - cd.x[1];
             const double dz = (double)Z - cd.x[2];
             int nn, br;
-            const double sf = coop_solid_fraction<NT>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, a.N, a.sw,
-                                                      a.frac, nn, br);
+            const double sf = solid_fraction<NT>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, a.N, a.sw,
+                                                 a.frac, a.rc, a.vc, nn, br);
             if (!inb) continue;
             const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (ssf > SOLFRAC_MIN ? 1 : 0);
             bool set = false, zero = false;
@@ -457,15 +477,16 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
 template <int NT>
 __global__ __launch_bounds__(256) void k_subcell(int64_t n, const double* __restrict__ d3,
                                                  const double* __restrict__ r, int N, double sw,
-                                                 double frac, double h, double* __restrict__ sf,
-                                                 int32_t* __restrict__ cnt, int32_t* __restrict__ br) {
+                                                 double frac, double h, double rc, double vc,
+                                                 double* __restrict__ sf, int32_t* __restrict__ cnt,
+                                                 int32_t* __restrict__ br) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = k < n;
     const double rr = act ? r[k] : 1.0;
     const double r_p = rr + h, r_m = rr - h;
     const double dx = act ? d3[3 * k] : 0.0, dy = act ? d3[3 * k + 1] : 0.0, dz = act ? d3[3 * k + 2] : 0.0;
     int nn, bb;
-    const double v = coop_solid_fraction<NT>(act, dx, dy, dz, r_p * r_p, r_m * r_m, rr * rr, N, sw, frac, nn, bb);
+    const double v = solid_fraction<NT>(act, dx, dy, dz, r_p * r_p, r_m * r_m, rr * rr, rr, N, sw, frac, rc, vc, nn, bb);
     if (act) {
         sf[k] = v;
         cnt[k] = nn;
@@ -536,6 +557,9 @@ int run_voxelize(lbdem_psm* h) {
     fa.N = h->desc.slices_per_dim;
     fa.sw = 1.0 / (double)fa.N;
     fa.frac = 1.0 / (double)(fa.N * fa.N * fa.N);
+    fa.rc = h->cell_rc;
+    fa.vc = h->cell_vc;
+    const bool overlap = h->desc.voxel_mode == LBDEM_PSM_VOXEL_OVERLAP;
     const bool init_vel = (h->flags & LBDEM_PSM_INIT_VEL) != 0;
     fa.bin_cand_start = h->bin_cand_start;
     fa.bin_cand_count = h->bin_cand_count;
@@ -552,7 +576,10 @@ int run_voxelize(lbdem_psm* h) {
     fa.dv_cell = init_vel ? h->dv_cell : nullptr;
     fa.dv_us = h->dv_us;
     dim3 grid((unsigned)nbins), block(512);
-    if (init_vel) {
+    if (overlap) {
+        if (init_vel) hipLaunchKernelGGL((k_fold<-1, true>), grid, block, 0, h->stream, fa);
+        else hipLaunchKernelGGL((k_fold<-1, false>), grid, block, 0, h->stream, fa);
+    } else if (init_vel) {
         if (fa.N == 5) hipLaunchKernelGGL((k_fold<5, true>), grid, block, 0, h->stream, fa);
         else if (fa.N == 8) hipLaunchKernelGGL((k_fold<8, true>), grid, block, 0, h->stream, fa);
         else hipLaunchKernelGGL((k_fold<0, true>), grid, block, 0, h->stream, fa);
@@ -598,9 +625,12 @@ int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_
     const int N = h->desc.slices_per_dim;
     const double sw = 1.0 / (double)N, frac = 1.0 / (double)(N * N * N);
     dim3 grid((unsigned)std::max<int64_t>(1, (n + 255) / 256)), block(256);
-    if (N == 5) hipLaunchKernelGGL(k_subcell<5>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, d_sf, d_cnt, d_br);
-    else if (N == 8) hipLaunchKernelGGL(k_subcell<8>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, d_sf, d_cnt, d_br);
-    else hipLaunchKernelGGL(k_subcell<0>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, d_sf, d_cnt, d_br);
+    const double rc = h->cell_rc, vc = h->cell_vc;
+    if (h->desc.voxel_mode == LBDEM_PSM_VOXEL_OVERLAP)
+        hipLaunchKernelGGL(k_subcell<-1>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, d_sf, d_cnt, d_br);
+    else if (N == 5) hipLaunchKernelGGL(k_subcell<5>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, d_sf, d_cnt, d_br);
+    else if (N == 8) hipLaunchKernelGGL(k_subcell<8>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, d_sf, d_cnt, d_br);
+    else hipLaunchKernelGGL(k_subcell<0>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, d_sf, d_cnt, d_br);
     PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }

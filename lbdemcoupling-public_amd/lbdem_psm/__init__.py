@@ -12,6 +12,9 @@ The binding mirrors the reference's coupling surface for the PSM hot path:
 * ``Lattice.sum_force_torque`` -- ``SumForceTorque3D`` + the ``map(tag)`` fold of
   ``getForcesFromLattice`` (src/ibProcessors3D.hh:202-275,
   src/ibDataExchangeWrappers3D.hh:160-246).
+* ``Lattice.diff_forward`` / ``diff_backward`` -- a differentiable window over
+  frozen geometry with the hand-written adjoint (no counterpart in the
+  reference); ``lbdem_psm.autograd`` wraps it as a ``torch.autograd.Function``.
 
 All compute runs in the HIP library ``lib/liblbdem_psm.so``.  There is no
 CPU fallback: if the library is missing, ``load_library`` raises.
@@ -38,6 +41,10 @@ FP64 = 0
 FP32_DEV = 1
 FP16_STORE = 2
 PRECISIONS = {"fp64": FP64, "fp32_dev": FP32_DEV, "fp16_store": FP16_STORE}
+
+VOXEL_SUBCELL = 0
+VOXEL_OVERLAP = 1
+VOXEL_MODES = {"subcell": VOXEL_SUBCELL, "overlap": VOXEL_OVERLAP}
 
 INIT_VEL = 1
 AUTO_IMAGES = 2
@@ -74,7 +81,7 @@ class Desc(ctypes.Structure):
         ("slices_per_dim", ctypes.c_int32),
         ("body_accel", ctypes.c_double * 3),
         ("device", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("voxel_mode", ctypes.c_int32),
         ("max_solid_cells", ctypes.c_int64),
     ]
 
@@ -126,6 +133,12 @@ _SIGS = {
     "lbdem_psm_profile": ([_P, ctypes.c_int32], ctypes.c_int),
     "lbdem_psm_profile_read": ([_P, _P, _P, ctypes.c_int32], ctypes.c_int),
     "lbdem_psm_generate_spheres": ([_P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "lbdem_psm_set_tau": ([_P, ctypes.c_double], ctypes.c_int),
+    "lbdem_psm_clear_records": ([_P], ctypes.c_int),
+    "lbdem_psm_perturb_solid_fraction": ([_P, _P], ctypes.c_int),
+    "lbdem_psm_diff_forward": ([_P, ctypes.c_int32, ctypes.c_int32, _P], ctypes.c_int),
+    "lbdem_psm_diff_backward": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "lbdem_psm_diff_release": ([_P], ctypes.c_int),
 }
 
 
@@ -166,7 +179,7 @@ def _check(rc: int, handle=None):
 
 
 def make_desc(nx, ny, nz, periodic=(1, 1, 1), tau=1.0, precision="fp64", slices_per_dim=5,
-              body_accel=(0.0, 0.0, 0.0), device=0, max_solid_cells=0) -> Desc:
+              body_accel=(0.0, 0.0, 0.0), device=0, max_solid_cells=0, voxel_mode="subcell") -> Desc:
     d = Desc()
     load_library().lbdem_psm_desc_init(ctypes.byref(d))
     d.nx, d.ny, d.nz = int(nx), int(ny), int(nz)
@@ -178,6 +191,7 @@ def make_desc(nx, ny, nz, periodic=(1, 1, 1), tau=1.0, precision="fp64", slices_
     d.slices_per_dim = int(slices_per_dim)
     d.device = int(device)
     d.max_solid_cells = int(max_solid_cells)
+    d.voxel_mode = VOXEL_MODES[voxel_mode] if isinstance(voxel_mode, str) else int(voxel_mode)
     return d
 
 
@@ -239,10 +253,11 @@ class Lattice:
     """A D3Q19 PSM lattice on one GPU (one lbdem_psm handle)."""
 
     def __init__(self, nx, ny, nz, periodic=(1, 1, 1), tau=1.0, precision="fp64",
-                 slices_per_dim=5, body_accel=(0.0, 0.0, 0.0), device=0, max_solid_cells=0):
+                 slices_per_dim=5, body_accel=(0.0, 0.0, 0.0), device=0, max_solid_cells=0,
+                 voxel_mode="subcell"):
         self.lib = load_library()
         self.desc = make_desc(nx, ny, nz, periodic, tau, precision, slices_per_dim, body_accel,
-                              device, max_solid_cells)
+                              device, max_solid_cells, voxel_mode)
         self.n = (int(nx), int(ny), int(nz))
         self.nc = self.n[0] * self.n[1] * self.n[2]
         self.periodic = tuple(int(bool(p)) for p in periodic)
@@ -352,6 +367,49 @@ class Lattice:
         self._c(self.lib.lbdem_psm_subcell_counts(self.h, n, _ptr(d), _ptr(r), _ptr(sf), _ptr(cnt),
                                                   _ptr(br)))
         return sf, cnt, br
+
+    # differentiable window ---------------------------------------------
+    def set_tau(self, tau):
+        self._c(self.lib.lbdem_psm_set_tau(self.h, float(tau)))
+        self.desc.tau = float(tau)
+
+    def clear_records(self):
+        self._c(self.lib.lbdem_psm_clear_records(self.h))
+
+    def perturb_solid_fraction(self, dsf):
+        dsf = _f64(dsf).ravel()
+        assert dsf.shape[0] == self.nc
+        self._c(self.lib.lbdem_psm_perturb_solid_fraction(self.h, _ptr(dsf)))
+
+    def diff_forward(self, nsteps, checkpoint_every):
+        """Runs the window; returns (nsteps, n, 6) forces and torques."""
+        ft = np.zeros((int(nsteps), self.np_, 6))
+        self._c(self.lib.lbdem_psm_diff_forward(self.h, int(nsteps), int(checkpoint_every),
+                                                _ptr(ft)))
+        return ft
+
+    def diff_backward(self, g_ft=None, g_fT=None, want_f0=False):
+        """Adjoint of the last diff_forward: dict of gradients."""
+        n = self.np_
+        gft = None if g_ft is None else _f64(g_ft)
+        gfT = None if g_fT is None else _f64(g_fT).reshape(Q, -1)
+        gx = np.zeros((n, 3))
+        gv = np.zeros((n, 3))
+        gw = np.zeros((n, 3))
+        gtau = ctypes.c_double()
+        gsf = np.zeros(self.nc)
+        gB = np.zeros(self.nc)
+        gf0 = np.zeros((Q, self.nc)) if want_f0 else None
+        self._c(self.lib.lbdem_psm_diff_backward(self.h, _ptr(gft), _ptr(gfT), _ptr(gx), _ptr(gv),
+                                                 _ptr(gw), ctypes.byref(gtau), _ptr(gsf), _ptr(gB),
+                                                 _ptr(gf0)))
+        out = {"x": gx, "v": gv, "w": gw, "tau": gtau.value, "sf": gsf, "B": gB}
+        if want_f0:
+            out["f0"] = gf0
+        return out
+
+    def diff_release(self):
+        self._c(self.lib.lbdem_psm_diff_release(self.h))
 
     # diagnostics -------------------------------------------------------
     def conservation(self):

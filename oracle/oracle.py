@@ -42,6 +42,8 @@ def load():
         "orc_equilibrium": ([ctypes.c_double, _P, _P], None),
         "orc_collide_cell": ([_P, ctypes.c_double, ctypes.c_double, _P, _P], None),
         "orc_last_n_voxelized": ([_P], ctypes.c_int64),
+        "orc_overlap_fraction": ([ctypes.c_double, ctypes.c_double, _P, _P], ctypes.c_double),
+        "orc_set_voxel_mode": ([_P, ctypes.c_int], None),
     }
     for name, (a, r) in sigs.items():
         fn = getattr(lib, name)
@@ -63,6 +65,14 @@ def calc_solid_fraction(dx, dy, dz, r, N):
     return sf, n.value, br.value
 
 
+def overlap_fraction(d, r):
+    """Analytic sphere-cell overlap: (sf, dsf/dd, branch)."""
+    der = ctypes.c_double()
+    br = ctypes.c_int32()
+    sf = load().orc_overlap_fraction(float(d), float(r), ctypes.byref(der), ctypes.byref(br))
+    return sf, der.value, br.value
+
+
 def equilibrium(rho_bar, j):
     j = np.ascontiguousarray(j, dtype=np.float64)
     out = np.zeros(Q)
@@ -80,7 +90,7 @@ def collide_cell(f, tau, sf, us):
 
 class OracleLattice:
     def __init__(self, nx, ny, nz, periodic=(1, 1, 1), tau=1.0, slices_per_dim=5,
-                 body_accel=(0.0, 0.0, 0.0)):
+                 body_accel=(0.0, 0.0, 0.0), voxel_mode=0):
         self.lib = load()
         self.n = (int(nx), int(ny), int(nz))
         self.nc = self.n[0] * self.n[1] * self.n[2]
@@ -90,6 +100,7 @@ class OracleLattice:
                                      int(slices_per_dim), _p(acc))
         if not self.h:
             raise ValueError("invalid oracle lattice parameters")
+        self.lib.orc_set_voxel_mode(self.h, int(voxel_mode))
         self.np_ = 0
 
     def close(self):

@@ -177,6 +177,7 @@ struct orc_lattice {
     std::vector<int64_t> map_ids;   /* id -> first index of the last set_spheres */
     std::map<int64_t, int64_t> idmap;
     int64_t last_n_vox = 0;
+    int voxel_mode = 0;
 };
 
 extern "C" {
@@ -268,6 +269,38 @@ double orc_calc_solid_fraction(double dx, double dy, double dz, double r, int N,
     return fraction * (double)n;
 }
 
+double orc_overlap_fraction(double d, double r, double* dsf_dd, int32_t* branch_out) {
+    const double pi = 3.14159265358979323846;
+    const double rc = std::cbrt(3.0 / (4.0 * pi));
+    const double vc = ((4.0 / 3.0) * pi) * ((rc * rc) * rc);
+    double sf, der = 0.0;
+    int br;
+    if (d >= r + rc) {
+        sf = 0.0; br = 0;
+    } else if (d <= std::fabs(r - rc)) {
+        if (r >= rc) { sf = 1.0; br = 1; }
+        else { sf = (((4.0 / 3.0) * pi) * ((r * r) * r)) / vc; br = 4; }
+    } else {
+        /* lens: V = pi t^2 (d^2 + 2 d (r + rc) - 3 (r - rc)^2) / (12 d),
+         * t = r + rc - d; dV/dd = -pi a^2 with a the radius of the
+         * intersection circle */
+        const double t = (r + rc) - d;
+        const double rm = r - rc;
+        const double poly = ((d * d) + (2.0 * d) * (r + rc)) - (3.0 * rm) * rm;
+        sf = (((pi * t) * t) * poly) / (12.0 * d) / vc;
+        const double s = ((d * d) - (rc * rc)) + (r * r);
+        double a2 = (((4.0 * d) * d) * (r * r) - s * s) / ((4.0 * d) * d);
+        if (a2 < 0.0) a2 = 0.0;
+        der = -(pi * a2) / vc;
+        br = 3;
+    }
+    if (dsf_dd) *dsf_dd = der;
+    if (branch_out) *branch_out = br;
+    return sf;
+}
+
+void orc_set_voxel_mode(orc_lattice* L, int mode) { L->voxel_mode = mode ? 1 : 0; }
+
 int orc_set_spheres(orc_lattice* L, int64_t n, const double* x3, const double* v3,
                     const double* w3, const double* r, const int64_t* id,
                     const int32_t* type, const int32_t* exclude, int n_exclude, int init_vel,
@@ -341,7 +374,9 @@ int orc_set_spheres(orc_lattice* L, int64_t n, const double* x3, const double* v
                     Record& p = L->rec[c];
                     const double xg = (double)iX, yg = (double)iY, zg = (double)iZ;
                     const double dx = xg - x[0], dy = yg - x[1], dz = zg - x[2];
-                    const double sf = orc_calc_solid_fraction(dx, dy, dz, rr, N, nullptr, nullptr);
+                    const double sf = L->voxel_mode
+                        ? orc_overlap_fraction(std::sqrt((dx * dx + dy * dy) + dz * dz), rr, nullptr, nullptr)
+                        : orc_calc_solid_fraction(dx, dy, dz, rr, N, nullptr, nullptr);
                     const double sf_old = p.sf;
                     const int id_old = (int)p.id;
                     const int decFlag = (sf > SOLFRAC_MIN) + 2 * (sf_old > SOLFRAC_MIN);

@@ -82,6 +82,21 @@ void orc_set_cell_records(orc_lattice* L, const double* sf, const int64_t* id,
 double orc_calc_solid_fraction(double dx, double dy, double dz, double r, int N,
                                int32_t* n_out, int32_t* branch_out);
 
+/* Analytic overlap (the differentiable voxelization): the fraction of a
+ * unit-volume cell covered by a sphere of radius r whose centre is at
+ * distance d from the cell centre, with the cell represented by the sphere
+ * of equal volume (radius (3/(4 pi))^(1/3)) at its centre -- the lens volume
+ * of two spheres.  dsf_dd (may be NULL): derivative with respect to d.
+ * branch_out (may be NULL): 0 outside, 1 cell inside the sphere, 3 lens,
+ * 4 sphere inside the cell sphere. */
+double orc_overlap_fraction(double d, double r, double* dsf_dd, int32_t* branch_out);
+
+/* Voxelization rule of orc_set_spheres: 0 = sub-cell count of the reference
+ * (default), 1 = analytic overlap (orc_overlap_fraction of the distance
+ * sqrt((dx^2 + dy^2) + dz^2)).  Ownership, thresholds and setValues stay
+ * the reference's. */
+void orc_set_voxel_mode(orc_lattice* L, int mode);
+
 /* FP64 global mass deviation sum_i sum_c fbar and momentum sum c_i fbar. */
 void orc_conservation(const orc_lattice* L, double* mass, double* mom3);
 

@@ -40,7 +40,7 @@ def test_python_binding_covers_abi():
 
 def test_desc_layout_matches_header():
     # nx,ny,nz,periodic[3] (24) + tau (8) + precision, slices (8) + accel (24) + device,
-    # reserved (8) + max_solid_cells (8)
+    # voxel_mode (8) + max_solid_cells (8)
     assert ctypes.sizeof(L.Desc) == 80
 
 
@@ -50,7 +50,7 @@ def test_desc_validation():
     for field, val, code in [("tau", 0.5, L.ERR_NUMERIC), ("tau", float("nan"), L.ERR_NUMERIC),
                              ("nx", 0, L.ERR_INVALID_ARG), ("slices_per_dim", 0, L.ERR_INVALID_ARG),
                              ("slices_per_dim", 17, L.ERR_INVALID_ARG), ("precision", 7, L.ERR_INVALID_ARG),
-                             ("max_solid_cells", 10 ** 6, L.ERR_INVALID_ARG)]:
+                             ("max_solid_cells", 10 ** 6, L.ERR_INVALID_ARG), ("voxel_mode", 2, L.ERR_INVALID_ARG)]:
         d = L.make_desc(8, 8, 8)
         setattr(d, field, val)
         assert L.check_desc(d) == code, field
