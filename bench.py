@@ -107,6 +107,55 @@ def run_gpu(wl, precision, steps, warmup, spheres, slices=8):
     }
 
 
+ADJOINT_CASE = dict(n=(128, 128, 256), periodic=(0, 0, 1), tau=0.8, body=(0.0, 0.0, 1e-6), count=200,
+                    radius=5.0, sigma_v=0.005, steps=500, checkpoint_every=50,
+                    desc="differentiable channel 128x128x256, walls x/y, body force 1e-6 along z, "
+                         "200 spheres r=5, v~N(0,0.005), analytic overlap, adjoint of 500 steps, "
+                         "checkpoint every 50")
+
+
+def run_adjoint(precision):
+    """Forward window and hand-written adjoint of d(total z-drag)/d(params)."""
+    c = ADJOINT_CASE
+    x, v, w, r, ids = L.generate_spheres(c["n"], c["count"], L.RADIUS_CONST, a=c["radius"],
+                                         periodic=c["periodic"], gap=0.5, wall_gap=1.5,
+                                         sigma_v=c["sigma_v"], sigma_w=0.0, seed=44)
+    lat = L.Lattice(*c["n"], periodic=c["periodic"], tau=c["tau"], precision=precision,
+                    body_accel=c["body"], voxel_mode="overlap")
+    try:
+        lat.set_spheres(x, v, w, r, ids)
+        lat.diff_forward(4, 2)            # warm-up of both passes (allocations, code load)
+        lat.diff_backward(np.zeros((4, len(r), 6)))
+        lat.synchronize()
+        t0 = time.perf_counter()
+        ft = lat.diff_forward(c["steps"], c["checkpoint_every"])
+        t1 = time.perf_counter()
+        G = np.zeros_like(ft)
+        G[:, :, 2] = 1.0
+        g = lat.diff_backward(G)
+        t2 = time.perf_counter()
+        # the plain coupled step of the same geometry (no checkpoints), for the cost ratio
+        lat.collide_stream(2)
+        lat.synchronize()
+        t3 = time.perf_counter()
+        for _ in range(c["steps"]):
+            lat.collide_stream(1)
+            lat.sum_force_torque()
+        lat.synchronize()
+        t4 = time.perf_counter()
+        lat.diff_release()
+    finally:
+        lat.close()
+    fwd, bwd, plain = t1 - t0, t2 - t1, t4 - t3
+    return {"forward_s": round(fwd, 4), "backward_s": round(bwd, 4), "plain_forward_s": round(plain, 4),
+            "backward_over_forward": round(bwd / fwd, 3),
+            "total_over_plain_forward": round((fwd + bwd) / plain, 3),
+            "drag_z_total": float(ft[:, :, 2].sum()),
+            "grad_tau": g["tau"], "grad_v_norm": float(np.linalg.norm(g["v"])),
+            "grad_x_norm": float(np.linalg.norm(g["x"])), "grad_w_norm": float(np.linalg.norm(g["w"])),
+            "grad_sf_norm": float(np.linalg.norm(g["sf"])), "_g": g}
+
+
 def measured_traffic(workload, kernel):
     try:
         t = json.load(open(TRAFFIC_JSON))
@@ -251,6 +300,7 @@ def main():
     ap.add_argument("--workload", default="dense256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-extra", action="store_true", help="skip the 512^3-class FP32 line")
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-window line")
     ap.add_argument("--summarize-profiles", metavar="DIR",
                     help="turn rocprofv3 outputs under DIR (kernel trace db, fetch/ and write/ "
                          "counter csv) into profiles/*.md and profiles/collide_stream_traffic.json")
@@ -307,6 +357,16 @@ def main():
                                "fp64": {"mlups": round(r64["mlups"], 1),
                                         "kernel_gbs": round(r64["kernel_gbs"], 1)},
                                "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3)}
+
+    if not args.no_adjoint:
+        adj = {}
+        for prec in ("fp64", "fp32_dev"):
+            log(f"adjoint window {prec}: {ADJOINT_CASE['steps']} steps forward + backward")
+            adj[prec] = run_adjoint(prec)
+            log(f"adjoint {prec}: forward {adj[prec]['forward_s']:.3f} s, backward {adj[prec]['backward_s']:.3f} s")
+        g64, g32 = adj["fp64"].pop("_g"), adj["fp32_dev"].pop("_g")
+        extra["adjoint"] = {"workload": ADJOINT_CASE["desc"], "fp64": adj["fp64"], "fp32_dev": adj["fp32_dev"],
+                            "fp32_grad_rel_err_vs_fp64": {k: rel(g32[k], g64[k]) for k in ("v", "w", "x", "sf")}}
 
     log(f"cpu baseline: {args.cpu_steps} oracle steps")
     cpu = cpu_baseline(wl, sph, args.cpu_steps)

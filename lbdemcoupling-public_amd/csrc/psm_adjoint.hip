@@ -58,132 +58,105 @@ struct EqBar {
     }
 };
 
-/* Adjoint of the base dynamics fb = BGK(fP) (Guo forcing when FORCE):
- * fPbar += (d fb / d fP)^T gb, omega_bar += (d fb / d omega) . gb. */
+/* Scalars of the base dynamics (BGK, Guo forcing when FORCE) of one cell,
+ * computed once; fb(i) then gives the post-BGK population i with the
+ * operation order of the forward kernel (bit-identical in FP64). */
 template <typename T, bool FORCE>
-__device__ __forceinline__ void bgk_vjp(const T* fP, const T* gb, T omega, const T* acc, T* fPbar,
-                                        T& omega_bar) {
-    T rhoBar, j[3];
-    moments<T>(fP, rhoBar, j);
-    const T rho = rhoBar + T(1);
-    const T invRho = T(1) / rho;
-    T jE[3] = {j[0], j[1], j[2]};
-    T u[3] = {T(0), T(0), T(0)};
-    if (FORCE) {
+struct BgkCell {
+    T rhoBar, rho, invRho, j[3], u[3], jE[3], jSqr, p, omm, pref, ua;
+    __device__ __forceinline__ void init(const T* f, T omega, const T* acc) {
+        moments<T>(f, rhoBar, j);
+        rho = rhoBar + T(1);
+        invRho = T(1) / rho;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            u[d] = j[d] * invRho + T(0.5) * acc[d];
-            jE[d] = rho * u[d];
-        }
-    }
-    const T jSqr = norm_sqr<T>(jE);
-    const T p = T(1.5) * invRho;
-    const T omm = T(1) - omega;
-    EqBar<T> eb;
-    eb.zero();
-#pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        const T eq = equilibrium<T>(i, rhoBar, p, jE, jSqr);
-        fPbar[i] += omm * gb[i];
-        omega_bar += gb[i] * (eq - fP[i]);
-        eb.add(i, omega * gb[i], cdot<T>(i, jE), jSqr);
-    }
-    T rb_bar = eb.sW;
-    T ir_bar = T(1.5) * eb.sQ;
-    T jE_bar[3];
-    eb.j_bar(invRho, jE, jE_bar);
-    T j_bar[3];
-    if (FORCE) {
-        /* fb_i += pref W_i G_i, pref = (1 - omega/2) rho,
-         * G_i = 3 (c_i.a - u.a) + 9 (c_i.u)(c_i.a) */
-        const T pref = (T(1) - T(0.5) * omega) * rho;
-        const T ua = (u[0] * acc[0] + u[1] * acc[1]) + u[2] * acc[2];
-        T sg = T(0), sgw = T(0), sga[3] = {T(0), T(0), T(0)};
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const T ca = cdot<T>(i, acc);
-            const T cu = cdot<T>(i, u);
-            const T G = T(3) * (ca - ua) + T(9) * (cu * ca);
-            const T wg = wgt<T>(i) * gb[i];
-            sg += wg * G;
-            sgw += wg;
+        for (int d = 0; d < 3; ++d) { jE[d] = j[d]; u[d] = T(0); }
+        if (FORCE) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                if (cc(i, d) == 1) sga[d] += wg * ca;
-                else if (cc(i, d) == -1) sga[d] -= wg * ca;
+                u[d] = j[d] * invRho + T(0.5) * acc[d];
+                jE[d] = rho * u[d];
             }
         }
-        omega_bar += T(-0.5) * rho * sg;
-        rb_bar += (T(1) - T(0.5) * omega) * sg;
-        T u_bar[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) u_bar[d] = pref * (T(-3) * acc[d] * sgw + T(9) * sga[d]);
-        /* jE = rho u */
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            rb_bar += jE_bar[d] * u[d];
-            u_bar[d] += rho * jE_bar[d];
-        }
-        /* u = j invRho + a/2 */
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            j_bar[d] = invRho * u_bar[d];
-            ir_bar += u_bar[d] * j[d];
-        }
-    } else {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) j_bar[d] = jE_bar[d];
+        jSqr = norm_sqr<T>(jE);
+        p = T(1.5) * invRho;
+        omm = T(1) - omega;
+        pref = (T(1) - T(0.5) * omega) * rho;
+        ua = (u[0] * acc[0] + u[1] * acc[1]) + u[2] * acc[2];
     }
-    rb_bar += -(invRho * invRho) * ir_bar;
-#pragma unroll
-    for (int i = 0; i < Q; ++i) fPbar[i] += rb_bar + cdot<T>(i, j_bar);
-}
+    __device__ __forceinline__ T eq(int i) const { return equilibrium<T>(i, rhoBar, p, jE, jSqr); }
+    __device__ __forceinline__ T guo(int i, const T* acc) const {
+        const T ca = cdot<T>(i, acc);
+        const T cu = cdot<T>(i, u);
+        return T(3) * (ca - ua) + T(9) * (cu * ca);
+    }
+    __device__ __forceinline__ T fb(int i, T fi, T omega, const T* acc) const {
+        T v = fi * omm + omega * eq(i);
+        if (FORCE) v = v + (pref * wgt<T>(i)) * guo(i, acc);
+        return v;
+    }
+};
 
-/* Adjoint of the PSM update of a cell with a record.  Forward
- * (src/ibCompositeDynamics3D.hh:146-201), with fb = BGK(fP) for a partial
- * cell and fb = fP, B = 1 for a full one:
+/* The adjoint of one cell's update splits in two stages.
+ *
+ * Forward (src/ibCompositeDynamics3D.hh:130-203): fb = BGK(fP); a cell
+ * without a record streams fb.  A cell with a record (fb = fP, B = 1 when
+ * full) streams
  *   out_0 = fP_0 + (1 - B)(fb_0 - fP_0) + B (E_0 - S_0)
- *   out_i = fP_i + (1 - B)(fb_i - fP_i) + B D_i,  i = 1..18,
+ *   out_i = fP_i + (1 - B)(fb_i - fP_i) + B D_i,   i = 1..18,
  *   D_i = (fP_o - E_o) - (fP_i - S_i), o = opposite of i,
- *   F = -sum_{i=1..9} c_i B (D_i - D_{i+9}),
- * E = eq(rho(fb), j(fb)), S = eq(rho(fb), us (1 + rhoBar(fb))).
- * lam: adjoint of out; gF: adjoint of F.  Accumulates fPbar, omega_bar,
- * B_bar, us_bar; returns the recomputed F. */
+ * and produces F = -sum_{i=1..9} c_i B (D_i - D_{i+9}), with
+ * E = eq(rho(fb), j(fb)) and S = eq(rho(fb), us (1 + rhoBar(fb))).
+ *
+ * Stage 1 (record cells only, blend_vjp): given lam = adjoint of out and
+ * gF = adjoint of F, the direct part pb of the adjoint of fP, B_bar, us_bar,
+ * and the adjoint of fb in the form gb_i = coef lam_i + gr + c_i . gj
+ * (coef = 1 - B; a full cell has no BGK, its moment adjoint goes into pb
+ * and coef = gr = gj = 0).
+ * Stage 2 (every cell, bgk_vjp_store): the BGK adjoint of gb (a cell
+ * without a record has gb = lam) plus pb, and omega_bar. */
 template <typename T, bool FORCE>
-__device__ __forceinline__ void psm_vjp(const T* fP, const T* lam, const T* gF, bool full, T B,
-                                        const T* us, T omega, const T* acc, T* fPbar, T& omega_bar,
-                                        T& B_bar, T* us_bar, T* F) {
-    T fb[Q];
-    if (full) {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) fb[i] = fP[i];
-    } else {
-        bgk_all<T, FORCE>(fP, fb, omega, acc);
-    }
+__device__ __forceinline__ void blend_vjp(const T* fP, const T* lam, T* pb, T omega, const T* acc, bool full,
+                                          T B, const T* us, const T* gF, T& B_bar, T* us_bar, T* F, T& coef,
+                                          T& gr, T* gj) {
+    BgkCell<T, FORCE> s;
+    s.init(fP, omega, acc);
     T rhoB, jB[3];
-    moments<T>(fb, rhoB, jB);
-    const T invRho = T(1) / (rhoB + T(1));
+    if (full) {
+        rhoB = s.rhoBar;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) jB[d] = s.j[d];
+    } else {
+        rhoB = T(0);
+        jB[0] = jB[1] = jB[2] = T(0);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const T v = s.fb(i, fP[i], omega, acc);
+            rhoB = rhoB + v;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (cc(i, d) == 1) jB[d] = jB[d] + v;
+                else if (cc(i, d) == -1) jB[d] = jB[d] - v;
+            }
+        }
+    }
+    const T invB = T(1) / (rhoB + T(1));
     const T jSqr = norm_sqr<T>(jB);
-    const T p = T(1.5) * invRho;
+    const T p = T(1.5) * invB;
     const T onePlus = T(1) + rhoB;
     T uP[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) uP[d] = us[d] * onePlus;
     const T uPSqr = norm_sqr<T>(uP);
-    const T oneMinB = T(1) - B;
-
     EqBar<T> e1, e2;
     e1.zero();
     e2.zero();
-    T fbbar[Q];
     T Bb = T(0);
     F[0] = F[1] = F[2] = T(0);
     {
-        const T E0 = eq_rest<T>(rhoB, p, jSqr), S0 = eq_rest<T>(rhoB, p, uPSqr);
-        const T D0 = E0 - S0;
-        fPbar[0] += B * lam[0];
-        fbbar[0] = oneMinB * lam[0];
-        Bb += lam[0] * (fP[0] - fb[0]) + lam[0] * D0;
+        const T fb0 = full ? fP[0] : s.fb(0, fP[0], omega, acc);
+        const T D0 = eq_rest<T>(rhoB, p, jSqr) - eq_rest<T>(rhoB, p, uPSqr);
+        pb[0] = B * lam[0];
+        Bb += lam[0] * (fP[0] - fb0) + lam[0] * D0;
         const T c0 = B * lam[0];
         e1.add(0, c0, T(0), jSqr);
         e2.add(0, -c0, T(0), uPSqr);
@@ -191,27 +164,26 @@ __device__ __forceinline__ void psm_vjp(const T* fP, const T* lam, const T* gF, 
 #pragma unroll
     for (int i = 1; i <= 9; ++i) {
         const int o = i + 9;
+        const T fbi = full ? fP[i] : s.fb(i, fP[i], omega, acc);
+        const T fbo = full ? fP[o] : s.fb(o, fP[o], omega, acc);
         T Ei, Eo, Si, So;
         eq_pair<T>(i, rhoB, p, jB, jSqr, Ei, Eo);
         eq_pair<T>(i, rhoB, p, uP, uPSqr, Si, So);
         const T Di = (fP[o] - Eo) - (fP[i] - Si);
         const T Do = (fP[i] - Ei) - (fP[o] - So);
         const T diff = B * Di - B * Do;
-        T s = T(0);
+        T sg = T(0);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            if (cc(i, d) == 1) { F[d] -= diff; s += gF[d]; }
-            else if (cc(i, d) == -1) { F[d] += diff; s -= gF[d]; }
+            if (cc(i, d) == 1) { F[d] -= diff; sg += gF[d]; }
+            else if (cc(i, d) == -1) { F[d] += diff; sg -= gF[d]; }
         }
-        const T Dbi = lam[i] - s, Dbo = lam[o] + s;
-        fPbar[i] += B * lam[i];
-        fPbar[o] += B * lam[o];
-        fbbar[i] = oneMinB * lam[i];
-        fbbar[o] = oneMinB * lam[o];
-        Bb += (lam[i] * (fP[i] - fb[i]) + lam[o] * (fP[o] - fb[o])) + (Dbi * Di + Dbo * Do);
+        /* adjoints of B*D_i and B*D_o (F = -sum c_i (B D_i - B D_o)) */
+        const T Dbi = lam[i] - sg, Dbo = lam[o] + sg;
+        Bb += (lam[i] * (fP[i] - fbi) + lam[o] * (fP[o] - fbo)) + (Dbi * Di + Dbo * Do);
         const T ci = B * Dbi, co = B * Dbo;
-        fPbar[i] += co - ci;
-        fPbar[o] += ci - co;
+        pb[i] = B * lam[i] + (co - ci);
+        pb[o] = B * lam[o] + (ci - co);
         const T cjB = cdot<T>(i, jB), cjS = cdot<T>(i, uP);
         e1.add(i, -co, cjB, jSqr);
         e1.add(o, -ci, -cjB, jSqr);
@@ -221,22 +193,26 @@ __device__ __forceinline__ void psm_vjp(const T* fP, const T* lam, const T* gF, 
     T rb_bar = e1.sW + e2.sW;
     const T ir_bar = T(1.5) * (e1.sQ + e2.sQ);
     T jB_bar[3], uP_bar[3];
-    e1.j_bar(invRho, jB, jB_bar);
-    e2.j_bar(invRho, uP, uP_bar);
+    e1.j_bar(invB, jB, jB_bar);
+    e2.j_bar(invB, uP, uP_bar);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         us_bar[d] += onePlus * uP_bar[d];
         rb_bar += us[d] * uP_bar[d];
     }
-    rb_bar += -(invRho * invRho) * ir_bar;
-#pragma unroll
-    for (int i = 0; i < Q; ++i) fbbar[i] += rb_bar + cdot<T>(i, jB_bar);
+    rb_bar += -(invB * invB) * ir_bar;
     if (full) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) fPbar[i] += fbbar[i];
+        for (int i = 0; i < Q; ++i) pb[i] += rb_bar + cdot<T>(i, jB_bar);
+        coef = T(0);
+        gr = T(0);
+        gj[0] = gj[1] = gj[2] = T(0);
     } else {
         B_bar += Bb;
-        bgk_vjp<T, FORCE>(fP, fbbar, omega, acc, fPbar, omega_bar);
+        coef = T(1) - B;
+        gr = rb_bar;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gj[d] = jB_bar[d];
     }
 }
 
@@ -263,7 +239,7 @@ struct AdjArgs {
     double omega, ooo;
     double acc[3];
     const uint8_t* sflag;
-    int nb1, nb2;
+    int nb0, nb1, nb2;
     const double* rsf;
     const double* rus;
     const int32_t* rid;
@@ -274,10 +250,115 @@ struct AdjArgs {
     const double* xs;           /* positions of the torque arms */
     double* gacc;               /* 7 per record slot (SoA): gB, gus[3], F x gT[3] */
     double* opart;              /* one omega-gradient partial per workgroup */
+    const uint32_t* bin_live;   /* live records per bin */
+    void* pb;                   /* 19 per record slot (SoA, arithmetic type): stage-1 adjoint of fP */
+    void* pc;                   /* 5 per record slot (SoA): coef, gr, gj[3] */
 };
 
-/* One backward step: lout = adjoint of f(t) from lin = adjoint of f(t+1).
- * Same grid as the forward kernel: blockIdx.x = row, blockIdx.y = z chunk. */
+/* Adjoint of the pushed populations at (x, y, z): pull from x + c_i, or
+ * from the own opposite slot where the push bounced at a wall. */
+template <int PREC>
+__device__ __forceinline__ void pull_adjoint(const typename Store<PREC>::S* __restrict__ lin, const AdjArgs& a,
+                                             int x, int y, int z, typename Store<PREC>::T* lam) {
+    const int64_t row = (int64_t)x * a.ny + y;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int xd = x + cx(i), yd = y + cy(i), zd = z + cz(i);
+        bool wall = false;
+        if (xd < 0 || xd >= a.nx) { if (a.px) xd = xd < 0 ? xd + a.nx : xd - a.nx; else wall = true; }
+        if (yd < 0 || yd >= a.ny) { if (a.py) yd = yd < 0 ? yd + a.ny : yd - a.ny; else wall = true; }
+        if (zd < 0 || zd >= a.nz) { if (a.pz) zd = zd < 0 ? zd + a.nz : zd - a.nz; else wall = true; }
+        if (wall) lam[i] = Store<PREC>::load(lin + pop_at(row, opp(i), z, a.nzp));
+        else lam[i] = Store<PREC>::load(lin + pop_at((int64_t)xd * a.ny + yd, i, zd, a.nzp));
+    }
+}
+
+/* Stage 1 for the cell l of bin b (if it holds a record). */
+template <int PREC, bool FORCE>
+__device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::S* __restrict__ fin,
+                                                    const typename Store<PREC>::S* __restrict__ lin,
+                                                    const AdjArgs& a, int64_t b, int l) {
+    typedef typename Store<PREC>::T T;
+    const int bz = (int)(b % a.nb2);
+    const int by = (int)((b / a.nb2) % a.nb1);
+    const int bx = (int)(b / ((int64_t)a.nb2 * a.nb1));
+    const int x = bx * BIN + (l >> 6), y = by * BIN + ((l >> 3) & 7), z = bz * BIN + (l & 7);
+    if (x >= a.nx || y >= a.ny || z >= a.nz) return;
+    const int64_t row = (int64_t)x * a.ny + y;
+    if (!a.sflag[row * a.nz + z]) return;
+    const int64_t r = b * 512 + l;
+    const T omega = T(a.omega);
+    const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
+    T lam[Q], fP[Q], pb[Q];
+    pull_adjoint<PREC>(lin, a, x, y, z, lam);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load(fin + pop_at(row, i, z, a.nzp));
+    const double sf = a.rsf[r];
+    const bool full = sf > SOLFRAC_MAX;
+    const T B = full ? T(1) : T((sf * a.ooo) / ((1.0 - sf) + a.ooo));
+    const T us[3] = {T(a.rus[r]), T(a.rus[a.rcap + r]), T(a.rus[2 * a.rcap + r])};
+    T gF[3] = {T(0), T(0), T(0)};
+    double gT[3] = {0.0, 0.0, 0.0};
+    bool obj = false;
+    if (a.gft) {
+        const int id = a.rid[r];
+        const int o = (id >= 1 && id < a.id_map_size) ? a.id_map[id] : -1;
+        if (o >= 0) {
+            obj = true;
+            const double d[3] = {min_image((double)x - a.xs[3 * o], a.nx),
+                                 min_image((double)y - a.xs[3 * o + 1], a.ny),
+                                 min_image((double)z - a.xs[3 * o + 2], a.nz)};
+            const double* g = a.gft + 6 * (int64_t)o;
+            gT[0] = g[3]; gT[1] = g[4]; gT[2] = g[5];
+            /* d(gT . (d x F))/dF = gT x d */
+            gF[0] = T(g[0] + (gT[1] * d[2] - gT[2] * d[1]));
+            gF[1] = T(g[1] + (gT[2] * d[0] - gT[0] * d[2]));
+            gF[2] = T(g[2] + (gT[0] * d[1] - gT[1] * d[0]));
+        }
+    }
+    T Bb = T(0), usb[3] = {T(0), T(0), T(0)}, F[3], coef, gr, gj[3];
+    blend_vjp<T, FORCE>(fP, lam, pb, omega, acc, full, B, us, gF, Bb, usb, F, coef, gr, gj);
+    T* pbo = (T*)a.pb;
+    T* pco = (T*)a.pc;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) pbo[i * a.rcap + r] = pb[i];
+    pco[r] = coef;
+    pco[a.rcap + r] = gr;
+    pco[2 * a.rcap + r] = gj[0];
+    pco[3 * a.rcap + r] = gj[1];
+    pco[4 * a.rcap + r] = gj[2];
+    double* g = a.gacc;
+    if (!full) g[r] += (double)Bb;
+    g[a.rcap + r] += (double)usb[0];
+    g[2 * a.rcap + r] += (double)usb[1];
+    g[3 * a.rcap + r] += (double)usb[2];
+    if (obj) {
+        const double Fx = F[0], Fy = F[1], Fz = F[2];
+        g[4 * a.rcap + r] += Fy * gT[2] - Fz * gT[1];
+        g[5 * a.rcap + r] += Fz * gT[0] - Fx * gT[2];
+        g[6 * a.rcap + r] += Fx * gT[1] - Fy * gT[0];
+    }
+}
+
+/* Stage 1: one workgroup per 8^3 bin holding records, two cells per lane
+ * (256 lanes: the register budget of a lane is not halved by a 512-lane
+ * workgroup, and only the record cells do work). */
+template <int PREC, bool FORCE>
+__global__ __launch_bounds__(256) void k_adjoint_records(const typename Store<PREC>::S* __restrict__ fin,
+                                                         const typename Store<PREC>::S* __restrict__ lin,
+                                                         AdjArgs a) {
+    const int64_t b = blockIdx.x;
+    if (a.bin_live[b] == 0u) return;
+    for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) adjoint_record_cell<PREC, FORCE>(fin, lin, a, b, l);
+}
+
+/* Stage 2, every cell: lout = adjoint of f(t) from lin = adjoint of f(t+1).
+ * Same grid as the forward kernel: blockIdx.x = row, blockIdx.y = z chunk.
+ * The adjoint of fb is gb_i = coef lam_i + gr + c_i . gj (lam itself for a
+ * cell without a record); the BGK adjoint is
+ *   fPbar_i = pb_i + (1 - omega) gb_i + rb + c_i . jb,
+ * with (rb, jb) the adjoint of the pre-collision moments, two passes over
+ * gb (sums, then stores). */
 template <int PREC, bool FORCE>
 __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* __restrict__ fin,
                                                  const typename Store<PREC>::S* __restrict__ lin,
@@ -287,70 +368,84 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
     const int row = blockIdx.x;
     const int z = blockIdx.y * blockDim.x + threadIdx.x;
     const bool active = z < a.nz;
-    const int x = row / a.ny;
-    const int y = row - x * a.ny;
+    const int x = __builtin_amdgcn_readfirstlane(row / a.ny);
+    const int y = __builtin_amdgcn_readfirstlane(row - x * a.ny);
     double wbar = 0.0;
     if (active) {
         const T omega = T(a.omega);
         const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
-        T lam[Q], fP[Q], fPbar[Q];
+        T lam[Q], fP[Q];
+        pull_adjoint<PREC>(lin, a, x, y, z, lam);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load(fin + pop_at(row, i, z, a.nzp));
+        const bool rec = a.sflag[(int64_t)row * a.nz + z] != 0;
+        int64_t r = 0;
+        T coef = T(1), gr = T(0), gj[3] = {T(0), T(0), T(0)};
+        if (rec) {
+            r = rec_index(x, y, z, a.nb1, a.nb2);
+            const T* pc = (const T*)a.pc;
+            coef = pc[r];
+            gr = pc[a.rcap + r];
+            gj[0] = pc[2 * a.rcap + r];
+            gj[1] = pc[3 * a.rcap + r];
+            gj[2] = pc[4 * a.rcap + r];
+        }
+        BgkCell<T, FORCE> s;
+        s.init(fP, omega, acc);
+        EqBar<T> eb;
+        eb.zero();
+        T om = T(0), sgG = T(0), sgw = T(0), sga[3] = {T(0), T(0), T(0)};
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            int xd = x + cx(i), yd = y + cy(i), zd = z + cz(i);
-            bool wall = false;
-            if (xd < 0 || xd >= a.nx) { if (a.px) xd = xd < 0 ? xd + a.nx : xd - a.nx; else wall = true; }
-            if (yd < 0 || yd >= a.ny) { if (a.py) yd = yd < 0 ? yd + a.ny : yd - a.ny; else wall = true; }
-            if (zd < 0 || zd >= a.nz) { if (a.pz) zd = zd < 0 ? zd + a.nz : zd - a.nz; else wall = true; }
-            if (wall) lam[i] = Store<PREC>::load(lin + pop_at(row, opp(i), z, a.nzp));
-            else lam[i] = Store<PREC>::load(lin + pop_at((int64_t)xd * a.ny + yd, i, zd, a.nzp));
-            fP[i] = Store<PREC>::load(fin + pop_at(row, i, z, a.nzp));
-            fPbar[i] = T(0);
-        }
-        T om = T(0);
-        const int64_t c = (int64_t)row * a.nz + z;
-        if (!a.sflag[c]) {
-            bgk_vjp<T, FORCE>(fP, lam, omega, acc, fPbar, om);
-        } else {
-            const int64_t r = rec_index(x, y, z, a.nb1, a.nb2);
-            const double sf = a.rsf[r];
-            const bool full = sf > SOLFRAC_MAX;
-            const T B = full ? T(1) : T((sf * a.ooo) / ((1.0 - sf) + a.ooo));
-            const T us[3] = {T(a.rus[r]), T(a.rus[a.rcap + r]), T(a.rus[2 * a.rcap + r])};
-            T gF[3] = {T(0), T(0), T(0)};
-            double gT[3] = {0.0, 0.0, 0.0};
-            bool obj = false;
-            if (a.gft) {
-                const int id = a.rid[r];
-                const int o = (id >= 1 && id < a.id_map_size) ? a.id_map[id] : -1;
-                if (o >= 0) {
-                    obj = true;
-                    const double d[3] = {min_image((double)x - a.xs[3 * o], a.nx),
-                                         min_image((double)y - a.xs[3 * o + 1], a.ny),
-                                         min_image((double)z - a.xs[3 * o + 2], a.nz)};
-                    const double* g = a.gft + 6 * (int64_t)o;
-                    gT[0] = g[3]; gT[1] = g[4]; gT[2] = g[5];
-                    /* d(gT . (d x F))/dF = gT x d */
-                    gF[0] = T(g[0] + (gT[1] * d[2] - gT[2] * d[1]));
-                    gF[1] = T(g[1] + (gT[2] * d[0] - gT[0] * d[2]));
-                    gF[2] = T(g[2] + (gT[0] * d[1] - gT[1] * d[0]));
+            const T gb = coef * lam[i] + gr + cdot<T>(i, gj);
+            om += gb * (s.eq(i) - fP[i]);
+            eb.add(i, omega * gb, cdot<T>(i, s.jE), s.jSqr);
+            if (FORCE) {
+                const T wg = wgt<T>(i) * gb;
+                const T ca = cdot<T>(i, acc);
+                sgG += wg * s.guo(i, acc);
+                sgw += wg;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (cc(i, d) == 1) sga[d] += wg * ca;
+                    else if (cc(i, d) == -1) sga[d] -= wg * ca;
                 }
             }
-            T Bb = T(0), usb[3] = {T(0), T(0), T(0)}, F[3];
-            psm_vjp<T, FORCE>(fP, lam, gF, full, B, us, omega, acc, fPbar, om, Bb, usb, F);
-            double* g = a.gacc;
-            if (!full) g[r] += (double)Bb;
-            g[a.rcap + r] += (double)usb[0];
-            g[2 * a.rcap + r] += (double)usb[1];
-            g[3 * a.rcap + r] += (double)usb[2];
-            if (obj) {
-                const double Fx = F[0], Fy = F[1], Fz = F[2];
-                g[4 * a.rcap + r] += Fy * gT[2] - Fz * gT[1];
-                g[5 * a.rcap + r] += Fz * gT[0] - Fx * gT[2];
-                g[6 * a.rcap + r] += Fx * gT[1] - Fy * gT[0];
-            }
         }
+        T rb2 = eb.sW;
+        T ir2 = T(1.5) * eb.sQ;
+        T jE_bar[3], j2[3];
+        eb.j_bar(s.invRho, s.jE, jE_bar);
+        if (FORCE) {
+            /* fb_i += pref W_i G_i: pref = (1 - omega/2) rho, G_i depends on
+             * u; jE = rho u, u = j invRho + a/2 */
+            om += T(-0.5) * s.rho * sgG;
+            rb2 += (T(1) - T(0.5) * omega) * sgG;
+            T u_bar[3];
 #pragma unroll
-        for (int i = 0; i < Q; ++i) Store<PREC>::store(lout + pop_at(row, i, z, a.nzp), fPbar[i]);
+            for (int d = 0; d < 3; ++d) {
+                u_bar[d] = s.pref * (T(-3) * acc[d] * sgw + T(9) * sga[d]);
+                rb2 += jE_bar[d] * s.u[d];
+                u_bar[d] += s.rho * jE_bar[d];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                j2[d] = s.invRho * u_bar[d];
+                ir2 += u_bar[d] * s.j[d];
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) j2[d] = jE_bar[d];
+        }
+        rb2 += -(s.invRho * s.invRho) * ir2;
+        const T* pb = (const T*)a.pb;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const T gb = coef * lam[i] + gr + cdot<T>(i, gj);
+            T v = s.omm * gb + (rb2 + cdot<T>(i, j2));
+            if (rec) v = pb[i * a.rcap + r] + v;
+            Store<PREC>::store(lout + pop_at(row, i, z, a.nzp), v);
+        }
         wbar = (double)om;
     }
     const double s = block_sum_256(wbar, sh);
@@ -582,7 +677,8 @@ int diff_free(lbdem_psm* h) {
     dfree_d(D.lam[0]); dfree_d(D.lam[1]);
     D.lam[0] = D.lam[1] = nullptr;
     dfree_d(D.d_ft); dfree_d(D.gacc); dfree_d(D.Fscr); dfree_d(D.opart); dfree_d(D.gpart);
-    dfree_d(D.cell_out); dfree_d(D.tau_part);
+    dfree_d(D.cell_out); dfree_d(D.tau_part); dfree_d(D.pb); dfree_d(D.pc);
+    D.pb = D.pc = nullptr;
     D.d_ft = D.gacc = D.Fscr = D.opart = D.gpart = D.cell_out = D.tau_part = nullptr;
     D.opart_n = D.tau_part_n = 0;
     D.ready = false;
@@ -608,22 +704,36 @@ static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lo
     a.omega = h->omega; a.ooo = h->ooo;
     for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
     a.sflag = h->sflag;
-    a.nb1 = h->g.nb[1]; a.nb2 = h->g.nb[2];
+    a.nb0 = h->g.nb[0]; a.nb1 = h->g.nb[1]; a.nb2 = h->g.nb[2];
     a.rsf = h->rec.sf; a.rus = h->rec.us; a.rid = h->rec.id; a.rcap = h->rec_cap;
     a.gft = gft;
     a.id_map = h->id_map; a.id_map_size = h->id_map_size;
     a.xs = h->psum_x;
     a.gacc = h->diff.gacc;
     a.opart = h->diff.opart;
+    a.bin_live = h->bin_any[h->cur_any];
+    a.pb = h->diff.pb;
+    a.pc = h->diff.pc;
+    dim3 gridr((unsigned)h->nbins), blockr(256);
     dim3 grid((unsigned)(a.nx * a.ny), (unsigned)((a.nz + 255) / 256)), block(256);
     if (h->precision == LBDEM_PSM_FP64) {
         typedef double S;
-        if (h->has_force) hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
-        else hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+        if (h->has_force) {
+            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP64, true>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a);
+            hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+        } else {
+            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP64, false>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a);
+            hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+        }
     } else {
         typedef float S;
-        if (h->has_force) hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
-        else hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+        if (h->has_force) {
+            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP32_DEV, true>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a);
+            hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+        } else {
+            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP32_DEV, false>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a);
+            hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+        }
     }
     PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
@@ -754,6 +864,9 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
         if (!D.lam[k]) PSM_HIP(h, dalloc_d(&D.lam[k], h->pop_bytes));
     if (!D.gacc) PSM_HIP(h, dalloc_d(&D.gacc, 7 * rcap * sizeof(double)));
     if (!D.Fscr) PSM_HIP(h, dalloc_d(&D.Fscr, 3 * rcap * sizeof(double)));
+    const size_t tsz = h->precision == LBDEM_PSM_FP64 ? sizeof(double) : sizeof(float);
+    if (!D.pb) PSM_HIP(h, dalloc_d(&D.pb, (size_t)Q * rcap * tsz));
+    if (!D.pc) PSM_HIP(h, dalloc_d(&D.pc, (size_t)5 * rcap * tsz));
     const int64_t nblk = (int64_t)h->g.n[0] * h->g.n[1] * ((h->g.n[2] + 255) / 256);
     if (D.opart_n < nblk + 8) {
         dfree_d(D.opart);

@@ -375,6 +375,8 @@ struct lbdem_psm {
         double* cell_out = nullptr;  /* 2*Nc dense gsf, gB */
         double* tau_part = nullptr;
         int64_t tau_part_n = 0;
+        void* pb = nullptr;          /* 19 per record slot: stage-1 adjoint of the record cells */
+        void* pc = nullptr;          /* 5 per record slot: coefficients of the BGK adjoint */
     } diff;
 
     /* profiling */

@@ -264,3 +264,57 @@ def test_window_errors():
                   np.array([1]))
     with pytest.raises(L.PSMError):
         h.diff_forward(1, 1)
+
+
+def test_adjoint_differentiable_channel_full_size():
+    """The 128x128x256 differentiable channel (walls x/y, body force 1e-6 along
+    z, 200 spheres r=5, v~N(0,0.005)), adjoint of 500 steps with a checkpoint
+    every 50: d(total z-drag) against central differences (eps 1e-6, FP64)
+    on 20 random parameters, for the FP64 and the FP32-deviation adjoint."""
+    n, nsteps, K = (128, 128, 256), 500, 50
+    x, v, w, r, ids = L.generate_spheres(n, 200, L.RADIUS_CONST, a=5.0, periodic=(0, 0, 1), gap=0.5,
+                                         wall_gap=1.5, sigma_v=0.005, sigma_w=0.0, seed=44)
+    grads = {}
+    for prec in ("fp64", "fp32_dev"):
+        lat = L.Lattice(*n, periodic=(0, 0, 1), tau=0.8, precision=prec, body_accel=(0, 0, 1e-6),
+                        voxel_mode="overlap")
+        lat.set_spheres(x, v, w, r, ids)
+        ft = lat.diff_forward(nsteps, K)
+        G = np.zeros_like(ft)
+        G[:, :, 2] = 1.0
+        grads[prec] = lat.diff_backward(G)
+        if prec == "fp64":
+            rec = lat.cell_records()
+        lat.close()
+    lat = L.Lattice(*n, periodic=(0, 0, 1), tau=0.8, precision="fp64", body_accel=(0, 0, 1e-6),
+                    voxel_mode="overlap")
+    lat.init_equilibrium()
+    f0 = lat.get_populations()
+    rng = np.random.default_rng(13)
+    partial = np.nonzero((rec["sf"] > 0.05) & (rec["sf"] < 0.95))[0]
+    params = ([("x", (int(rng.integers(200)), int(rng.integers(3)))) for _ in range(5)]
+              + [("v", (int(rng.integers(200)), int(rng.integers(3)))) for _ in range(6)]
+              + [("w", (int(rng.integers(200)), int(rng.integers(3)))) for _ in range(4)]
+              + [("tau", ())]
+              + [("sf", (int(c),)) for c in rng.choice(partial, 4, replace=False)])
+    assert len(params) == 20
+
+    def J(name, ix, dv):
+        a = {"x": x.copy(), "v": v.copy(), "w": w.copy(), "tau": np.array(0.8), "sf": np.zeros(lat.nc)}
+        a[name][ix] += dv
+        lat.set_tau(float(a["tau"]))
+        lat.set_populations(f0)
+        lat.clear_records()
+        lat.set_spheres(a["x"], a["v"], a["w"], r, ids)
+        if name == "sf":
+            lat.perturb_solid_fraction(a["sf"])
+        return float(lat.diff_forward(nsteps, nsteps)[:, :, 2].sum())
+
+    eps = 1e-6
+    for name, ix in params:
+        fd = (J(name, ix, eps) - J(name, ix, -eps)) / (2 * eps)
+        g64 = float(np.asarray(grads["fp64"][name])[ix])
+        g32 = float(np.asarray(grads["fp32_dev"][name])[ix])
+        scale = np.abs(grads["fp64"][name]).max()
+        assert abs(g64 - fd) <= 1e-5 * max(abs(fd), 1e-3 * scale), (name, ix, g64, fd)
+        assert abs(g32 - fd) <= 2e-3 * max(abs(fd), 1e-2 * scale), (name, ix, g32, fd)
