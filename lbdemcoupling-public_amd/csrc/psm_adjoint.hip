@@ -273,6 +273,35 @@ __device__ __forceinline__ void pull_adjoint(const typename Store<PREC>::S* __re
     }
 }
 
+/* pull_adjoint for a wavefront-uniform (x, y): the row-block bases are
+ * uniform (scalar registers), only the 32-bit z offsets are per lane. */
+template <int PREC, bool PZ>
+__device__ __forceinline__ void pull_adjoint_row(const typename Store<PREC>::S* __restrict__ lin, const AdjArgs& a,
+                                                 int x, int y, int z, typename Store<PREC>::T* lam) {
+    typedef typename Store<PREC>::S S;
+    const int64_t row = (int64_t)x * a.ny + y;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        int xd = x + cx(i), yd = y + cy(i);
+        bool wall = false;
+        if (xd < 0 || xd >= a.nx) { if (a.px) xd = xd < 0 ? xd + a.nx : xd - a.nx; else wall = true; }
+        if (yd < 0 || yd >= a.ny) { if (a.py) yd = yd < 0 ? yd + a.ny : yd - a.ny; else wall = true; }
+        int zd = z + cz(i);
+        bool zwall = false;
+        if (cz(i) != 0) {
+            if (PZ) zd = zd < 0 ? zd + a.nz : (zd >= a.nz ? zd - a.nz : zd);
+            else zwall = zd < 0 || zd >= a.nz;
+        }
+        if (wall || zwall) {
+            const char* own = (const char*)(lin + pop_at(row, opp(i), 0, a.nzp));
+            lam[i] = Store<PREC>::load((const S*)(own + (uint32_t)z * (uint32_t)sizeof(S)));
+        } else {
+            const char* src = (const char*)(lin + pop_at((int64_t)xd * a.ny + yd, i, 0, a.nzp));
+            lam[i] = Store<PREC>::load((const S*)(src + (uint32_t)zd * (uint32_t)sizeof(S)));
+        }
+    }
+}
+
 /* Stage 1 for the cell l of bin b (if it holds a record). */
 template <int PREC, bool FORCE>
 __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::S* __restrict__ fin,
@@ -340,16 +369,45 @@ __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::
     }
 }
 
-/* Stage 1: one workgroup per 8^3 bin holding records, two cells per lane
- * (256 lanes: the register budget of a lane is not halved by a 512-lane
- * workgroup, and only the record cells do work). */
+/* Stage 1 over the compact list of record slots (one lane per record). */
 template <int PREC, bool FORCE>
 __global__ __launch_bounds__(256) void k_adjoint_records(const typename Store<PREC>::S* __restrict__ fin,
                                                          const typename Store<PREC>::S* __restrict__ lin,
-                                                         AdjArgs a) {
+                                                         AdjArgs a, const int32_t* __restrict__ list,
+                                                         int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int64_t r = list[k];
+    adjoint_record_cell<PREC, FORCE>(fin, lin, a, r >> 9, (int)(r & 511));
+}
+
+/* The record slots of the frozen geometry, appended per wavefront (one
+ * atomic per wavefront; the order is irrelevant, every record writes only
+ * its own slots). */
+__global__ __launch_bounds__(256) void k_list_records(const uint8_t* __restrict__ sflag,
+                                                      const uint32_t* __restrict__ bin_live, int n0, int n1,
+                                                      int n2, int nb1, int nb2, int32_t* __restrict__ list,
+                                                      unsigned long long* __restrict__ count, int64_t cap) {
     const int64_t b = blockIdx.x;
-    if (a.bin_live[b] == 0u) return;
-    for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) adjoint_record_cell<PREC, FORCE>(fin, lin, a, b, l);
+    if (bin_live[b] == 0u) return;
+    const int bz = (int)(b % nb2);
+    const int by = (int)((b / nb2) % nb1);
+    const int bx = (int)(b / ((int64_t)nb2 * nb1));
+    const int lane = (int)__lane_id();
+    for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) {
+        const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
+        const bool live = X < n0 && Y < n1 && Z < n2 && sflag[((int64_t)X * n1 + Y) * n2 + Z] != 0;
+        const unsigned long long m = __ballot(live);
+        if (m == 0ull) continue;
+        unsigned long long base = 0;
+        const int leader = __ffsll((long long)m) - 1;
+        if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+        base = __shfl(base, leader);
+        if (live) {
+            const unsigned long long pos = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+            if ((int64_t)pos < cap) list[pos] = (int32_t)(b * 512 + l);
+        }
+    }
 }
 
 /* Stage 2, every cell: lout = adjoint of f(t) from lin = adjoint of f(t+1).
@@ -359,7 +417,7 @@ __global__ __launch_bounds__(256) void k_adjoint_records(const typename Store<PR
  *   fPbar_i = pb_i + (1 - omega) gb_i + rb + c_i . jb,
  * with (rb, jb) the adjoint of the pre-collision moments, two passes over
  * gb (sums, then stores). */
-template <int PREC, bool FORCE>
+template <int PREC, bool FORCE, bool PZ>
 __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* __restrict__ fin,
                                                  const typename Store<PREC>::S* __restrict__ lin,
                                                  typename Store<PREC>::S* __restrict__ lout, AdjArgs a) {
@@ -374,11 +432,15 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
     if (active) {
         const T omega = T(a.omega);
         const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
+        typedef typename Store<PREC>::S S;
         T lam[Q], fP[Q];
-        pull_adjoint<PREC>(lin, a, x, y, z, lam);
+        pull_adjoint_row<PREC, PZ>(lin, a, x, y, z, lam);
+        const char* frow = (const char*)(fin + pop_at(row, 0, 0, a.nzp));
+        const uint32_t qstride = (uint32_t)a.nzp * (uint32_t)sizeof(S);
+        const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(S);
 #pragma unroll
-        for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load(fin + pop_at(row, i, z, a.nzp));
-        const bool rec = a.sflag[(int64_t)row * a.nz + z] != 0;
+        for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load((const S*)(frow + i * qstride + zoff));
+        const bool rec = a.sflag[(uint32_t)row * (uint32_t)a.nz + (uint32_t)z] != 0;
         int64_t r = 0;
         T coef = T(1), gr = T(0), gj[3] = {T(0), T(0), T(0)};
         if (rec) {
@@ -398,6 +460,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const T gb = coef * lam[i] + gr + cdot<T>(i, gj);
+            lam[i] = gb;
             om += gb * (s.eq(i) - fP[i]);
             eb.add(i, omega * gb, cdot<T>(i, s.jE), s.jSqr);
             if (FORCE) {
@@ -438,14 +501,19 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
             for (int d = 0; d < 3; ++d) j2[d] = jE_bar[d];
         }
         rb2 += -(s.invRho * s.invRho) * ir2;
-        const T* pb = (const T*)a.pb;
 #pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const T gb = coef * lam[i] + gr + cdot<T>(i, gj);
-            T v = s.omm * gb + (rb2 + cdot<T>(i, j2));
-            if (rec) v = pb[i * a.rcap + r] + v;
-            Store<PREC>::store(lout + pop_at(row, i, z, a.nzp), v);
+        for (int i = 0; i < Q; ++i) lam[i] = s.omm * lam[i] + (rb2 + cdot<T>(i, j2));
+        if (rec) {
+            /* the record cell's direct part, all loads in flight before the adds */
+            const T* pb = (const T*)a.pb;
+#pragma unroll
+            for (int i = 0; i < Q; ++i) fP[i] = pb[i * a.rcap + r];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) lam[i] = fP[i] + lam[i];
         }
+        char* orow = (char*)(lout + pop_at(row, 0, 0, a.nzp));
+#pragma unroll
+        for (int i = 0; i < Q; ++i) Store<PREC>::store((S*)(orow + i * qstride + zoff), lam[i]);
         wbar = (double)om;
     }
     const double s = block_sum_256(wbar, sh);
@@ -678,7 +746,11 @@ int diff_free(lbdem_psm* h) {
     D.lam[0] = D.lam[1] = nullptr;
     dfree_d(D.d_ft); dfree_d(D.gacc); dfree_d(D.Fscr); dfree_d(D.opart); dfree_d(D.gpart);
     dfree_d(D.cell_out); dfree_d(D.tau_part); dfree_d(D.pb); dfree_d(D.pc);
+    dfree_d(D.rec_list); dfree_d(D.d_nrec);
     D.pb = D.pc = nullptr;
+    D.rec_list = nullptr;
+    D.d_nrec = nullptr;
+    D.nrec = 0;
     D.d_ft = D.gacc = D.Fscr = D.opart = D.gpart = D.cell_out = D.tau_part = nullptr;
     D.opart_n = D.tau_part_n = 0;
     D.ready = false;
@@ -714,25 +786,31 @@ static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lo
     a.bin_live = h->bin_any[h->cur_any];
     a.pb = h->diff.pb;
     a.pc = h->diff.pc;
-    dim3 gridr((unsigned)h->nbins), blockr(256);
+    const int64_t nrec = h->diff.nrec;
+    dim3 gridr((unsigned)std::max<int64_t>(1, (nrec + 255) / 256)), blockr(256);
+    const int32_t* list = h->diff.rec_list;
     dim3 grid((unsigned)(a.nx * a.ny), (unsigned)((a.nz + 255) / 256)), block(256);
     if (h->precision == LBDEM_PSM_FP64) {
         typedef double S;
         if (h->has_force) {
-            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP64, true>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a);
-            hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP64, true>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a, list, nrec);
+            if (a.pz) hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, true, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+            else hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, true, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
         } else {
-            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP64, false>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a);
-            hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP64, false>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a, list, nrec);
+            if (a.pz) hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, false, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+            else hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP64, false, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
         }
     } else {
         typedef float S;
         if (h->has_force) {
-            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP32_DEV, true>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a);
-            hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP32_DEV, true>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a, list, nrec);
+            if (a.pz) hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, true, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+            else hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, true, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
         } else {
-            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP32_DEV, false>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a);
-            hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+            hipLaunchKernelGGL((k_adjoint_records<LBDEM_PSM_FP32_DEV, false>), gridr, blockr, 0, h->stream, (const S*)fin, (const S*)lin, a, list, nrec);
+            if (a.pz) hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, false, true>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
+            else hipLaunchKernelGGL((k_adjoint<LBDEM_PSM_FP32_DEV, false, false>), grid, block, 0, h->stream, (const S*)fin, (const S*)lin, (S*)lout, a);
         }
     }
     PSM_HIP(h, hipGetLastError());
@@ -813,6 +891,14 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
     PSM_HIP(h, dalloc_d(&D.d_ft, (size_t)nsteps * (size_t)std::max<int64_t>(np, 1) * 6 * sizeof(double)));
     if (np > 0) PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->px, 3 * np * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
+    /* record slots of the frozen geometry, for the adjoint's first stage */
+    if (!D.rec_list) PSM_HIP(h, dalloc_d(&D.rec_list, h->rec_cap * sizeof(int32_t)));
+    if (!D.d_nrec) PSM_HIP(h, dalloc_d(&D.d_nrec, sizeof(unsigned long long)));
+    PSM_HIP(h, hipMemsetAsync(D.d_nrec, 0, sizeof(unsigned long long), h->stream));
+    hipLaunchKernelGGL(k_list_records, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, h->sflag,
+                       h->bin_any[h->cur_any], h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2], D.rec_list,
+                       D.d_nrec, h->rec_cap);
+    PSM_HIP(h, hipGetLastError());
     for (int t = 0; t < nsteps; ++t) {
         if (t % K == 0)
             PSM_HIP(h, hipMemcpyAsync(D.ckpt[t / K], h->f[h->cur_f], h->pop_bytes, hipMemcpyDeviceToDevice, h->stream));
@@ -825,7 +911,10 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
                                       hipMemcpyDeviceToDevice, h->stream));
         }
     }
+    unsigned long long nrec = 0;
+    PSM_HIP(h, hipMemcpyAsync(&nrec, D.d_nrec, sizeof(nrec), hipMemcpyDeviceToHost, h->stream));
     PSM_HIP(h, hipStreamSynchronize(h->stream));
+    D.nrec = std::min<int64_t>((int64_t)nrec, h->rec_cap);
     if (ft && np > 0) {
         std::vector<double> tmp((size_t)nsteps * np * 6);
         PSM_HIP(h, hipMemcpy(tmp.data(), D.d_ft, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));

@@ -377,6 +377,9 @@ struct lbdem_psm {
         int64_t tau_part_n = 0;
         void* pb = nullptr;          /* 19 per record slot: stage-1 adjoint of the record cells */
         void* pc = nullptr;          /* 5 per record slot: coefficients of the BGK adjoint */
+        int32_t* rec_list = nullptr; /* record slots of the frozen geometry */
+        unsigned long long* d_nrec = nullptr;
+        int64_t nrec = 0;
     } diff;
 
     /* profiling */
