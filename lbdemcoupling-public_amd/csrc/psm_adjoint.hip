@@ -222,24 +222,25 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
-/* Fixed-order sum over a 256-thread block; the result is valid in thread 0. */
+/* Fixed-order sum over a block of at most 256 threads (a multiple of 64);
+ * the result is valid in thread 0. */
 __device__ __forceinline__ double block_sum_256(double v, double* sh) {
     v = wave_sum_d(v);
     const int wave = threadIdx.x >> 6;
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[wave] = v;
     __syncthreads();
-    return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    const int nw = (int)(blockDim.x >> 6);
+    return (sh[0] + (nw > 1 ? sh[1] : 0.0)) + ((nw > 2 ? sh[2] : 0.0) + (nw > 3 ? sh[3] : 0.0));
 }
 
 struct AdjArgs {
     int nx, ny, nz;
     int px, py, pz;
-    int64_t nzp;
+    PopLayout L;
     double omega, ooo;
     double acc[3];
     const uint8_t* sflag;
-    int nb0, nb1, nb2;
     const double* rsf;
     const double* rus;
     const int32_t* rid;
@@ -268,8 +269,8 @@ __device__ __forceinline__ void pull_adjoint(const typename Store<PREC>::S* __re
         if (xd < 0 || xd >= a.nx) { if (a.px) xd = xd < 0 ? xd + a.nx : xd - a.nx; else wall = true; }
         if (yd < 0 || yd >= a.ny) { if (a.py) yd = yd < 0 ? yd + a.ny : yd - a.ny; else wall = true; }
         if (zd < 0 || zd >= a.nz) { if (a.pz) zd = zd < 0 ? zd + a.nz : zd - a.nz; else wall = true; }
-        if (wall) lam[i] = Store<PREC>::load(lin + pop_at(row, opp(i), z, a.nzp));
-        else lam[i] = Store<PREC>::load(lin + pop_at((int64_t)xd * a.ny + yd, i, zd, a.nzp));
+        if (wall) lam[i] = Store<PREC>::load(lin + pop_at(row, opp(i), z, a.L));
+        else lam[i] = Store<PREC>::load(lin + pop_at((int64_t)xd * a.ny + yd, i, zd, a.L));
     }
 }
 
@@ -293,35 +294,31 @@ __device__ __forceinline__ void pull_adjoint_row(const typename Store<PREC>::S* 
             else zwall = zd < 0 || zd >= a.nz;
         }
         if (wall || zwall) {
-            const char* own = (const char*)(lin + pop_at(row, opp(i), 0, a.nzp));
-            lam[i] = Store<PREC>::load((const S*)(own + (uint32_t)z * (uint32_t)sizeof(S)));
+            const char* own = (const char*)(lin + pop_at(row, opp(i), 0, a.L));
+            lam[i] = Store<PREC>::load((const S*)(own + a.L.zoff(z) * (uint32_t)sizeof(S)));
         } else {
-            const char* src = (const char*)(lin + pop_at((int64_t)xd * a.ny + yd, i, 0, a.nzp));
-            lam[i] = Store<PREC>::load((const S*)(src + (uint32_t)zd * (uint32_t)sizeof(S)));
+            const char* src = (const char*)(lin + pop_at((int64_t)xd * a.ny + yd, i, 0, a.L));
+            lam[i] = Store<PREC>::load((const S*)(src + a.L.zoff(zd) * (uint32_t)sizeof(S)));
         }
     }
 }
 
-/* Stage 1 for the cell l of bin b (if it holds a record). */
+/* Stage 1 for the record slot r (= cell index) if it holds a record. */
 template <int PREC, bool FORCE>
 __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::S* __restrict__ fin,
                                                     const typename Store<PREC>::S* __restrict__ lin,
-                                                    const AdjArgs& a, int64_t b, int l) {
+                                                    const AdjArgs& a, int64_t r) {
     typedef typename Store<PREC>::T T;
-    const int bz = (int)(b % a.nb2);
-    const int by = (int)((b / a.nb2) % a.nb1);
-    const int bx = (int)(b / ((int64_t)a.nb2 * a.nb1));
-    const int x = bx * BIN + (l >> 6), y = by * BIN + ((l >> 3) & 7), z = bz * BIN + (l & 7);
-    if (x >= a.nx || y >= a.ny || z >= a.nz) return;
+    int x, y, z;
+    rec_cell(r, a.ny, a.nz, x, y, z);
     const int64_t row = (int64_t)x * a.ny + y;
-    if (!a.sflag[row * a.nz + z]) return;
-    const int64_t r = b * 512 + l;
+    if (!a.sflag[r]) return;
     const T omega = T(a.omega);
     const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
     T lam[Q], fP[Q], pb[Q];
     pull_adjoint<PREC>(lin, a, x, y, z, lam);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load(fin + pop_at(row, i, z, a.nzp));
+    for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load(fin + pop_at(row, i, z, a.L));
     const double sf = a.rsf[r];
     const bool full = sf > SOLFRAC_MAX;
     const T B = full ? T(1) : T((sf * a.ooo) / ((1.0 - sf) + a.ooo));
@@ -377,8 +374,7 @@ __global__ __launch_bounds__(256) void k_adjoint_records(const typename Store<PR
                                                          int64_t n) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    const int64_t r = list[k];
-    adjoint_record_cell<PREC, FORCE>(fin, lin, a, r >> 9, (int)(r & 511));
+    adjoint_record_cell<PREC, FORCE>(fin, lin, a, (int64_t)list[k]);
 }
 
 /* The record slots of the frozen geometry, appended per wavefront (one
@@ -405,13 +401,14 @@ __global__ __launch_bounds__(256) void k_list_records(const uint8_t* __restrict_
         base = __shfl(base, leader);
         if (live) {
             const unsigned long long pos = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
-            if ((int64_t)pos < cap) list[pos] = (int32_t)(b * 512 + l);
+            if ((int64_t)pos < cap) list[pos] = (int32_t)rec_index(X, Y, Z, n1, n2);
         }
     }
 }
 
 /* Stage 2, every cell: lout = adjoint of f(t) from lin = adjoint of f(t+1).
- * Same grid as the forward kernel: blockIdx.x = row, blockIdx.y = z chunk.
+ * One workgroup per (row, z chunk) block of the store, the chunk index
+ * fastest, as the forward kernel.
  * The adjoint of fb is gb_i = coef lam_i + gr + c_i . gj (lam itself for a
  * cell without a record); the BGK adjoint is
  *   fPbar_i = pb_i + (1 - omega) gb_i + rb + c_i . jb,
@@ -423,28 +420,29 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
                                                  typename Store<PREC>::S* __restrict__ lout, AdjArgs a) {
     typedef typename Store<PREC>::T T;
     __shared__ double sh[4];
-    const int row = blockIdx.x;
-    const int z = blockIdx.y * blockDim.x + threadIdx.x;
-    const bool active = z < a.nz;
+    const int nzc = a.L.nzc;
+    const int row = blockIdx.x / nzc;
+    const int chunk = blockIdx.x - row * nzc;
+    const int z = chunk * a.L.zc + threadIdx.x;
     const int x = __builtin_amdgcn_readfirstlane(row / a.ny);
     const int y = __builtin_amdgcn_readfirstlane(row - x * a.ny);
     double wbar = 0.0;
-    if (active) {
+    if (z < a.nz) {
         const T omega = T(a.omega);
         const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
         typedef typename Store<PREC>::S S;
         T lam[Q], fP[Q];
         pull_adjoint_row<PREC, PZ>(lin, a, x, y, z, lam);
-        const char* frow = (const char*)(fin + pop_at(row, 0, 0, a.nzp));
-        const uint32_t qstride = (uint32_t)a.nzp * (uint32_t)sizeof(S);
-        const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(S);
+        const char* frow = (const char*)(fin + pop_at(row, 0, chunk * a.L.zc, a.L));
+        const uint32_t qstride = (uint32_t)a.L.zc * (uint32_t)sizeof(S);
+        const uint32_t zoff = (uint32_t)threadIdx.x * (uint32_t)sizeof(S);
 #pragma unroll
         for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load((const S*)(frow + i * qstride + zoff));
         const bool rec = a.sflag[(uint32_t)row * (uint32_t)a.nz + (uint32_t)z] != 0;
         int64_t r = 0;
         T coef = T(1), gr = T(0), gj[3] = {T(0), T(0), T(0)};
         if (rec) {
-            r = rec_index(x, y, z, a.nb1, a.nb2);
+            r = rec_index(x, y, z, a.ny, a.nz);
             const T* pc = (const T*)a.pc;
             coef = pc[r];
             gr = pc[a.rcap + r];
@@ -511,13 +509,13 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
 #pragma unroll
             for (int i = 0; i < Q; ++i) lam[i] = fP[i] + lam[i];
         }
-        char* orow = (char*)(lout + pop_at(row, 0, 0, a.nzp));
+        char* orow = (char*)(lout + pop_at(row, 0, chunk * a.L.zc, a.L));
 #pragma unroll
         for (int i = 0; i < Q; ++i) Store<PREC>::store((S*)(orow + i * qstride + zoff), lam[i]);
         wbar = (double)om;
     }
     const double s = block_sum_256(wbar, sh);
-    if (threadIdx.x == 0) a.opart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] += s;
+    if (threadIdx.x == 0) a.opart[blockIdx.x] += s;
 }
 
 /* ---- reductions of the accumulated adjoints ---- */
@@ -533,7 +531,7 @@ __global__ __launch_bounds__(256) void k_sum_vec(const double* __restrict__ in, 
 
 /* Per record slot: dense dJ/dB and dJ/dsf (partial cells; B = sf o / ((1 -
  * sf) + o), o = tau - 1/2) and the tau partial sum_r dJ/dB dB/do. */
-__global__ __launch_bounds__(256) void k_grad_cells(int n0, int n1, int n2, int nb1, int nb2,
+__global__ __launch_bounds__(256) void k_grad_cells(
                                                     int64_t rcap, const uint8_t* __restrict__ sflag,
                                                     const double* __restrict__ rsf,
                                                     const double* __restrict__ gacc, double ooo,
@@ -543,13 +541,7 @@ __global__ __launch_bounds__(256) void k_grad_cells(int n0, int n1, int n2, int 
     double tp = 0.0;
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rcap;
          r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t b = r >> 9;
-        const int l = (int)(r & 511);
-        const int X = (int)(b / ((int64_t)nb1 * nb2)) * BIN + (l >> 6);
-        const int Y = (int)((b / nb2) % nb1) * BIN + ((l >> 3) & 7);
-        const int Z = (int)(b % nb2) * BIN + (l & 7);
-        if (X >= n0 || Y >= n1 || Z >= n2) continue;
-        const int64_t c = ((int64_t)X * n1 + Y) * n2 + Z;
+        const int64_t c = r;   /* record slot = cell index */
         double gB = 0.0, gsf = 0.0;
         if (sflag[c]) {
             const double sf = rsf[r];
@@ -653,8 +645,8 @@ __global__ __launch_bounds__(256) void k_grad_particle(GradArgs a) {
         for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) {
             const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
             if (X >= a.n[0] || Y >= a.n[1] || Z >= a.n[2]) continue;
-            if (!a.sflag[((int64_t)X * a.n[1] + Y) * a.n[2] + Z]) continue;
-            const int64_t r = b * 512 + l;
+            const int64_t r = rec_index(X, Y, Z, a.n[1], a.n[2]);
+            if (!a.sflag[r]) continue;
             if (a.rcand[r] != e) continue;
             record_grad(a, r, X, Y, Z, o, acc);
         }
@@ -679,14 +671,9 @@ __global__ __launch_bounds__(256) void k_grad_orphans(GradArgs a, const int32_t*
         if (id < 1 || id >= id_map_size) continue;
         const int o = id_map[id];
         if (o < 0) continue;
-        const int64_t b = r >> 9;
-        const int l = (int)(r & 511);
-        const int bz = (int)(b % a.nb[2]);
-        const int by = (int)((b / a.nb[2]) % a.nb[1]);
-        const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
-        const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
-        if (X >= a.n[0] || Y >= a.n[1] || Z >= a.n[2]) continue;
-        if (!a.sflag[((int64_t)X * a.n[1] + Y) * a.n[2] + Z]) continue;
+        int X, Y, Z;
+        rec_cell(r, a.n[1], a.n[2], X, Y, Z);
+        if (!a.sflag[r]) continue;
         double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         record_grad(a, r, X, Y, Z, o, acc);
         for (int m = 0; m < 9; ++m) atomicAdd(&a.gpart[9 * (int64_t)o + m], acc[m]);
@@ -695,7 +682,7 @@ __global__ __launch_bounds__(256) void k_grad_orphans(GradArgs a, const int32_t*
 
 /* sf += dsf on live records; flags counters[1] bit 16 when a result leaves
  * (SOLFRAC_MIN, 1] or dsf is non-zero on a cell without a record. */
-__global__ __launch_bounds__(256) void k_perturb_sf(int n0, int n1, int n2, int nb1, int nb2,
+__global__ __launch_bounds__(256) void k_perturb_sf(int n0, int n1, int n2,
                                                     const uint8_t* __restrict__ sflag,
                                                     double* __restrict__ rsf,
                                                     const double* __restrict__ dsf,
@@ -709,13 +696,9 @@ __global__ __launch_bounds__(256) void k_perturb_sf(int n0, int n1, int n2, int 
             atomicOr((unsigned long long*)&counters[1], 16ull);
             continue;
         }
-        const int Z = (int)(c % n2);
-        const int Y = (int)((c / n2) % n1);
-        const int X = (int)(c / ((int64_t)n2 * n1));
-        const int64_t r = rec_index(X, Y, Z, nb1, nb2);
-        const double v = rsf[r] + d;
+        const double v = rsf[c] + d;   /* record slot = cell index */
         if (!(v > SOLFRAC_MIN && v <= 1.0)) atomicOr((unsigned long long*)&counters[1], 16ull);
-        else rsf[r] = v;
+        else rsf[c] = v;
     }
 }
 
@@ -772,11 +755,10 @@ static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lo
     AdjArgs a;
     a.nx = h->g.n[0]; a.ny = h->g.n[1]; a.nz = h->g.n[2];
     a.px = h->g.periodic[0]; a.py = h->g.periodic[1]; a.pz = h->g.periodic[2];
-    a.nzp = h->g.nzp;
+    a.L = h->g.L;
     a.omega = h->omega; a.ooo = h->ooo;
     for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
     a.sflag = h->sflag;
-    a.nb0 = h->g.nb[0]; a.nb1 = h->g.nb[1]; a.nb2 = h->g.nb[2];
     a.rsf = h->rec.sf; a.rus = h->rec.us; a.rid = h->rec.id; a.rcap = h->rec_cap;
     a.gft = gft;
     a.id_map = h->id_map; a.id_map_size = h->id_map_size;
@@ -789,7 +771,7 @@ static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lo
     const int64_t nrec = h->diff.nrec;
     dim3 gridr((unsigned)std::max<int64_t>(1, (nrec + 255) / 256)), blockr(256);
     const int32_t* list = h->diff.rec_list;
-    dim3 grid((unsigned)(a.nx * a.ny), (unsigned)((a.nz + 255) / 256)), block(256);
+    dim3 grid((unsigned)((int64_t)a.nx * a.ny * a.L.nzc)), block((unsigned)a.L.zc);
     if (h->precision == LBDEM_PSM_FP64) {
         typedef double S;
         if (h->has_force) {
@@ -857,7 +839,7 @@ int lbdem_psm_perturb_solid_fraction(lbdem_psm* h, const double* dsf) {
     if (e == hipSuccess) e = hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_perturb_sf, dim3((unsigned)std::min<int64_t>(8192, (nc + 255) / 256)), dim3(256), 0,
-                           h->stream, h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2], h->sflag, h->rec.sf,
+                           h->stream, h->g.n[0], h->g.n[1], h->g.n[2], h->sflag, h->rec.sf,
                            d, h->counters);
         e = hipGetLastError();
     }
@@ -956,7 +938,7 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
     const size_t tsz = h->precision == LBDEM_PSM_FP64 ? sizeof(double) : sizeof(float);
     if (!D.pb) PSM_HIP(h, dalloc_d(&D.pb, (size_t)Q * rcap * tsz));
     if (!D.pc) PSM_HIP(h, dalloc_d(&D.pc, (size_t)5 * rcap * tsz));
-    const int64_t nblk = (int64_t)h->g.n[0] * h->g.n[1] * ((h->g.n[2] + 255) / 256);
+    const int64_t nblk = (int64_t)h->g.n[0] * h->g.n[1] * h->g.L.nzc;
     if (D.opart_n < nblk + 8) {
         dfree_d(D.opart);
         PSM_HIP(h, dalloc_d(&D.opart, (nblk + 8) * sizeof(double)));
@@ -1013,9 +995,8 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
     /* reductions */
     double* red = h->red_tmp;   /* [0] omega adjoint, [1] tau adjoint through B */
     hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, h->stream, D.opart, nblk, red);
-    hipLaunchKernelGGL(k_grad_cells, dim3((unsigned)gblocks), dim3(256), 0, h->stream, h->g.n[0], h->g.n[1],
-                       h->g.n[2], h->g.nb[1], h->g.nb[2], rcap, h->sflag, h->rec.sf, D.gacc, h->ooo, D.cell_out, nc,
-                       D.tau_part);
+    hipLaunchKernelGGL(k_grad_cells, dim3((unsigned)gblocks), dim3(256), 0, h->stream, rcap, h->sflag, h->rec.sf,
+                       D.gacc, h->ooo, D.cell_out, nc, D.tau_part);
     hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, h->stream, D.tau_part, (int64_t)gblocks, red + 1);
     if (np > 0) {
         GradArgs ga;

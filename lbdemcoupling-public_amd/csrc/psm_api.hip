@@ -278,7 +278,7 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
         h->g.nb[d] = (nn[d] + BIN - 1) / BIN;
     }
     h->g.nc = (int64_t)nn[0] * nn[1] * nn[2];
-    h->g.nzp = ((int64_t)nn[2] + 63) / 64 * 64;
+    h->g.L = make_layout(nn[2]);
     h->nbins = (int64_t)h->g.nb[0] * h->g.nb[1] * h->g.nb[2];
     h->omega = 1.0 / desc->tau;
     h->ooo = 1.0 / h->omega - 0.5;          /* src/ibCompositeDynamics3D.hh:178 */
@@ -291,8 +291,8 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
     h->has_force = desc->body_accel[0] != 0.0 || desc->body_accel[1] != 0.0 || desc->body_accel[2] != 0.0;
     h->precision = desc->precision;
     const size_t S = h->precision == LBDEM_PSM_FP64 ? 8 : (h->precision == LBDEM_PSM_FP32_DEV ? 4 : 2);
-    h->pop_bytes = (size_t)Q * (size_t)nn[0] * (size_t)nn[1] * (size_t)h->g.nzp * S;
-    h->rec_cap = h->nbins * 512;                 /* one record slot per cell of the bin grid */
+    h->pop_bytes = (size_t)nn[0] * (size_t)nn[1] * (size_t)h->g.L.rowlen() * S;
+    h->rec_cap = h->g.nc;                        /* one record slot per cell */
     h->max_records = desc->max_solid_cells;
     h->orphan_cap = std::min<int64_t>(h->rec_cap, (int64_t)1 << 24);
     hipError_t e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking);
@@ -559,7 +559,7 @@ int lbdem_psm_get_cell_records(lbdem_psm* h, double* sf, int64_t* id, double* us
             for (int Z = 0; Z < n2; ++Z) {
                 const int64_t c = ((int64_t)X * n1 + Y) * n2 + Z;
                 const bool live = flag[c] != 0;
-                const int64_t r = rec_index(X, Y, Z, h->g.nb[1], h->g.nb[2]);
+                const int64_t r = rec_index(X, Y, Z, n1, n2);
                 if (sf) sf[c] = live ? rsf[r] : 0.0;
                 if (id) id[c] = live ? rid[r] : 0;
                 for (int d = 0; d < 3; ++d) {
@@ -586,12 +586,12 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
                 if (!(sf[c] > SOLFRAC_MIN)) continue;
                 const int64_t pid = id ? id[c] : 0;
                 if (pid < 0 || pid >= (int64_t)INT32_MAX) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "bad record id");
-                const int64_t r = rec_index(X, Y, Z, h->g.nb[1], h->g.nb[2]);
+                const int64_t r = rec_index(X, Y, Z, n1, n2);
                 flag[c] = 1;
                 rsf[r] = sf[c];
                 rid[r] = (int32_t)pid;
                 for (int d = 0; d < 3; ++d) rus[d * cap + r] = us ? us[d * nc + c] : 0.0;
-                any[r >> 9] += 1u;
+                any[((int64_t)(X >> 3) * h->g.nb[1] + (Y >> 3)) * h->g.nb[2] + (Z >> 3)] += 1u;
                 orph.push_back((int32_t)r);
             }
     const int64_t m = (int64_t)orph.size();

@@ -5,13 +5,12 @@
  * IBcompositeDynamics (src/ibCompositeDynamics3D.hh:130-203;
  * examples/sedimentingSphere/sedimentingSphere.cpp:221).
  *
- * Layout: populations are stored as f - t in FP64, FP32 or FP16, row-blocked
- * structure of arrays: row r = x*ny + y holds the 19 z-lines of its
- * populations back to back, f[(r*19 + q)*nzp + z].  One workgroup owns one
- * z-row chunk of 256 cells: every lane reads its cell's 19 populations with
- * unit stride (fully coalesced, all inside one row block), collides, and
- * pushes each population to its neighbour x + c_q, i.e. into one of 9 row
- * blocks.
+ * Layout: populations are stored as f - t in FP64, FP32 or FP16, in
+ * z-chunked row blocks (PopLayout): the 19 populations of a (row, z chunk)
+ * are consecutive stretches.  One workgroup owns one block: every lane reads
+ * its cell's 19 populations with unit stride (fully coalesced, one
+ * contiguous stretch per workgroup), collides, and pushes each population to
+ * its neighbour x + c_q, i.e. into one of 9 row blocks.
  * Cells with a coupling record are marked by a 1-byte flag; only for those
  * is the record (solid fraction, particle velocity) read and the per-cell
  * hydrodynamic force written.
@@ -25,7 +24,7 @@ template <int PREC>
 __global__ __launch_bounds__(256) void k_init_eq(typename Store<PREC>::S* __restrict__ f,
                                                  const double* __restrict__ rho,
                                                  const double* __restrict__ u, int64_t nc, int nz,
-                                                 int64_t nzp) {
+                                                 PopLayout L) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
          c += (int64_t)gridDim.x * blockDim.x) {
         const int64_t row = c / nz;
@@ -39,7 +38,7 @@ __global__ __launch_bounds__(256) void k_init_eq(typename Store<PREC>::S* __rest
         const double p = 1.5 * invRho;
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            Store<PREC>::store(f + pop_at(row, i, z, nzp), (typename Store<PREC>::T)equilibrium<double>(i, rhoBar, p, j, jSqr));
+            Store<PREC>::store(f + pop_at(row, i, z, L), (typename Store<PREC>::T)equilibrium<double>(i, rhoBar, p, j, jSqr));
     }
 }
 
@@ -47,7 +46,7 @@ __global__ __launch_bounds__(256) void k_init_eq(typename Store<PREC>::S* __rest
 template <int PREC>
 __global__ __launch_bounds__(256) void k_pop_convert(typename Store<PREC>::S* __restrict__ f,
                                                      double* __restrict__ f64, int64_t nc, int nz,
-                                                     int64_t nzp, int to_store) {
+                                                     PopLayout L, int to_store) {
     const int64_t total = (int64_t)Q * nc;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total;
          k += (int64_t)gridDim.x * blockDim.x) {
@@ -55,7 +54,7 @@ __global__ __launch_bounds__(256) void k_pop_convert(typename Store<PREC>::S* __
         const int64_t c = k - (int64_t)q * nc;
         const int64_t row = c / nz;
         const int z = (int)(c - row * nz);
-        const size_t at = pop_at(row, q, z, nzp);
+        const size_t at = pop_at(row, q, z, L);
         if (to_store) Store<PREC>::store(f + at, (typename Store<PREC>::T)f64[k]);
         else f64[k] = (double)Store<PREC>::load(f + at);
     }
@@ -65,7 +64,7 @@ __global__ __launch_bounds__(256) void k_pop_convert(typename Store<PREC>::S* __
  * block (fixed tree); stage 2: one block sums the partials in index order. */
 template <int PREC>
 __global__ __launch_bounds__(256) void k_cons_partial(const typename Store<PREC>::S* __restrict__ f,
-                                                      int64_t nc, int nz, int64_t nzp,
+                                                      int64_t nc, int nz, PopLayout L,
                                                       double* __restrict__ part) {
     __shared__ double sh[5][256];
     double m = 0.0, jx = 0.0, jy = 0.0, jz = 0.0, umax = 0.0, nonfin = 0.0;
@@ -77,7 +76,7 @@ __global__ __launch_bounds__(256) void k_cons_partial(const typename Store<PREC>
         const int z = (int)(c - row * nz);
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            fc[i] = (double)Store<PREC>::load(f + pop_at(row, i, z, nzp));
+            fc[i] = (double)Store<PREC>::load(f + pop_at(row, i, z, L));
             bad |= !isfinite(fc[i]);
         }
         double rb, j[3];
@@ -137,7 +136,7 @@ __global__ __launch_bounds__(256) void k_define_velocity(typename Store<PREC>::S
                                                          const int32_t* __restrict__ dv_cell,
                                                          const double* __restrict__ dv_us,
                                                          const int64_t* __restrict__ count,
-                                                         int64_t cap, int nz, int64_t nzp) {
+                                                         int64_t cap, int nz, PopLayout L) {
     const int64_t n = count[0] < cap ? count[0] : cap;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * blockDim.x) {
@@ -151,7 +150,7 @@ __global__ __launch_bounds__(256) void k_define_velocity(typename Store<PREC>::S
         const double p = 1.5 * (1.0 / (0.0 + 1.0));
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            Store<PREC>::store(f + pop_at(row, i, z, nzp), (typename Store<PREC>::T)equilibrium<double>(i, 0.0, p, u, uSqr));
+            Store<PREC>::store(f + pop_at(row, i, z, L), (typename Store<PREC>::T)equilibrium<double>(i, 0.0, p, u, uSqr));
     }
 }
 
@@ -160,12 +159,10 @@ static CollideArgs make_args(lbdem_psm* h) {
     a.nx = h->g.n[0]; a.ny = h->g.n[1]; a.nz = h->g.n[2];
     a.px = h->g.periodic[0]; a.py = h->g.periodic[1]; a.pz = h->g.periodic[2];
     a.nc = h->g.nc;
-    a.nzp = h->g.nzp;
+    a.L = h->g.L;
     a.omega = h->omega; a.ooo = h->ooo;
     for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
     a.sflag = h->sflag;
-    a.nb1 = h->g.nb[1];
-    a.nb2 = h->g.nb[2];
     a.rsf = h->rec.sf; a.rus = h->rec.us; a.rF = h->rec.F; a.rcap = h->rec_cap;
     return a;
 }
@@ -175,7 +172,7 @@ void launch_cs_prec<LBDEM_PSM_FP64>(lbdem_psm* h, const CollideArgs& a, const vo
     typedef typename Store<LBDEM_PSM_FP64>::S S;
     const S* fin = (const S*)vin;
     S* fout = (S*)vout;
-    dim3 grid((unsigned)(a.nx * a.ny), (unsigned)((a.nz + 255) / 256)), block(256);
+    dim3 grid((unsigned)((int64_t)a.nx * a.ny * a.L.nzc)), block((unsigned)a.L.zc);
     if (h->has_force) {
         if (a.pz) hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, true>), grid, block, 0, h->stream, fin, fout, a);
         else hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, false>), grid, block, 0, h->stream, fin, fout, a);
@@ -226,13 +223,13 @@ int launch_init_equilibrium(lbdem_psm* h, const double* d_rho, const double* d_u
     dim3 grid(grid_for(nc)), block(256);
     switch (h->precision) {
         case LBDEM_PSM_FP64:
-            hipLaunchKernelGGL(k_init_eq<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)h->f[h->cur_f], d_rho, d_u, nc, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_init_eq<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)h->f[h->cur_f], d_rho, d_u, nc, h->g.n[2], h->g.L);
             break;
         case LBDEM_PSM_FP32_DEV:
-            hipLaunchKernelGGL(k_init_eq<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)h->f[h->cur_f], d_rho, d_u, nc, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_init_eq<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)h->f[h->cur_f], d_rho, d_u, nc, h->g.n[2], h->g.L);
             break;
         default:
-            hipLaunchKernelGGL(k_init_eq<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)h->f[h->cur_f], d_rho, d_u, nc, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_init_eq<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)h->f[h->cur_f], d_rho, d_u, nc, h->g.n[2], h->g.L);
             break;
     }
     PSM_HIP(h, hipGetLastError());
@@ -247,17 +244,17 @@ int launch_pop_convert_buf(lbdem_psm* h, void* buf, double* d_f64, bool to_store
     const int64_t nc = h->g.nc;
     const int64_t total = (int64_t)Q * nc;
     const int nz = h->g.n[2];
-    const int64_t nzp = h->g.nzp;
+    const PopLayout L = h->g.L;
     dim3 grid(grid_for(total)), block(256);
     switch (h->precision) {
         case LBDEM_PSM_FP64:
-            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)buf, d_f64, nc, nz, nzp, (int)to_store);
+            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)buf, d_f64, nc, nz, L, (int)to_store);
             break;
         case LBDEM_PSM_FP32_DEV:
-            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)buf, d_f64, nc, nz, nzp, (int)to_store);
+            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)buf, d_f64, nc, nz, L, (int)to_store);
             break;
         default:
-            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)buf, d_f64, nc, nz, nzp, (int)to_store);
+            hipLaunchKernelGGL(k_pop_convert<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)buf, d_f64, nc, nz, L, (int)to_store);
             break;
     }
     PSM_HIP(h, hipGetLastError());
@@ -272,13 +269,13 @@ int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
     double* out = h->red_tmp + 6 * 4096;
     switch (h->precision) {
         case LBDEM_PSM_FP64:
-            hipLaunchKernelGGL(k_cons_partial<LBDEM_PSM_FP64>, dim3(nblocks), dim3(256), 0, h->stream, (const double*)h->f[h->cur_f], nc, h->g.n[2], h->g.nzp, part);
+            hipLaunchKernelGGL(k_cons_partial<LBDEM_PSM_FP64>, dim3(nblocks), dim3(256), 0, h->stream, (const double*)h->f[h->cur_f], nc, h->g.n[2], h->g.L, part);
             break;
         case LBDEM_PSM_FP32_DEV:
-            hipLaunchKernelGGL(k_cons_partial<LBDEM_PSM_FP32_DEV>, dim3(nblocks), dim3(256), 0, h->stream, (const float*)h->f[h->cur_f], nc, h->g.n[2], h->g.nzp, part);
+            hipLaunchKernelGGL(k_cons_partial<LBDEM_PSM_FP32_DEV>, dim3(nblocks), dim3(256), 0, h->stream, (const float*)h->f[h->cur_f], nc, h->g.n[2], h->g.L, part);
             break;
         default:
-            hipLaunchKernelGGL(k_cons_partial<LBDEM_PSM_FP16_STORE>, dim3(nblocks), dim3(256), 0, h->stream, (const _Float16*)h->f[h->cur_f], nc, h->g.n[2], h->g.nzp, part);
+            hipLaunchKernelGGL(k_cons_partial<LBDEM_PSM_FP16_STORE>, dim3(nblocks), dim3(256), 0, h->stream, (const _Float16*)h->f[h->cur_f], nc, h->g.n[2], h->g.L, part);
             break;
     }
     hipLaunchKernelGGL(k_cons_final, dim3(1), dim3(256), 0, h->stream, part, nblocks, out);
@@ -298,13 +295,13 @@ int launch_define_velocity(lbdem_psm* h) {
     dim3 grid(1024), block(256);
     switch (h->precision) {
         case LBDEM_PSM_FP64:
-            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (double*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.L);
             break;
         case LBDEM_PSM_FP32_DEV:
-            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (float*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.L);
             break;
         default:
-            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.nzp);
+            hipLaunchKernelGGL(k_define_velocity<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (_Float16*)h->f[h->cur_f], h->dv_cell, h->dv_us, h->counters + 5, h->orphan_cap, h->g.n[2], h->g.L);
             break;
     }
     PSM_HIP(h, hipGetLastError());

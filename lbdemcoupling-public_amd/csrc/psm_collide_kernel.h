@@ -20,11 +20,10 @@ struct CollideArgs {
     int nx, ny, nz;
     int px, py, pz;
     int64_t nc;
-    int64_t nzp;
+    PopLayout L;
     double omega, ooo;
     double acc[3];
     const uint8_t* sflag;
-    int nb1, nb2;            /* bins along y and z (record index) */
     const double* rsf;
     const double* rus;
     double* rF;
@@ -91,12 +90,13 @@ __device__ __forceinline__ void bgk_all(const T* f, T* fb, T omega, const T* acc
     }
 }
 
-/* One workgroup per z-row (x, y uniform): per direction the destination
- * row block and the x/y wall test are uniform, only z varies per lane. */
+/* Push of one cell's post-collision populations.  x, y are wave-uniform:
+ * per direction the destination row and the x/y wall test are uniform
+ * (scalar), only the z offset is per lane. */
 template <typename S, bool PZ>
 struct Pusher {
     S* fout;
-    int64_t nzp;
+    PopLayout L;
     int x, y, nx, ny, nz, px, py;
     __device__ __forceinline__ void put(int i, int z, S v) const {
         int xd = x + cx(i), yd = y + cy(i);
@@ -110,11 +110,11 @@ struct Pusher {
             else zwall = zd < 0 || zd >= nz;
         }
         if (wall || zwall) {
-            char* own = (char*)(fout + pop_at((int64_t)x * ny + y, opp(i), 0, nzp));
-            *(S*)(own + (uint32_t)z * (uint32_t)sizeof(S)) = v;
+            char* own = (char*)(fout + pop_at((int64_t)x * ny + y, opp(i), 0, L));
+            *(S*)(own + L.zoff(z) * (uint32_t)sizeof(S)) = v;
         } else {
-            char* dst = (char*)(fout + pop_at((int64_t)xd * ny + yd, i, 0, nzp));
-            *(S*)(dst + (uint32_t)zd * (uint32_t)sizeof(S)) = v;
+            char* dst = (char*)(fout + pop_at((int64_t)xd * ny + yd, i, 0, L));
+            *(S*)(dst + L.zoff(zd) * (uint32_t)sizeof(S)) = v;
         }
     }
 };
@@ -222,15 +222,20 @@ __device__ __forceinline__ void psm_blend_fast(const float* fp, float* fo, bool 
     }
 }
 
-/* One lane per cell: blockIdx.x = row (x*ny + y), blockIdx.y = z chunk. */
+/* One lane per cell, one workgroup per (row, z chunk) block of the store
+ * (blockDim.x = chunk length), the chunk index fastest: the grid sweeps the
+ * store once, in address order. */
 template <int PREC, bool FORCE, bool PZ>
 __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PREC>::S* __restrict__ fin,
                                                         typename Store<PREC>::S* __restrict__ fout,
                                                         CollideArgs a) {
     typedef typename Store<PREC>::T T;
     typedef typename Store<PREC>::S S;
-    const int row = blockIdx.x;
-    const int z = blockIdx.y * blockDim.x + threadIdx.x;
+    const int nzc = a.L.nzc;
+    const int row = blockIdx.x / nzc;
+    const int chunk = blockIdx.x - row * nzc;
+    const int zl = threadIdx.x;
+    const int z = chunk * a.L.zc + zl;
     if (z >= a.nz) return;
     /* x, y are wave-uniform: keep them (and the row-block addresses derived
      * from them) in scalar registers */
@@ -238,14 +243,15 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     const int y = __builtin_amdgcn_readfirstlane(row - x * a.ny);
     const T omega = T(a.omega);
     const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
-    Pusher<S, PZ> push{fout, a.nzp, x, y, a.nx, a.ny, a.nz, a.px, a.py};
-    const char* frow = (const char*)(fin + pop_at(row, 0, 0, a.nzp));
-    const size_t qstride = (size_t)a.nzp * sizeof(S);
+    Pusher<S, PZ> push{fout, a.L, x, y, a.nx, a.ny, a.nz, a.px, a.py};
+    /* this workgroup's (row, chunk) block: 19 contiguous stretches of zc */
+    const char* frow = (const char*)(fin + pop_at(row, 0, chunk * a.L.zc, a.L));
+    const size_t qstride = (size_t)a.L.zc * sizeof(S);
 
-    const uint32_t c = (uint32_t)row * (uint32_t)a.nz + (uint32_t)z;
+    const int64_t c = rec_index(x, y, z, a.ny, a.nz);
     const bool solid = a.sflag[c] != 0;
-    const int64_t s = solid ? rec_index(x, y, z, a.nb1, a.nb2) : 0;
-    const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(S);
+    const int64_t s = solid ? c : 0;
+    const uint32_t zoff = (uint32_t)zl * (uint32_t)sizeof(S);
     T fp[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) fp[i] = (T)(*(const S*)(frow + i * qstride + zoff));

@@ -254,10 +254,9 @@ template <> struct Store<LBDEM_PSM_FP16_STORE> {
     static __device__ __forceinline__ void store(S* p, T v) { *p = (_Float16)v; }
 };
 
-/* Coupling records (IBdynamicsParticleData), structure of arrays in the
- * bin-permuted dense layout: the cell (x, y, z) of bin b at local index
- * l = ((x&7)*8 + (y&7))*8 + (z&7) owns entry b*512 + l.  A record is
- * live when the cell's solid flag is set. */
+/* Coupling records (IBdynamicsParticleData), structure of arrays with one
+ * slot per cell (slot = cell index, rec_index).  A record is live when the
+ * cell's solid flag is set. */
 struct Records {
     double* sf;      /* solid fraction */
     int32_t* id;     /* particle tag */
@@ -278,9 +277,67 @@ struct Cand {
     int e;              /* entry index */
 };
 
-__host__ __device__ inline int64_t rec_index(int x, int y, int z, int nb1, int nb2) {
-    const int64_t bin = ((int64_t)(x >> 3) * nb1 + (y >> 3)) * nb2 + (z >> 3);
-    return bin * 512 + ((x & 7) * 8 + (y & 7)) * 8 + (z & 7);
+/* Record slot of the cell (x, y, z): the cell index itself, so that a
+ * z-run of cells reads its records as contiguous stretches in the order the
+ * collide sweep visits them. */
+__host__ __device__ inline int64_t rec_index(int x, int y, int z, int ny, int nz) {
+    return ((int64_t)x * ny + y) * nz + z;
+}
+
+__host__ __device__ inline void rec_cell(int64_t r, int ny, int nz, int& x, int& y, int& z) {
+    z = (int)(r % nz);
+    const int64_t xy = r / nz;
+    y = (int)(xy % ny);
+    x = (int)(xy / ny);
+}
+
+/* Population store layout, z-chunked row blocks: the z-line of row
+ * r = x*ny + y is cut into nzc chunks of zc cells, and each (row, chunk)
+ * block holds its 19 populations back to back,
+ *   f[((r*nzc + z/zc)*19 + q)*zc + z%zc].
+ * A workgroup owns one (row, chunk) block: its 19 loads read one contiguous
+ * 19*zc-element stretch, and its pushes land in the blocks of the 8
+ * neighbouring rows (and, across a chunk boundary, of the next chunk).
+ * zc is a multiple of 64; with more than one chunk it is a power of two
+ * (zsh = log2 zc). */
+struct PopLayout {
+    int zc;
+    int zsh;
+    int nzc;
+    __host__ __device__ int64_t rowlen() const { return (int64_t)nzc * Q * zc; }   /* elements per row */
+    /* offset of z inside a row, population 0 */
+    __host__ __device__ uint32_t zoff(int z) const {
+        return nzc > 1 ? (uint32_t)(z >> zsh) * (uint32_t)(Q * zc) + (uint32_t)(z & (zc - 1)) : (uint32_t)z;
+    }
+};
+
+__host__ __device__ inline size_t pop_at(int64_t row, int q, int z, const PopLayout& L) {
+    return (size_t)row * (size_t)L.rowlen() + (size_t)q * (size_t)L.zc + (size_t)L.zoff(z);
+}
+
+/* Chunk length for a z extent: one chunk (nz rounded up to 64) up to 256
+ * cells, else the largest power of two in {256, 128, 64} that pads by at
+ * most nz/8. */
+inline PopLayout make_layout(int nz) {
+    PopLayout L;
+    if (nz <= 256) {
+        L.zc = (nz + 63) / 64 * 64;
+        L.zsh = 0;
+        L.nzc = 1;
+        return L;
+    }
+    const int cand[3] = {256, 128, 64};
+    const int sh[3] = {8, 7, 6};
+    for (int k = 0; k < 3; ++k) {
+        const int nzc = (nz + cand[k] - 1) / cand[k];
+        if ((int64_t)(nzc * cand[k] - nz) * 8 <= nz || k == 2) {
+            L.zc = cand[k];
+            L.zsh = sh[k];
+            L.nzc = nzc;
+            return L;
+        }
+    }
+    return L;
 }
 
 struct Geom {
@@ -288,16 +345,8 @@ struct Geom {
     int periodic[3];
     int nb[3];       /* bins per axis */
     int64_t nc;
-    int64_t nzp;     /* z stride of the population store (nz padded to 64) */
+    PopLayout L;     /* population store layout */
 };
-
-/* Population store layout, row-blocked structure of arrays: for row
- * r = x*ny + y the 19 populations are consecutive z-lines,
- * f[(r*19 + q)*nzp + z].  A workgroup working on one row touches only its
- * own row block and the 8 neighbouring ones. */
-__host__ __device__ inline size_t pop_at(int64_t row, int q, int z, int64_t nzp) {
-    return ((size_t)row * Q + (size_t)q) * (size_t)nzp + (size_t)z;
-}
 
 }  // namespace psm
 
@@ -315,7 +364,7 @@ struct lbdem_psm {
     int cur_f = 0;
     uint8_t* sflag = nullptr;        /* per cell: 1 when the cell holds a coupling record */
     psm::Records rec;                /* dense, bin-permuted, updated in place */
-    int64_t rec_cap = 0;             /* nbins * 512 */
+    int64_t rec_cap = 0;             /* one slot per cell */
     int64_t nbins = 0;
     uint32_t* bin_any[2] = {nullptr, nullptr};   /* per bin: live records (this / next step) */
     int cur_any = 0;

@@ -36,17 +36,6 @@ struct ForceArgs {
     int64_t ent_cap;
 };
 
-__device__ __forceinline__ void rec_cell(int64_t ridx, const int* nb, int& X, int& Y, int& Z) {
-    const int64_t b = ridx >> 9;
-    const int l = (int)(ridx & 511);
-    const int bz = (int)(b % nb[2]);
-    const int by = (int)((b / nb[2]) % nb[1]);
-    const int bx = (int)(b / ((int64_t)nb[2] * nb[1]));
-    X = bx * BIN + (l >> 6);
-    Y = by * BIN + ((l >> 3) & 7);
-    Z = bz * BIN + (l & 7);
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
@@ -68,8 +57,8 @@ __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
         const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
         const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
         int e = -1;
-        if (valid && a.sflag[((int64_t)X * a.n[1] + Y) * a.n[2] + Z]) {
-            const int64_t r = b * 512 + l;
+        const int64_t r = valid ? rec_index(X, Y, Z, a.n[1], a.n[2]) : 0;
+        if (valid && a.sflag[r]) {
             e = a.rec.cand[r];
             sF[l][0] = a.rec.F[r];
             sF[l][1] = a.rec.F[a.rcap + r];
@@ -149,7 +138,7 @@ __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_
         const int o = id_map[id];
         if (o < 0) continue;
         int X, Y, Z;
-        rec_cell(r, a.nb, X, Y, Z);
+        rec_cell(r, a.n[1], a.n[2], X, Y, Z);
         const double dx = min_image((double)X - a.xs[3 * o], a.n[0]);
         const double dy = min_image((double)Y - a.xs[3 * o + 1], a.n[1]);
         const double dz = min_image((double)Z - a.xs[3 * o + 2], a.n[2]);

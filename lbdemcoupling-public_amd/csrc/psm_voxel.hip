@@ -380,7 +380,7 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
     const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
     const int64_t cidx = valid ? ((int64_t)X * a.n[1] + Y) * a.n[2] + Z : 0;
-    const int64_t ridx = b * 512 + l;
+    const int64_t ridx = cidx;   /* record slot = cell index */
 
     const bool had = valid && a.sflag[cidx] != 0;
     int kind = had ? ST_OLD : ST_ZERO, sid = 0, se = -1;
