@@ -251,6 +251,7 @@ struct FoldArgs {
     int N;
     double sw, frac;
     double rc, vc;             /* cell sphere of the analytic overlap */
+    double esamp;              /* sqrt(3) (1/2 - 1/(2N)): farthest sub-cell sample from the centre */
     const int32_t* bin_cand_start;
     const int32_t* bin_cand_count;
     const Cand* cand;
@@ -304,13 +305,31 @@ __device__ __forceinline__ unsigned wave_count(double dx, double dy, double dz, 
     return total;
 }
 
+/* Broadcast of lane L's double (L wave-uniform) into scalar registers. */
+__device__ __forceinline__ double readlane_d(double v, int L) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, L);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), L);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 /* calcSolidFraction (src/ibProcessors3D.hh:116-158) for one query per lane,
  * called by all 64 lanes together: the two distance shortcuts per lane, then
- * every lane that needs a count gets it from wave_count in turn. */
+ * every lane that needs a count gets it from wave_count in turn.
+ *
+ * Between the reference's shortcut radii r -+ sqrt(3.1)/2, a cell whose
+ * centre lies within ri (resp. beyond ro) of the sphere centre has every
+ * sub-cell sample strictly inside (resp. outside) the sphere: the samples
+ * lie within e = sqrt(3) (1/2 - 1/(2N)) of the cell centre, and
+ * ri = r - e - tol, ro = r + e + tol with a margin tol far above the
+ * rounding error of the sample test.  Their count (N^3, resp. 0) and
+ * solid fraction (fraction * count) are then known without counting; the
+ * result is the same as the full count's, bit for bit. */
 template <int NT>
 __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, double dy, double dz,
-                                                      double rp2, double rm2, double rsq, int Nrt,
-                                                      double sw, double frac, int& n, int& br) {
+                                                      double rp2, double rm2, double rsq, double ri2,
+                                                      double ro2, int Nrt, double sw, double frac, int& n,
+                                                      int& br) {
     const int N = NT > 0 ? NT : Nrt;
     double sf = 0.0;
     n = 0;
@@ -323,7 +342,10 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
         } else if (dist < rm2) {
             br = 1; n = N * N * N; sf = 1.0;
         } else {
-            br = 2; need = true;
+            br = 2;
+            if (dist < ri2) { n = N * N * N; sf = frac * (double)(N * N * N); }
+            else if (dist > ro2) { n = 0; sf = frac * 0.0; }
+            else need = true;
         }
     }
     unsigned long long pend = __ballot(need);
@@ -331,11 +353,20 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
     while (pend) {
         const int L = __ffsll(pend) - 1;
         pend &= pend - 1;
-        const double bx = __shfl(dx, L), by = __shfl(dy, L), bz = __shfl(dz, L), b2 = __shfl(rsq, L);
+        const double bx = readlane_d(dx, L), by = readlane_d(dy, L), bz = readlane_d(dz, L);
+        const double b2 = readlane_d(rsq, L);
         const unsigned cnt = wave_count<NT>(bx, by, bz, b2, N, sw);
         if (lane == L) { n = (int)cnt; sf = frac * (double)cnt; }
     }
     return sf;
+}
+
+/* Squared radii of the exact no-count band of coop_solid_fraction. */
+__device__ __forceinline__ void exact_band(double r, double e, double& ri2, double& ro2) {
+    const double tol = 1e-9 * (1.0 + r);
+    const double ri = r - e - tol, ro = r + e + tol;
+    ri2 = ri > 0.0 ? ri * ri : -1.0;
+    ro2 = ro * ro;
 }
 
 /* The voxelization rule: NT >= 0 the reference's sub-cell count (NT = N
@@ -345,15 +376,17 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
 template <int NT>
 __device__ __forceinline__ double solid_fraction(bool active, double dx, double dy, double dz,
                                                  double rp2, double rm2, double rsq, double r, int Nrt,
-                                                 double sw, double frac, double rc, double vc, int& n,
-                                                 int& br) {
+                                                 double sw, double frac, double rc, double vc, double esamp,
+                                                 int& n, int& br) {
     if constexpr (NT < 0) {
         n = 0;
         br = 0;
         if (!active) return 0.0;
         return overlap_fraction(sqrt((dx * dx + dy * dy) + dz * dz), r, rc, vc, nullptr, &br);
     } else {
-        return coop_solid_fraction<NT>(active, dx, dy, dz, rp2, rm2, rsq, Nrt, sw, frac, n, br);
+        double ri2, ro2;
+        exact_band(r, esamp, ri2, ro2);
+        return coop_solid_fraction<NT>(active, dx, dy, dz, rp2, rm2, rsq, ri2, ro2, Nrt, sw, frac, n, br);
     }
 }
 
@@ -366,8 +399,6 @@ enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
  * back in place. */
 template <int NT, bool INITVEL>
 __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
-    __shared__ Cand cs[64];
-
     const int64_t b = blockIdx.x;
     const int ncand = a.bin_cand_count[b];
     if (ncand == 0 && a.cnt_old[b] == 0u) return;   /* nothing here, nothing was here */
@@ -381,6 +412,9 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
     const int64_t cidx = valid ? ((int64_t)X * a.n[1] + Y) * a.n[2] + Z : 0;
     const int64_t ridx = cidx;   /* record slot = cell index */
+    /* a wavefront is the x-slice X of the bin: 8 y by 8 z cells */
+    const int wX = __builtin_amdgcn_readfirstlane(X);
+    const int y0 = by * BIN, z0 = bz * BIN;
 
     const bool had = valid && a.sflag[cidx] != 0;
     int kind = had ? ST_OLD : ST_ZERO, sid = 0, se = -1;
@@ -392,13 +426,15 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
         sid = a.rec.id[ridx];
     }
 
-    for (int c0 = 0; c0 < ncand; c0 += 64) {
-        const int nchunk = min(64, ncand - c0);
-        __syncthreads();
-        if ((int)threadIdx.x < nchunk) cs[threadIdx.x] = a.cand[cstart + c0 + threadIdx.x];
-        __syncthreads();
-        for (int j = 0; j < nchunk; ++j) {
-            const Cand& cd = cs[j];
+    /* candidates in key order; their data are wave-uniform */
+    const Cand* __restrict__ cands = a.cand + cstart;
+    {
+        for (int j = 0; j < ncand; ++j) {
+            const Cand& cd = cands[j];
+            /* whole-wavefront rejection against the candidate's box */
+            if (wX < cd.lo[0] || wX > cd.hi[0] || y0 + BIN - 1 < cd.lo[1] || y0 > cd.hi[1] ||
+                z0 + BIN - 1 < cd.lo[2] || z0 > cd.hi[2])
+                continue;
             const bool inb = valid && X >= cd.lo[0] && X <= cd.hi[0] && Y >= cd.lo[1] && Y <= cd.hi[1] &&
                              Z >= cd.lo[2] && Z <= cd.hi[2];
             const double dx = (double)X - cd.x[0];
@@ -406,7 +442,7 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
             const double dz = (double)Z - cd.x[2];
             int nn, br;
             const double sf = solid_fraction<NT>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, a.N, a.sw,
-                                                 a.frac, a.rc, a.vc, nn, br);
+                                                 a.frac, a.rc, a.vc, a.esamp, nn, br);
             if (!inb) continue;
             const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (ssf > SOLFRAC_MIN ? 1 : 0);
             bool set = false, zero = false;
@@ -478,15 +514,16 @@ template <int NT>
 __global__ __launch_bounds__(256) void k_subcell(int64_t n, const double* __restrict__ d3,
                                                  const double* __restrict__ r, int N, double sw,
                                                  double frac, double h, double rc, double vc,
-                                                 double* __restrict__ sf, int32_t* __restrict__ cnt,
-                                                 int32_t* __restrict__ br) {
+                                                 double esamp, double* __restrict__ sf,
+                                                 int32_t* __restrict__ cnt, int32_t* __restrict__ br) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = k < n;
     const double rr = act ? r[k] : 1.0;
     const double r_p = rr + h, r_m = rr - h;
     const double dx = act ? d3[3 * k] : 0.0, dy = act ? d3[3 * k + 1] : 0.0, dz = act ? d3[3 * k + 2] : 0.0;
     int nn, bb;
-    const double v = solid_fraction<NT>(act, dx, dy, dz, r_p * r_p, r_m * r_m, rr * rr, rr, N, sw, frac, rc, vc, nn, bb);
+    const double v = solid_fraction<NT>(act, dx, dy, dz, r_p * r_p, r_m * r_m, rr * rr, rr, N, sw, frac, rc, vc,
+                                        esamp, nn, bb);
     if (act) {
         sf[k] = v;
         cnt[k] = nn;
@@ -559,6 +596,7 @@ int run_voxelize(lbdem_psm* h) {
     fa.frac = 1.0 / (double)(fa.N * fa.N * fa.N);
     fa.rc = h->cell_rc;
     fa.vc = h->cell_vc;
+    fa.esamp = std::sqrt(3.0) * (0.5 - 0.5 / (double)fa.N) + 1e-12;
     const bool overlap = h->desc.voxel_mode == LBDEM_PSM_VOXEL_OVERLAP;
     const bool init_vel = (h->flags & LBDEM_PSM_INIT_VEL) != 0;
     fa.bin_cand_start = h->bin_cand_start;
@@ -626,11 +664,12 @@ int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_
     const double sw = 1.0 / (double)N, frac = 1.0 / (double)(N * N * N);
     dim3 grid((unsigned)std::max<int64_t>(1, (n + 255) / 256)), block(256);
     const double rc = h->cell_rc, vc = h->cell_vc;
+    const double es = std::sqrt(3.0) * (0.5 - 0.5 / (double)N) + 1e-12;
     if (h->desc.voxel_mode == LBDEM_PSM_VOXEL_OVERLAP)
-        hipLaunchKernelGGL(k_subcell<-1>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, d_sf, d_cnt, d_br);
-    else if (N == 5) hipLaunchKernelGGL(k_subcell<5>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, d_sf, d_cnt, d_br);
-    else if (N == 8) hipLaunchKernelGGL(k_subcell<8>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, d_sf, d_cnt, d_br);
-    else hipLaunchKernelGGL(k_subcell<0>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, d_sf, d_cnt, d_br);
+        hipLaunchKernelGGL(k_subcell<-1>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, es, d_sf, d_cnt, d_br);
+    else if (N == 5) hipLaunchKernelGGL(k_subcell<5>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, es, d_sf, d_cnt, d_br);
+    else if (N == 8) hipLaunchKernelGGL(k_subcell<8>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, es, d_sf, d_cnt, d_br);
+    else hipLaunchKernelGGL(k_subcell<0>, grid, block, 0, h->stream, n, d_d3, d_r, N, sw, frac, h->sqrt3half, rc, vc, es, d_sf, d_cnt, d_br);
     PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }
