@@ -289,7 +289,15 @@ __device__ __forceinline__ unsigned wave_count(double dx, double dy, double dz, 
         const double ax = dx + (-0.5 + ((double)i + 0.5) * sw);
         const double ay = dy + (-0.5 + ((double)j + 0.5) * sw);
         const double sxy = ax * ax + ay * ay;
-        if (NT > 0) {
+        if (NT > 0 && NT * NT == 64) {
+            /* the 64 sample columns fill the wavefront: every lane is active,
+             * and the compare's lane mask is the ballot */
+#pragma unroll
+            for (int k = 0; k < (NT > 0 ? NT : 1); ++k) {
+                const double az = dz + (-0.5 + ((double)k + 0.5) * sw);
+                total += (unsigned)__popcll(__ballot(sxy + az * az < rsq));
+            }
+        } else if (NT > 0) {
 #pragma unroll
             for (int k = 0; k < (NT > 0 ? NT : 1); ++k) {
                 const double az = dz + (-0.5 + ((double)k + 0.5) * sw);
@@ -440,6 +448,16 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
             const double dx = (double)X - cd.x[0];
             const double dy = (double)Y - cd.x[1];
             const double dz = (double)Z - cd.x[2];
+            /* Nothing changes for a lane outside the reach r + h of the
+             * sphere (its solid fraction is 0) unless it holds a record of
+             * this particle (the particle has left the cell, case 2 of the
+             * rule); a lane with no record stays empty.  Skip the candidate
+             * when that holds for every lane of the wavefront. */
+            {
+                const bool near = inb && dx * dx + dy * dy + dz * dz <= cd.rp2;
+                const bool mine = inb && ssf > SOLFRAC_MIN && sid == cd.id;
+                if (__ballot(near || mine) == 0ull) continue;
+            }
             int nn, br;
             const double sf = solid_fraction<NT>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, a.N, a.sw,
                                                  a.frac, a.rc, a.vc, a.esamp, nn, br);
