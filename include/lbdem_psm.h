@@ -153,6 +153,30 @@ int lbdem_psm_synchronize(lbdem_psm* h);
 int lbdem_psm_profile(lbdem_psm* h, int32_t enable);
 int lbdem_psm_profile_read(lbdem_psm* h, double* ms4, int64_t* count4, int32_t reset);
 
+/* Zhang-Kwok periodic pressure difference across the periodic axis
+ * `dimension` (ZhangPeriodicPressureFunctional3D,
+ * src/periodicPressureFunctionals3D.hh:118-147, applied as in
+ * examples/showcaseRectChannel/showcaseRectChannel.cpp:258-273) around every
+ * subsequent collide+stream: the mean densities of the inlet plane (index 0)
+ * and the outlet plane (index n-1) over the cross-section box are taken
+ * before the step; after it the populations entering the inlet (c_d = +1)
+ * are rescaled (in the f + t form) by rho_in / <rho>_outlet and those
+ * entering the outlet (c_d = -1) by rho_out / <rho>_inlet.
+ * range4 = {a0, a1, b0, b1}: inclusive box on the other two axes in
+ * increasing axis order; NULL = 1..n-2 on both (the showcase's inlet and
+ * outlet).  enable = 0 switches it off. */
+int lbdem_psm_set_periodic_pressure(lbdem_psm* h, int32_t enable, int32_t dimension, double rho_in,
+                                    double rho_out, const int32_t* range4);
+
+/* Equilibrium populations of PoiseuilleProfileAndPressureGradient
+ * (src/periodicPressureFunctionals3D.hh:259-298): density
+ * rho_hi - (rho_hi - rho_lo) i / (n_d - 1) along `dimension` and velocity
+ * u_max p(j, n_j) p(k, n_k) along it, p(i, n) = 1 - ((i - n/2)/(n/2))^2.
+ * n3: the profile sizes (NULL = the lattice's; the showcase passes
+ * nx - 1 along the axis). */
+int lbdem_psm_init_poiseuille(lbdem_psm* h, int32_t dimension, double rho_hi, double rho_lo, double u_max,
+                              const int32_t* n3);
+
 /* ---- Differentiable window (no counterpart in the reference) ----
  *
  * A window of nsteps collide+stream+force-sum steps over frozen geometry:

@@ -861,6 +861,8 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
     if (h->precision == LBDEM_PSM_FP16_STORE)
         return set_err(h, LBDEM_PSM_ERR_STATE, "the differentiable window supports FP64 and FP32_DEV storage");
     if (!h->have_particles) return set_err(h, LBDEM_PSM_ERR_STATE, "diff_forward before set_spheres");
+    if (h->pp.enabled)
+        return set_err(h, LBDEM_PSM_ERR_STATE, "the differentiable window does not include the periodic-pressure boundary");
     auto& D = h->diff;
     D.ready = false;
     PSM_HIP(h, hipStreamSynchronize(h->stream));

@@ -353,7 +353,7 @@ int lbdem_psm_destroy(lbdem_psm* h) {
     dfree(h->bin_fill); dfree(h->cand_key); dfree(h->cand_bin); dfree(h->p_nent);
     dfree(h->p_ent_start); dfree(h->owner_ent); dfree(h->partial); dfree(h->pft); dfree(h->psum_x);
     dfree(h->scan_tmp); dfree(h->counters); dfree(h->red_tmp); dfree(h->orphan);
-    dfree(h->dv_cell); dfree(h->dv_us);
+    dfree(h->dv_cell); dfree(h->dv_us); dfree(h->pp.buf);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     for (auto& s : h->spans) { (void)hipEventDestroy(s.a); (void)hipEventDestroy(s.b); }
     for (auto e : h->ev_pool) (void)hipEventDestroy(e);

@@ -192,13 +192,15 @@ static void launch_any(lbdem_psm* h, const CollideArgs& a, const void* fin, void
 
 int launch_collide_stream(lbdem_psm* h) {
     CollideArgs a = make_args(h);
+    int rc = periodic_pressure_pre(h);
+    if (rc) return rc;
     hipEvent_t ev = nullptr;
     prof_begin(h, 0, &ev);
     launch_any(h, a, h->f[h->cur_f], h->f[1 - h->cur_f]);
     PSM_HIP(h, hipGetLastError());
     prof_end(h, 0, ev);
     h->cur_f = 1 - h->cur_f;
-    return LBDEM_PSM_OK;
+    return periodic_pressure_post(h);
 }
 
 /* One collide+stream between caller-chosen buffers; the per-cell forces go

@@ -405,6 +405,15 @@ struct lbdem_psm {
     double* dv_us = nullptr;
     int64_t red_blocks = 0;
 
+    /* Zhang-Kwok periodic pressure across one periodic axis */
+    struct PeriodicPressure {
+        bool enabled = false;
+        int dim = 0;
+        double rho_in = 1.0, rho_out = 1.0;
+        int range[4] = {0, 0, 0, 0};   /* inclusive cross-section box on the other two axes */
+        double* buf = nullptr;         /* [0] inlet rhoBar sum, [1] outlet sum, partials from [8] */
+    } pp;
+
     /* differentiable window */
     int64_t geom_gen = 0;            /* bumped by every change of records or particles */
     struct Diff {
@@ -450,6 +459,8 @@ int launch_pop_convert(lbdem_psm* h, double* d_f64, bool to_device_store);
 int launch_pop_convert_buf(lbdem_psm* h, void* buf, double* d_f64, bool to_device_store);
 int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, double* rF);
 int diff_free(lbdem_psm* h);
+int periodic_pressure_pre(lbdem_psm* h);
+int periodic_pressure_post(lbdem_psm* h);
 int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
                         int64_t* n_nonfinite);
 int run_voxelize(lbdem_psm* h);

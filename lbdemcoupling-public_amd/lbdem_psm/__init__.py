@@ -139,6 +139,10 @@ _SIGS = {
     "lbdem_psm_diff_forward": ([_P, ctypes.c_int32, ctypes.c_int32, _P], ctypes.c_int),
     "lbdem_psm_diff_backward": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "lbdem_psm_diff_release": ([_P], ctypes.c_int),
+    "lbdem_psm_set_periodic_pressure": ([_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _P],
+                                        ctypes.c_int),
+    "lbdem_psm_init_poiseuille": ([_P, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P],
+                                  ctypes.c_int),
 }
 
 
@@ -367,6 +371,19 @@ class Lattice:
         self._c(self.lib.lbdem_psm_subcell_counts(self.h, n, _ptr(d), _ptr(r), _ptr(sf), _ptr(cnt),
                                                   _ptr(br)))
         return sf, cnt, br
+
+    # boundary conditions and initial fields ----------------------------
+    def set_periodic_pressure(self, dimension, rho_in, rho_out, range4=None, enable=True):
+        """Zhang-Kwok periodic pressure across a periodic axis (lbdem_psm_set_periodic_pressure)."""
+        r = None if range4 is None else np.ascontiguousarray(range4, dtype=np.int32)
+        self._c(self.lib.lbdem_psm_set_periodic_pressure(self.h, int(bool(enable)), int(dimension),
+                                                         float(rho_in), float(rho_out), _ptr(r)))
+
+    def init_poiseuille(self, dimension, rho_hi, rho_lo, u_max, n3=None):
+        """PoiseuilleProfileAndPressureGradient at equilibrium (lbdem_psm_init_poiseuille)."""
+        n = None if n3 is None else np.ascontiguousarray(n3, dtype=np.int32)
+        self._c(self.lib.lbdem_psm_init_poiseuille(self.h, int(dimension), float(rho_hi), float(rho_lo),
+                                                   float(u_max), _ptr(n)))
 
     # differentiable window ---------------------------------------------
     def set_tau(self, tau):

@@ -44,6 +44,8 @@ def load():
         "orc_last_n_voxelized": ([_P], ctypes.c_int64),
         "orc_overlap_fraction": ([ctypes.c_double, ctypes.c_double, _P, _P], ctypes.c_double),
         "orc_set_voxel_mode": ([_P, ctypes.c_int], None),
+        "orc_set_periodic_pressure": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P], None),
+        "orc_init_poiseuille": ([_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P], None),
     }
     for name, (a, r) in sigs.items():
         fn = getattr(lib, name)
@@ -168,6 +170,15 @@ class OracleLattice:
         ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.int64).ravel()
         us = None if us is None else np.ascontiguousarray(us, dtype=np.float64).reshape(3, -1)
         self.lib.orc_set_cell_records(self.h, _p(sf), _p(ids), _p(us))
+
+    def set_periodic_pressure(self, dimension, rho_in, rho_out, range4=None, enable=True):
+        r = None if range4 is None else np.ascontiguousarray(range4, dtype=np.int32)
+        self.lib.orc_set_periodic_pressure(self.h, int(bool(enable)), int(dimension), float(rho_in),
+                                           float(rho_out), _p(r))
+
+    def init_poiseuille(self, dimension, rho_hi, rho_lo, u_max, n3=None):
+        n = None if n3 is None else np.ascontiguousarray(n3, dtype=np.int32)
+        self.lib.orc_init_poiseuille(self.h, int(dimension), float(rho_hi), float(rho_lo), float(u_max), _p(n))
 
     def conservation(self):
         m = ctypes.c_double()

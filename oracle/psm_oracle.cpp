@@ -178,7 +178,59 @@ struct orc_lattice {
     std::map<int64_t, int64_t> idmap;
     int64_t last_n_vox = 0;
     int voxel_mode = 0;
+    /* periodic pressure */
+    bool pp = false;
+    int pp_dim = 0;
+    double pp_in = 1.0, pp_out = 1.0;
+    int pp_range[4] = {0, 0, 0, 0};
 };
+
+namespace {
+/* cell (i along dim, a, b along the other two axes in increasing order) */
+int64_t plane_cell(const orc_lattice* L, int dim, int i, int a, int b) {
+    int x, y, z;
+    if (dim == 0) { x = i; y = a; z = b; }
+    else if (dim == 1) { x = a; y = i; z = b; }
+    else { x = a; y = b; z = i; }
+    return ((int64_t)x * L->n[1] + y) * L->n[2] + z;
+}
+
+/* computeAverageDensity over the plane's box: 1 + mean rhoBar */
+double plane_avg_density(const orc_lattice* L, int plane) {
+    double sum = 0.0;
+    int64_t cnt = 0;
+    for (int a = L->pp_range[0]; a <= L->pp_range[1]; ++a)
+        for (int b = L->pp_range[2]; b <= L->pp_range[3]; ++b) {
+            const int64_t c = plane_cell(L, L->pp_dim, plane, a, b);
+            double rb = 0.0;
+            for (int i = 0; i < Q; ++i) rb = rb + L->f[c * Q + i];
+            sum += rb;
+            ++cnt;
+        }
+    return 1.0 + sum / (double)cnt;
+}
+
+/* ZhangPeriodicPressureFunctional3D::process on the plane's box */
+void zhang_rescale(orc_lattice* L, int plane, int dir, double rho_target, double rho_avg) {
+    const double rescale = rho_target / rho_avg;
+    for (int a = L->pp_range[0]; a <= L->pp_range[1]; ++a)
+        for (int b = L->pp_range[2]; b <= L->pp_range[3]; ++b) {
+            const int64_t c = plane_cell(L, L->pp_dim, plane, a, b);
+            for (int i = 0; i < Q; ++i) {
+                if (C[i][L->pp_dim] != dir) continue;
+                double fTmp = L->f[c * Q + i] + W[i];
+                fTmp *= rescale;
+                L->f[c * Q + i] = fTmp - W[i];
+            }
+        }
+}
+
+double poiseuille_velocity(double i, double n) {
+    const double nHalf = n / 2.;
+    const double x = (i - nHalf) / (nHalf);
+    return (1. - x * x);
+}
+}  // namespace
 
 extern "C" {
 
@@ -420,6 +472,11 @@ void orc_collide_stream(orc_lattice* L, int nsteps, int nthreads) {
     const int nx = L->n[0], ny = L->n[1], nz = L->n[2];
     const double* a = L->has_force ? L->accel : nullptr;
     for (int step = 0; step < nsteps; ++step) {
+        double avg_in = 0.0, avg_out = 0.0;
+        if (L->pp) {
+            avg_in = plane_avg_density(L, 0);
+            avg_out = plane_avg_density(L, L->n[L->pp_dim] - 1);
+        }
 #pragma omp parallel for num_threads(nt) schedule(static)
         for (int iX = 0; iX < nx; ++iX) {
             double cell[Q];
@@ -451,7 +508,51 @@ void orc_collide_stream(orc_lattice* L, int nsteps, int nthreads) {
                 }
         }
         L->f.swap(L->ftmp);
+        if (L->pp) {
+            zhang_rescale(L, 0, 1, L->pp_in, avg_out);
+            zhang_rescale(L, L->n[L->pp_dim] - 1, -1, L->pp_out, avg_in);
+        }
     }
+}
+
+void orc_set_periodic_pressure(orc_lattice* L, int enable, int dim, double rho_in, double rho_out,
+                               const int32_t* range4) {
+    L->pp = enable != 0;
+    L->pp_dim = dim;
+    L->pp_in = rho_in;
+    L->pp_out = rho_out;
+    const int ax = dim == 0 ? 1 : 0, bx = dim == 2 ? 1 : 2;
+    if (range4) {
+        for (int k = 0; k < 4; ++k) L->pp_range[k] = range4[k];
+    } else {
+        L->pp_range[0] = 1; L->pp_range[1] = L->n[ax] - 2;
+        L->pp_range[2] = 1; L->pp_range[3] = L->n[bx] - 2;
+    }
+}
+
+void orc_init_poiseuille(orc_lattice* L, int dim, double pHi, double pLo, double uMax, const int32_t* n3) {
+    int pn[3];
+    for (int d = 0; d < 3; ++d) pn[d] = n3 ? n3[d] : L->n[d];
+    for (int x = 0; x < L->n[0]; ++x)
+        for (int y = 0; y < L->n[1]; ++y)
+            for (int z = 0; z < L->n[2]; ++z) {
+                double density, u[3] = {0.0, 0.0, 0.0};
+                if (dim == 0) {
+                    density = pHi - (pHi - pLo) * (double)x / (double)(pn[0] - 1);
+                    u[0] = uMax * poiseuille_velocity((double)y, (double)pn[1]) * poiseuille_velocity((double)z, (double)pn[2]);
+                } else if (dim == 1) {
+                    density = pHi - (pHi - pLo) * (double)y / (double)(pn[1] - 1);
+                    u[1] = uMax * poiseuille_velocity((double)x, (double)pn[0]) * poiseuille_velocity((double)z, (double)pn[2]);
+                } else {
+                    density = pHi - (pHi - pLo) * (double)z / (double)(pn[2] - 1);
+                    u[2] = uMax * poiseuille_velocity((double)x, (double)pn[0]) * poiseuille_velocity((double)y, (double)pn[1]);
+                }
+                const double rhoBar = density - 1.0;
+                double j[3];
+                for (int d = 0; d < 3; ++d) j[d] = density * u[d];
+                const int64_t c = ((int64_t)x * L->n[1] + y) * L->n[2] + z;
+                equilibria(rhoBar, 1.0 / (rhoBar + 1.0), j, norm_sqr(j), &L->f[c * Q]);
+            }
 }
 
 /* SumForceTorque3D::process, src/ibProcessors3D.hh:202-263, on one block

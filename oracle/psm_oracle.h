@@ -97,6 +97,19 @@ double orc_overlap_fraction(double d, double r, double* dsf_dd, int32_t* branch_
  * the reference's. */
 void orc_set_voxel_mode(orc_lattice* L, int mode);
 
+/* Zhang-Kwok periodic pressure across the periodic axis dim around every
+ * later orc_collide_stream step (src/periodicPressureFunctionals3D.hh:118-147,
+ * examples/showcaseRectChannel/showcaseRectChannel.cpp:258-273).  range4:
+ * inclusive box {a0,a1,b0,b1} on the other two axes (NULL: 1..n-2). */
+void orc_set_periodic_pressure(orc_lattice* L, int enable, int dim, double rho_in, double rho_out,
+                               const int32_t* range4);
+
+/* PoiseuilleProfileAndPressureGradient at equilibrium
+ * (src/periodicPressureFunctionals3D.hh:259-298); n3: profile sizes
+ * (NULL: the lattice's). */
+void orc_init_poiseuille(orc_lattice* L, int dim, double rho_hi, double rho_lo, double u_max,
+                         const int32_t* n3);
+
 /* FP64 global mass deviation sum_i sum_c fbar and momentum sum c_i fbar. */
 void orc_conservation(const orc_lattice* L, double* mass, double* mom3);
 

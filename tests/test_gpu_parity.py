@@ -330,3 +330,66 @@ def test_record_capacity_error():
         g.set_spheres([[12.0, 12.0, 12.0]], [[0.0, 0, 0]], [[0.0, 0, 0]], [5.0], [1])
     assert e.value.code == L.ERR_CAPACITY
     g.close()
+
+
+def test_poiseuille_init_matches_oracle():
+    n = (12, 10, 14)
+    for prec, tol in (("fp64", 0.0), ("fp32_dev", 1e-7)):
+        g = L.Lattice(*n, periodic=(1, 0, 0), tau=0.8, precision=prec)
+        o = O.OracleLattice(*n, periodic=(1, 0, 0), tau=0.8)
+        for dim in (0, 2):
+            n3 = list(n)
+            n3[dim] -= 1
+            g.init_poiseuille(dim, 1.002, 0.998, 0.03, n3=n3)
+            o.init_poiseuille(dim, 1.002, 0.998, 0.03, n3=n3)
+            fg, fo = g.get_populations(), o.get_populations()
+            if tol == 0.0:
+                assert np.array_equal(fg, fo)
+            else:
+                assert np.abs(fg - fo).max() < tol
+
+
+def test_periodic_pressure_channel_matches_oracle():
+    """Pressure-driven channel (periodic x with the Zhang-Kwok boundary,
+    walls y/z) with spheres: populations and forces against the oracle."""
+    n = (24, 18, 20)
+    g, o = make_pair(n, periodic=(1, 0, 0), tau=0.6, N=5)
+    for lat in (g, o):
+        lat.init_poiseuille(0, 1.001, 0.999, 0.02, n3=(n[0] - 1, n[1], n[2]))
+        lat.set_periodic_pressure(0, 1.001, 0.999)
+    x = np.array([[8.2, 9.1, 10.3], [17.6, 7.4, 9.8]])
+    v = np.array([[0.004, 0.0, 0.001], [-0.002, 0.003, 0.0]])
+    w = np.zeros((2, 3))
+    r = np.array([3.5, 3.0])
+    ids = np.array([1, 2])
+    g.set_spheres(x, v, w, r, ids)
+    o.set_spheres(x, v, w, r, ids)
+    for _ in range(12):
+        g.collide_stream(1)
+        o.collide_stream(1, 1)
+    Fg, Tg = g.sum_force_torque(x)
+    Fo, To = o.sum_force_torque(x)
+    assert rel_err(g.get_populations(), o.get_populations()) < 1e-12
+    assert rel_err(Fg, Fo) < 1e-12
+    assert torque_err(Tg, To, Fo) < 1e-12
+    # switching it off stops the rescale
+    g.set_periodic_pressure(0, 1.0, 1.0, enable=False)
+    o.set_periodic_pressure(0, 1.0, 1.0, enable=False)
+    g.collide_stream(2)
+    o.collide_stream(2, 1)
+    assert rel_err(g.get_populations(), o.get_populations()) < 1e-12
+
+
+def test_periodic_pressure_errors():
+    g = L.Lattice(8, 8, 8, periodic=(1, 0, 1), tau=0.8)
+    with pytest.raises(L.PSMError):
+        g.set_periodic_pressure(1, 1.01, 0.99)          # y is not periodic
+    with pytest.raises(L.PSMError):
+        g.set_periodic_pressure(0, 1.01, 0.99, range4=(0, 8, 0, 7))   # box beyond the lattice
+    with pytest.raises(L.PSMError):
+        g.set_periodic_pressure(0, -1.0, 0.99)
+    g.set_periodic_pressure(0, 1.01, 0.99)
+    g.set_spheres(np.array([[4.0, 4.0, 4.0]]), np.zeros((1, 3)), np.zeros((1, 3)), np.array([2.0]),
+                  np.array([1]))
+    with pytest.raises(L.PSMError):
+        g.diff_forward(2, 1)                             # not part of the differentiable window

@@ -474,3 +474,84 @@ def test_define_velocity_sets_particle_equilibrium():
     assert np.allclose(f[:, c], O.equilibrium(0.0, v[0]), atol=1e-18, rtol=0)
     far = 0
     assert np.allclose(f[:, far], 0.0, atol=1e-18)
+
+
+# ---------------------------------------------------------------- boundaries
+
+def _macro(f):
+    C = np.array([[0, 0, 0], [-1, 0, 0], [0, -1, 0], [0, 0, -1], [-1, -1, 0], [-1, 1, 0], [-1, 0, -1],
+                  [-1, 0, 1], [0, -1, -1], [0, -1, 1], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0],
+                  [1, -1, 0], [1, 0, 1], [1, 0, -1], [0, 1, 1], [0, 1, -1]])
+    rho = 1.0 + f.sum(0)
+    return rho, (C.T @ f) / rho
+
+
+def test_poiseuille_initial_field_closed_form():
+    """PoiseuilleProfileAndPressureGradient: the density ramps from rho_hi at
+    index 0 to rho_lo at index n-1 of the profile size, the velocity is
+    u_max p(j) p(k) with p(n/2) = 1 and p(0) = 0, along the axis only."""
+    n = (10, 9, 12)
+    o = O.OracleLattice(*n, periodic=(1, 0, 0), tau=0.8)
+    o.init_poiseuille(0, 1.002, 0.998, 0.02, n3=(n[0] - 1, n[1], n[2]))
+    rho, u = _macro(o.get_populations())
+    rho, u = rho.reshape(n), u.reshape((3,) + n)
+    assert rho[0, 3, 3] == pytest.approx(1.002, abs=1e-14)
+    assert rho[n[0] - 2, 3, 3] == pytest.approx(0.998, abs=1e-14)      # index nx-1 of the profile size nx-1
+    assert np.allclose(np.diff(rho[:, 4, 5]), -0.004 / (n[0] - 2), atol=1e-14)
+    # p(j, 9) p(k, 12) at j = 4.5 is not a lattice point: check a direct value
+    pj = 1 - ((4 - 4.5) / 4.5) ** 2
+    pk = 1 - ((6 - 6.0) / 6.0) ** 2
+    assert u[0, 2, 4, 6] == pytest.approx(0.02 * pj * pk, rel=1e-13)
+    assert u[0, 2, 0, 6] == pytest.approx(0.0, abs=1e-16)
+    assert np.abs(u[1]).max() < 1e-17 and np.abs(u[2]).max() < 1e-17   # round-off of the moment sums
+
+
+def test_periodic_pressure_drives_poiseuille_flow():
+    """Zhang-Kwok periodic pressure across x in a channel with walls in y:
+    the density falls linearly along x and the steady centreline velocity
+    matches the plane-Poiseuille value u = g/(2 nu) (H/2)^2 (H = ny with
+    half-way bounce-back) for the measured mid-channel gradient
+    g = c_s^2 drho/dx / rho, within 3 %; the profile is a parabola."""
+    n, tau = (12, 14, 3), 0.8
+    o = O.OracleLattice(*n, periodic=(1, 0, 1), tau=tau)
+    o.init_equilibrium()
+    o.set_periodic_pressure(0, 1.0005, 0.9995, range4=(1, n[1] - 2, 0, n[2] - 1))
+    o.collide_stream(6000, 1)
+    rho, u = _macro(o.get_populations())
+    rho, ux = rho.reshape(n), u[0].reshape(n)
+    assert np.all(np.diff(rho[:, 7, 1]) < 0)
+    grad = (rho[5, 7, 1] - rho[7, 7, 1]) / 2.0
+    nu = (tau - 0.5) / 3.0
+    y = np.arange(n[1])
+    prof = ux[6, :, 1]
+    want = grad / 3.0 / rho[6, 7, 1] / (2 * nu) * (y + 0.5) * (n[1] - 0.5 - y)
+    assert prof.max() == pytest.approx(want.max(), rel=0.03)
+    fit = np.polyfit(y, prof, 2)
+    assert np.abs(np.polyval(fit, y) - prof).max() < 0.01 * prof.max()
+
+
+def test_periodic_pressure_touches_only_the_planes():
+    """The rescale acts on the populations that crossed the periodic
+    boundary into the inlet (c_x = +1) and outlet (c_x = -1) planes only."""
+    n = (8, 6, 5)
+    nc = int(np.prod(n))
+    res = []
+    for pp in (False, True):
+        o = O.OracleLattice(*n, periodic=(1, 1, 1), tau=0.7)
+        o.init_equilibrium(1 + 0.01 * np.random.default_rng(1).normal(size=nc),
+                           0.02 * np.random.default_rng(2).normal(size=(3, nc)))
+        if pp:
+            o.set_periodic_pressure(0, 1.01, 0.99)
+        o.collide_stream(1, 1)
+        res.append(o.get_populations().reshape(19, *n))
+    d = res[1] != res[0]
+    cx = np.array([0, -1, 0, 0, -1, -1, -1, -1, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0])
+    assert d.any()
+    for i in range(19):
+        for x in range(n[0]):
+            changed = d[i, x].any()
+            if changed:
+                assert (x == 0 and cx[i] == 1) or (x == n[0] - 1 and cx[i] == -1)
+            if cx[i] == 1 and x == 0 or cx[i] == -1 and x == n[0] - 1:
+                # the default box is 1..n-2 on the other axes: its interior changed, its rim did not
+                assert d[i, x, 1:-1, 1:-1].all() and not d[i, x, 0, :].any()
