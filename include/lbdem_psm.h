@@ -136,6 +136,43 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
 int lbdem_psm_subcell_counts(lbdem_psm* h, int64_t n, const double* d3, const double* r,
                              double* sf, int32_t* count, int32_t* branch);
 
+/* Density and velocity per cell, computed on the device (Palabos
+ * computeDensity / computeVelocity of the BGK cell; with Guo forcing
+ * u = j/rho + a/2): rho (Nc), u3 (3*Nc, component-major); either may be
+ * NULL. */
+int lbdem_psm_get_macroscopic(lbdem_psm* h, double* rho, double* u3);
+/* The descriptor the handle was created with (tau as last set). */
+int lbdem_psm_get_desc(const lbdem_psm* h, lbdem_psm_desc* out);
+
+/* ---- On-disk formats ----
+ * VTK XML ImageData (.vti), Float32 point data, x fastest, binary base64
+ * with a UInt32 byte-count header -- the format of Palabos' VtkImageOutput3D
+ * used by the reference's writeVTK (examples/benchmark/benchmark.cpp:74-99).
+ * data[k]: ncomp[k]*Nc doubles in the host layout (component-major, z
+ * fastest); scale may be NULL.  Host only, no device needed. */
+int lbdem_psm_vti_write(const char* path, int32_t nx, int32_t ny, int32_t nz, double spacing, int32_t nfields,
+                        const char* const* names, const int32_t* ncomp, const double* const* data,
+                        const double* scale);
+/* The reference's writeVTK fields of a lattice: velocity * vel_scale,
+ * density * rho_scale, pressure (rho - 1) * p_scale, solid fraction. */
+int lbdem_psm_write_vti(lbdem_psm* h, const char* path, double spacing, double vel_scale, double rho_scale,
+                        double p_scale);
+/* Checkpoint file: header, then per cell (z fastest) the 19 populations
+ * f - t in FP64 followed by the coupling record in the serialize order of
+ * IBcompositeDynamics (src/ibCompositeDynamics3D.hh:77-89): uPart[3],
+ * hydrodynamicForce[3], solidFraction, partId (int64).  The _arrays forms
+ * are host only (host layouts as get_populations / get_cell_records). */
+int lbdem_psm_checkpoint_write_arrays(const char* path, const lbdem_psm_desc* desc, const double* f,
+                                      const double* sf, const int64_t* id, const double* us, const double* F);
+int lbdem_psm_checkpoint_read_desc(const char* path, lbdem_psm_desc* desc);
+int lbdem_psm_checkpoint_read_arrays(const char* path, double* f, double* sf, int64_t* id, double* us,
+                                     double* F);
+int lbdem_psm_save_checkpoint(lbdem_psm* h, const char* path);
+/* Restores populations and records (the records' forces are recomputed by
+ * the next collision; all records are summed on the stale-record path
+ * until the next set_spheres). */
+int lbdem_psm_load_checkpoint(lbdem_psm* h, const char* path);
+
 /* FP64 on-device global sums: mass deviation sum(f - t) and momentum. */
 int lbdem_psm_conservation(lbdem_psm* h, double* mass, double* mom3);
 /* Health check: non-finite populations and max |u|; returns

@@ -195,6 +195,11 @@ extern "C" {
 
 const char* lbdem_psm_version(void) { return "lbdem_psm 0.1.0 (gfx950 HIP)"; }
 
+/* Error message setter for the host-only units of the library. */
+__attribute__((visibility("hidden"))) int lbdem_psm_internal_set_error(lbdem_psm* h, int code, const char* msg) {
+    return set_err(h, code, msg ? std::string(msg) : std::string());
+}
+
 int lbdem_psm_desc_init(lbdem_psm_desc* d) {
     if (!d) return LBDEM_PSM_ERR_INVALID_ARG;
     memset(d, 0, sizeof(*d));
@@ -638,6 +643,33 @@ int lbdem_psm_subcell_counts(lbdem_psm* h, int64_t n, const double* d3, const do
     dfree(dd); dfree(dr); dfree(dsf); dfree(dc); dfree(db);
     if (rc) return rc;
     PSM_HIP(h, e);
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_get_macroscopic(lbdem_psm* h, double* rho, double* u3) {
+    if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
+    const int64_t nc = h->g.nc;
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    double *dr = nullptr, *du = nullptr;
+    if (rho) PSM_HIP(h, dalloc(&dr, nc * sizeof(double)));
+    if (u3) {
+        hipError_t e = dalloc(&du, 3 * nc * sizeof(double));
+        if (e != hipSuccess) { dfree(dr); return hip_check(h, e, "hipMalloc"); }
+    }
+    int rc = (rho || u3) ? launch_macroscopic(h, dr, du) : LBDEM_PSM_OK;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (!rc && e == hipSuccess && rho) e = hipMemcpy(rho, dr, nc * sizeof(double), hipMemcpyDeviceToHost);
+    if (!rc && e == hipSuccess && u3) e = hipMemcpy(u3, du, 3 * nc * sizeof(double), hipMemcpyDeviceToHost);
+    dfree(dr);
+    dfree(du);
+    if (rc) return rc;
+    PSM_HIP(h, e);
+    return LBDEM_PSM_OK;
+}
+
+int lbdem_psm_get_desc(const lbdem_psm* h, lbdem_psm_desc* out) {
+    if (!h || !out) return LBDEM_PSM_ERR_INVALID_ARG;
+    *out = h->desc;
     return LBDEM_PSM_OK;
 }
 

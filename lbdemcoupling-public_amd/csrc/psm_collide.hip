@@ -292,6 +292,51 @@ int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
     return LBDEM_PSM_OK;
 }
 
+/* Density and velocity per cell (Palabos computeDensity / computeVelocity
+ * of the BGK cell; with Guo forcing u = j/rho + a/2). Host order: c = (x*ny
+ * + y)*nz + z, velocity component-major. */
+template <int PREC>
+__global__ __launch_bounds__(256) void k_macroscopic(const typename Store<PREC>::S* __restrict__ f, int64_t nc, int nz,
+                                                     PopLayout L, double a0, double a1, double a2,
+                                                     double* __restrict__ rho, double* __restrict__ u) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = c / nz;
+        const int z = (int)(c - row * nz);
+        double fc[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) fc[i] = (double)Store<PREC>::load(f + pop_at(row, i, z, L));
+        double rb, j[3];
+        moments<double>(fc, rb, j);
+        const double r = rb + 1.0;
+        if (rho) rho[c] = r;
+        if (u) {
+            u[c] = j[0] / r + 0.5 * a0;
+            u[nc + c] = j[1] / r + 0.5 * a1;
+            u[2 * nc + c] = j[2] / r + 0.5 * a2;
+        }
+    }
+}
+
+int launch_macroscopic(lbdem_psm* h, double* d_rho, double* d_u) {
+    const int64_t nc = h->g.nc;
+    dim3 grid(grid_for(nc)), block(256);
+    const double* a = h->desc.body_accel;
+    switch (h->precision) {
+        case LBDEM_PSM_FP64:
+            hipLaunchKernelGGL(k_macroscopic<LBDEM_PSM_FP64>, grid, block, 0, h->stream, (const double*)h->f[h->cur_f], nc, h->g.n[2], h->g.L, a[0], a[1], a[2], d_rho, d_u);
+            break;
+        case LBDEM_PSM_FP32_DEV:
+            hipLaunchKernelGGL(k_macroscopic<LBDEM_PSM_FP32_DEV>, grid, block, 0, h->stream, (const float*)h->f[h->cur_f], nc, h->g.n[2], h->g.L, a[0], a[1], a[2], d_rho, d_u);
+            break;
+        default:
+            hipLaunchKernelGGL(k_macroscopic<LBDEM_PSM_FP16_STORE>, grid, block, 0, h->stream, (const _Float16*)h->f[h->cur_f], nc, h->g.n[2], h->g.L, a[0], a[1], a[2], d_rho, d_u);
+            break;
+    }
+    PSM_HIP(h, hipGetLastError());
+    return LBDEM_PSM_OK;
+}
+
 int launch_define_velocity(lbdem_psm* h) {
     if (!h->dv_cell) return LBDEM_PSM_OK;
     dim3 grid(1024), block(256);

@@ -460,6 +460,7 @@ int launch_pop_convert_buf(lbdem_psm* h, void* buf, double* d_f64, bool to_devic
 int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, double* rF);
 int diff_free(lbdem_psm* h);
 int periodic_pressure_pre(lbdem_psm* h);
+int launch_macroscopic(lbdem_psm* h, double* d_rho, double* d_u);
 int periodic_pressure_post(lbdem_psm* h);
 int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
                         int64_t* n_nonfinite);

@@ -143,6 +143,17 @@ _SIGS = {
                                         ctypes.c_int),
     "lbdem_psm_init_poiseuille": ([_P, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P],
                                   ctypes.c_int),
+    "lbdem_psm_get_macroscopic": ([_P, _P, _P], ctypes.c_int),
+    "lbdem_psm_get_desc": ([_P, _P], ctypes.c_int),
+    "lbdem_psm_vti_write": ([ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                             ctypes.c_int32, _P, _P, _P, _P], ctypes.c_int),
+    "lbdem_psm_write_vti": ([_P, ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                             ctypes.c_double], ctypes.c_int),
+    "lbdem_psm_checkpoint_write_arrays": ([ctypes.c_char_p, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "lbdem_psm_checkpoint_read_desc": ([ctypes.c_char_p, _P], ctypes.c_int),
+    "lbdem_psm_checkpoint_read_arrays": ([ctypes.c_char_p, _P, _P, _P, _P, _P], ctypes.c_int),
+    "lbdem_psm_save_checkpoint": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "lbdem_psm_load_checkpoint": ([_P, ctypes.c_char_p], ctypes.c_int),
 }
 
 
@@ -236,6 +247,47 @@ def generate_spheres(box, count, radius_law=RADIUS_CONST, a=1.0, b=0.0, lo=0.0, 
     if rc != 0:
         raise PSMError(ERR_INVALID_ARG, "could not generate the sphere packing")
     return x, v, w, r, ids
+
+
+def vti_write(path, n, fields, spacing=1.0):
+    """Write {name: array} (scalar (Nc,) or vector (3, Nc), host layout) as VTK ImageData."""
+    names, ncomp, data, keep = [], [], [], []
+    for name, arr in fields.items():
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        m = 1 if a.ndim == 1 else a.shape[0]
+        keep.append(a)
+        names.append(name.encode())
+        ncomp.append(m)
+    nf = len(names)
+    cnames = (ctypes.c_char_p * max(nf, 1))(*names)
+    cncomp = np.ascontiguousarray(ncomp, dtype=np.int32)
+    cdata = (ctypes.c_void_p * max(nf, 1))(*[a.ctypes.data for a in keep])
+    rc = load_library().lbdem_psm_vti_write(str(path).encode(), int(n[0]), int(n[1]), int(n[2]), float(spacing),
+                                            nf, ctypes.cast(cnames, ctypes.c_void_p), _ptr(cncomp),
+                                            ctypes.cast(cdata, ctypes.c_void_p), None)
+    _check(rc)
+
+
+def checkpoint_write(path, desc, f, sf, ids, us, F):
+    """Checkpoint file from host arrays (lbdem_psm_checkpoint_write_arrays)."""
+    f = _f64(f).reshape(Q, -1)
+    sf, us, F = _f64(sf).ravel(), _f64(us).reshape(3, -1), _f64(F).reshape(3, -1)
+    ids = np.ascontiguousarray(ids, dtype=np.int64).ravel()
+    _check(load_library().lbdem_psm_checkpoint_write_arrays(str(path).encode(), ctypes.byref(desc), _ptr(f),
+                                                            _ptr(sf), _ptr(ids), _ptr(us), _ptr(F)))
+
+
+def checkpoint_read(path):
+    """(desc, dict of arrays) of a checkpoint file."""
+    d = Desc()
+    lib = load_library()
+    _check(lib.lbdem_psm_checkpoint_read_desc(str(path).encode(), ctypes.byref(d)))
+    nc = d.nx * d.ny * d.nz
+    out = {"f": np.zeros((Q, nc)), "sf": np.zeros(nc), "id": np.zeros(nc, dtype=np.int64),
+           "us": np.zeros((3, nc)), "F": np.zeros((3, nc))}
+    _check(lib.lbdem_psm_checkpoint_read_arrays(str(path).encode(), _ptr(out["f"]), _ptr(out["sf"]),
+                                                _ptr(out["id"]), _ptr(out["us"]), _ptr(out["F"])))
+    return d, out
 
 
 def advance_positions(x, v, n, periodic):
@@ -371,6 +423,24 @@ class Lattice:
         self._c(self.lib.lbdem_psm_subcell_counts(self.h, n, _ptr(d), _ptr(r), _ptr(sf), _ptr(cnt),
                                                   _ptr(br)))
         return sf, cnt, br
+
+    # fields and files --------------------------------------------------
+    def macroscopic(self):
+        """(rho (Nc,), u (3, Nc)) computed on the device."""
+        rho = np.zeros(self.nc)
+        u = np.zeros((3, self.nc))
+        self._c(self.lib.lbdem_psm_get_macroscopic(self.h, _ptr(rho), _ptr(u)))
+        return rho, u
+
+    def write_vti(self, path, spacing=1.0, vel_scale=1.0, rho_scale=1.0, p_scale=1.0):
+        self._c(self.lib.lbdem_psm_write_vti(self.h, str(path).encode(), float(spacing), float(vel_scale),
+                                             float(rho_scale), float(p_scale)))
+
+    def save_checkpoint(self, path):
+        self._c(self.lib.lbdem_psm_save_checkpoint(self.h, str(path).encode()))
+
+    def load_checkpoint(self, path):
+        self._c(self.lib.lbdem_psm_load_checkpoint(self.h, str(path).encode()))
 
     # boundary conditions and initial fields ----------------------------
     def set_periodic_pressure(self, dimension, rho_in, rho_out, range4=None, enable=True):

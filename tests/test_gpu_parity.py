@@ -393,3 +393,55 @@ def test_periodic_pressure_errors():
                   np.array([1]))
     with pytest.raises(L.PSMError):
         g.diff_forward(2, 1)                             # not part of the differentiable window
+
+
+def test_macroscopic_fields_match_populations():
+    n = (10, 8, 12)
+    g = L.Lattice(*n, periodic=(1, 1, 1), tau=0.8, body_accel=(0, 0, 1e-4))
+    rho0, u0 = random_state(n, 5)
+    g.init_equilibrium(rho0, u0)
+    g.collide_stream(2)
+    rho, u = g.macroscopic()
+    r_h, u_h = L.macroscopic(g.get_populations(), body_accel=(0, 0, 1e-4))
+    assert np.abs(rho - r_h).max() < 1e-15
+    assert np.abs(u - u_h).max() < 1e-15
+
+
+def test_checkpoint_save_load_continues_identically(tmp_path):
+    n = (20, 18, 22)
+    x = np.array([[6.2, 7.1, 8.3], [13.6, 10.4, 14.8]])
+    v = np.array([[0.004, 0.0, 0.001], [-0.002, 0.003, 0.0]])
+    w = np.array([[0.0, 0.001, 0.0], [0.0, 0.0, -0.001]])
+    r = np.array([3.5, 3.0])
+    ids = np.array([1, 2])
+    a = L.Lattice(*n, periodic=(1, 1, 1), tau=0.7, slices_per_dim=5)
+    a.init_equilibrium(*random_state(n, 8))
+    a.set_spheres(x, v, w, r, ids)
+    a.collide_stream(3)
+    a.save_checkpoint(tmp_path / "s.bin")
+    b = L.Lattice(*n, periodic=(1, 1, 1), tau=0.7, slices_per_dim=5)
+    b.load_checkpoint(tmp_path / "s.bin")
+    assert np.array_equal(a.get_populations(), b.get_populations())
+    ra, rb = a.cell_records(), b.cell_records()
+    for k in ("sf", "id", "us"):
+        assert np.array_equal(ra[k], rb[k])
+    # both continue with the same particles: identical populations and forces
+    for lat in (a, b):
+        lat.set_spheres(x, v, w, r, ids)
+        lat.collide_stream(2)
+    assert np.array_equal(a.get_populations(), b.get_populations())
+    Fa, Ta = a.sum_force_torque(x)
+    Fb, Tb = b.sum_force_torque(x)
+    assert np.array_equal(Fa, Fb) and np.array_equal(Ta, Tb)
+    # the reference's writeVTK fields of a lattice
+    from test_io import read_vti
+    a.write_vti(tmp_path / "a.vti", spacing=2.0, vel_scale=3.0)
+    nn, sp, got = read_vti(tmp_path / "a.vti")
+    rho, u = a.macroscopic()
+    sf = a.cell_records()["sf"]
+    c = (4 * n[1] + 5) * n[2] + 6
+    assert nn == n and sp == 2.0
+    assert got["density"][6, 5, 4, 0] == np.float32(rho[c])
+    assert got["pressure"][6, 5, 4, 0] == np.float32(rho[c] - 1.0)
+    assert np.array_equal(got["velocity"][6, 5, 4], (3.0 * u[:, c]).astype(np.float32))
+    assert got["solidfraction"][6, 5, 4, 0] == np.float32(sf[c])
