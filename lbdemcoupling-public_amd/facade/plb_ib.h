@@ -214,6 +214,9 @@ public:
     void setOmega(T om) { omega = om; }
     T getOmega() const { return omega; }
 
+    /* Creates the device lattice (populations at rest, rho = 1) on first
+     * use; later calls keep the populations (an initial field set by
+     * initializeAtEquilibrium survives, as in Palabos). */
     void initialize() {
         if (!h) {
             desc.nx = (int32_t)nx; desc.ny = (int32_t)ny; desc.nz = (int32_t)nz;
@@ -221,7 +224,6 @@ public:
             desc.tau = 1.0 / (double)omega;
             psmCheck(lbdem_psm_create(&desc, &h), nullptr);
         }
-        psmCheck(lbdem_psm_init_equilibrium(h, nullptr, nullptr), h);
     }
     void collideAndStream() { psmCheck(lbdem_psm_collide_stream(handle(), 1), h); }
 
@@ -261,6 +263,72 @@ private:
     bool qInitVel = false;
     int64_t nSet = 0;
 };
+
+/* PoiseuilleProfileAndPressureGradient (src/periodicPressureFunctionals3D.h,
+ * .hh:259-298) and initializeAtEquilibrium over the whole lattice: the field
+ * is written on the device (lbdem_psm_init_poiseuille). */
+template <typename T>
+struct PoiseuilleProfileAndPressureGradient {
+    PoiseuilleProfileAndPressureGradient(T pHi_, T pLo_, T uMax_, plint nx_, plint ny_, plint nz_, plint dim_)
+        : pHi(pHi_), pLo(pLo_), uMax(uMax_), dimension(dim_) { n[0] = (int32_t)nx_; n[1] = (int32_t)ny_; n[2] = (int32_t)nz_; }
+    T pHi, pLo, uMax;
+    int32_t n[3];
+    plint dimension;
+};
+
+template <typename T, template <typename U> class Descriptor>
+void initializeAtEquilibrium(MultiBlockLattice3D<T, Descriptor>& lattice, Box3D box,
+                             const PoiseuilleProfileAndPressureGradient<T>& f) {
+    (void)box;
+    psmCheck(lbdem_psm_init_poiseuille(lattice.handle(), (int32_t)f.dimension, f.pHi, f.pLo, f.uMax, f.n),
+             lattice.handle());
+}
+
+/* The reference's rectangular-channel driver computes the inlet/outlet mean
+ * densities before collideAndStream and applies two
+ * ZhangPeriodicPressureFunctional3D after it
+ * (examples/showcaseRectChannel/showcaseRectChannel.cpp:258-273).  Here the
+ * library does both around every collideAndStream once configured:
+ * the inlet box is the cross-section of the plane at index 0 along
+ * `dimension` (e.g. Box3D(0,0,1,ny-2,1,nz-2) for dimension 0). */
+template <typename T, template <typename U> class Descriptor>
+void setPeriodicPressure(MultiBlockLattice3D<T, Descriptor>& lattice, plint dimension, T rhoIn, T rhoOut,
+                         Box3D inlet) {
+    int32_t r[4];
+    const plint lo[3] = {inlet.x0, inlet.y0, inlet.z0}, hi[3] = {inlet.x1, inlet.y1, inlet.z1};
+    int k = 0;
+    for (int d = 0; d < 3; ++d) {
+        if (d == dimension) continue;
+        r[k++] = (int32_t)lo[d];
+        r[k++] = (int32_t)hi[d];
+    }
+    psmCheck(lbdem_psm_set_periodic_pressure(lattice.handle(), 1, (int32_t)dimension, rhoIn, rhoOut, r),
+             lattice.handle());
+}
+
+/* Palabos' saveBinaryBlock / loadBinaryBlock checkpoint of a lattice: the
+ * populations and the per-cell coupling record in the serialize order of
+ * IBcompositeDynamics (src/ibCompositeDynamics3D.hh:77-107). */
+template <typename T, template <typename U> class Descriptor>
+void saveBinaryBlock(MultiBlockLattice3D<T, Descriptor>& lattice, const std::string& fname) {
+    psmCheck(lbdem_psm_save_checkpoint(lattice.handle(), fname.c_str()), lattice.handle());
+}
+
+template <typename T, template <typename U> class Descriptor>
+void loadBinaryBlock(MultiBlockLattice3D<T, Descriptor>& lattice, const std::string& fname) {
+    psmCheck(lbdem_psm_load_checkpoint(lattice.handle(), fname.c_str()), lattice.handle());
+}
+
+/* writeVTK of the reference drivers (examples/benchmark/benchmark.cpp:74-99):
+ * velocity, density, pressure (rho - 1) * pFact and solid fraction as VTK
+ * ImageData in physical units. */
+template <typename T, template <typename U> class Descriptor>
+void writeVTK(MultiBlockLattice3D<T, Descriptor>& lattice, PhysUnits3D<T> const& units, const std::string& fname) {
+    const T pFact = units.getPhysForce(1) / (units.getPhysLength(1) * units.getPhysLength(1)) / 3.;
+    psmCheck(lbdem_psm_write_vti(lattice.handle(), fname.c_str(), units.getPhysLength(1), units.getPhysVel(1), 1.0,
+                                 pFact),
+             lattice.handle());
+}
 
 /* defineDynamics(lattice, box, new IBcompositeDynamics(new BGKdynamics(omega))):
  * every cell carries the same dynamics; only omega is kept. */
