@@ -7,9 +7,11 @@ velocities N(0, 0.01) and angular velocities N(0, 1e-3) per component,
 8^3 sub-cell sampling, prescribed kinematics (x += v each step), FP64.
 One step = voxelize + fused collide/stream + force/torque sum, all
 resident in HBM.  `value` is whole-step MLUPS of that FP64 run; the FP32
-deviation and FP16-storage variants of the same run, and the
-512x512x768 channel in FP32 deviation storage, are reported beside it with
-their drag error against FP64.
+deviation and FP16-storage variants of the same run are reported beside it
+with their drag, torque and velocity-field error against FP64, and so are
+the 512x512x768 channel (FP32 deviation and FP64), the 384^3 polydisperse
+suspension in all three precisions, and the differentiable 128x128x256
+channel (forward window and hand-written adjoint, FP64 and FP32).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -56,10 +58,11 @@ def log(msg):
 def make_spheres(wl, seed=2024):
     return L.generate_spheres(wl["n"], wl["count"], wl["law"], a=wl["a"], b=wl["b"], lo=wl["lo"],
                               hi=wl["hi"] if wl["hi"] > 0 else 1e30, periodic=wl["periodic"],
-                              gap=0.0, wall_gap=1.0, sigma_v=0.01, sigma_w=1e-3, seed=seed)
+                              gap=0.0, wall_gap=1.0, sigma_v=0.01, sigma_w=1e-3, seed=seed,
+                              max_attempts=10 ** 8)
 
 
-def run_gpu(wl, precision, steps, warmup, spheres, slices=8):
+def run_gpu(wl, precision, steps, warmup, spheres, slices=8, want_field=False):
     n = wl["n"]
     nc = int(np.prod(n))
     x, v, w, r, ids = spheres
@@ -80,6 +83,7 @@ def run_gpu(wl, precision, steps, warmup, spheres, slices=8):
         F, T = lat.particle_forces()
         st = lat.stats()
         rc, umax, nonfin = lat.check_health()
+        field = lat.macroscopic()[1] if want_field else None
     finally:
         lat.close()
     wall = t1 - t0
@@ -104,6 +108,7 @@ def run_gpu(wl, precision, steps, warmup, spheres, slices=8):
         "health_rc": rc,
         "F": F,
         "T": T,
+        "u": field,
     }
 
 
@@ -170,6 +175,14 @@ def measured_traffic(workload, kernel):
 def rel(a, b):
     den = np.abs(b).max()
     return float(np.abs(a - b).max() / den) if den > 0 else 0.0
+
+
+def field_err(u, ref):
+    """Relative L2 and Linf error of a velocity field against the FP64 one."""
+    if u is None or ref is None:
+        return None
+    d = u - ref
+    return {"l2": float(np.linalg.norm(d) / np.linalg.norm(ref)), "linf": float(np.abs(d).max() / np.abs(ref).max())}
 
 
 def cpu_baseline(wl, spheres, nsteps):
@@ -327,7 +340,7 @@ def main():
     res = {}
     for prec in ("fp64", "fp32_dev", "fp16_store"):
         log(f"{prec}: {args.warmup} warmup + {args.steps} timed steps")
-        res[prec] = run_gpu(wl, prec, args.steps, args.warmup, sph)
+        res[prec] = run_gpu(wl, prec, args.steps, args.warmup, sph, want_field=True)
         log(f"{prec}: {res[prec]['mlups']:.0f} MLUPS full step, kernel {res[prec]['kernel_gbs']:.0f} GB/s")
     f64 = res["fp64"]
 
@@ -339,7 +352,9 @@ def main():
                 "step_roofline_frac": round(rr["mlups"] * 1e6 * rr["bytes_per_lu"] / 1e9 / HBM_PEAK_GBS, 4),
                 "voxelize_ms": round(rr["voxelize_ms"], 4), "force_ms": round(rr["force_ms"], 4),
                 "drag_rel_err_vs_fp64": rel(rr["F"], ref["F"]),
-                "torque_rel_err_vs_fp64": rel(rr["T"], ref["T"])}
+                "torque_rel_err_vs_fp64": rel(rr["T"], ref["T"]),
+                "velocity_err_vs_fp64": field_err(rr.get("u"), ref.get("u")),
+                "max_u": rr["max_u"], "health_rc": rr["health_rc"]}
 
     extra = {}
     if not args.no_extra and args.workload == "dense256":
@@ -357,6 +372,26 @@ def main():
                                "fp64": {"mlups": round(r64["mlups"], 1),
                                         "kernel_gbs": round(r64["kernel_gbs"], 1)},
                                "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3)}
+
+    if not args.no_extra and args.workload == "dense256":
+        wl3 = WORKLOADS["poly384"]
+        log("poly384: generating 20000 spheres")
+        sph3 = make_spheres(wl3)
+        # 100 steps: with prescribed random kinematics this workload leaves
+        # the stable range (max |u| > 0.3) after ~200 steps in FP64 as well
+        steps3 = 100
+        r3 = {}
+        for prec in ("fp64", "fp32_dev", "fp16_store"):
+            r3[prec] = run_gpu(wl3, prec, steps3, 0, sph3, want_field=True)
+            log(f"poly384 {prec}: {r3[prec]['mlups']:.0f} MLUPS, kernel {r3[prec]['kernel_gbs']:.0f} GB/s")
+        extra["poly384"] = {"workload": wl3["desc"], "steps": steps3,
+                            "fp64": {"mlups": round(r3["fp64"]["mlups"], 1),
+                                     "kernel_gbs": round(r3["fp64"]["kernel_gbs"], 1),
+                                     "max_u": r3["fp64"]["max_u"], "health_rc": r3["fp64"]["health_rc"]},
+                            "fp32_dev": summary(r3["fp32_dev"], r3["fp64"]),
+                            "fp16_store": summary(r3["fp16_store"], r3["fp64"]),
+                            "fp32_over_fp64": round(r3["fp32_dev"]["mlups"] / r3["fp64"]["mlups"], 3)}
+        del r3
 
     if not args.no_adjoint:
         adj = {}
