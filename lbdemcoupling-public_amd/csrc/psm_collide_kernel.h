@@ -52,8 +52,7 @@ __device__ __forceinline__ T eq_rest(T rhoBar, T p, T jSqr) {
 /* BGK (with Guo forcing when FORCE) of all 19 populations of one cell, in
  * the operation order of the base dynamics. */
 template <typename T, bool FORCE>
-__device__ __forceinline__ void bgk_all(const T* f, T* fb, T omega, const T* acc) {
-    T rhoBar, j[3];
+__device__ __forceinline__ void bgk_all(const T* f, T* fb, T omega, const T* acc, T& rhoBar, T* j) {
     moments<T>(f, rhoBar, j);
     const T rho = rhoBar + T(1);
     const T invRho = recip<T>(rho);
@@ -90,32 +89,79 @@ __device__ __forceinline__ void bgk_all(const T* f, T* fb, T omega, const T* acc
     }
 }
 
-/* Push of one cell's post-collision populations.  x, y are wave-uniform:
- * per direction the destination row and the x/y wall test are uniform
- * (scalar), only the z offset is per lane. */
+/* Push of one cell's post-collision populations.  x, y are wave-uniform, so
+ * the nine destination rows (one per (cx, cy)) with their periodic wrap or
+ * wall test are set up once, in scalar registers; per direction the push
+ * adds the population's stretch offset and one of the lane's three z
+ * offsets.  A population leaving through a wall bounces back into its own
+ * cell as the opposite population. */
 template <typename S, bool PZ>
 struct Pusher {
-    S* fout;
-    PopLayout L;
-    int x, y, nx, ny, nz, px, py;
-    __device__ __forceinline__ void put(int i, int z, S v) const {
-        int xd = x + cx(i), yd = y + cy(i);
-        bool wall = false;
-        if (xd < 0 || xd >= nx) { if (px) xd = xd < 0 ? xd + nx : xd - nx; else wall = true; }
-        if (yd < 0 || yd >= ny) { if (py) yd = yd < 0 ? yd + ny : yd - ny; else wall = true; }
-        int zd = z + cz(i);
-        bool zwall = false;
-        if (cz(i) != 0) {
-            if (PZ) zd = zd < 0 ? zd + nz : (zd >= nz ? zd - nz : zd);
-            else zwall = zd < 0 || zd >= nz;
+    char* out;         /* fout as bytes */
+    int64_t row[9];    /* byte offset of the destination row of class (cx+1)*3 + (cy+1) */
+    bool wall[9];      /* that row lies beyond a non periodic x or y boundary */
+    int64_t qb;        /* bytes of one population stretch (zc cells) */
+    uint32_t zo[3];    /* lane's byte offset of z-1, z, z+1 inside a row */
+    bool zin[3];       /* lane: z-1, z+1 inside the domain (always, for periodic z) */
+
+    __device__ __forceinline__ Pusher(S* fout, const PopLayout& L, int x, int y, int z, int nx, int ny,
+                                      int nz, int px, int py) {
+        out = (char*)fout;
+        const int64_t rowbytes = L.rowlen() * (int64_t)sizeof(S);
+        int64_t xo[3], yo[3];
+        bool xw[3], yw[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int xd = x + d - 1, yd = y + d - 1;
+            xw[d] = yw[d] = false;
+            if (xd < 0 || xd >= nx) {
+                if (px) xd = xd < 0 ? xd + nx : xd - nx;
+                else { xw[d] = true; xd = x; }
+            }
+            if (yd < 0 || yd >= ny) {
+                if (py) yd = yd < 0 ? yd + ny : yd - ny;
+                else { yw[d] = true; yd = y; }
+            }
+            xo[d] = (int64_t)xd * ny * rowbytes;
+            yo[d] = (int64_t)yd * rowbytes;
         }
-        if (wall || zwall) {
-            char* own = (char*)(fout + pop_at((int64_t)x * ny + y, opp(i), 0, L));
-            *(S*)(own + L.zoff(z) * (uint32_t)sizeof(S)) = v;
-        } else {
-            char* dst = (char*)(fout + pop_at((int64_t)xd * ny + yd, i, 0, L));
-            *(S*)(dst + L.zoff(zd) * (uint32_t)sizeof(S)) = v;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            row[k] = xo[k / 3] + yo[k % 3];
+            wall[k] = xw[k / 3] || yw[k % 3];
         }
+        qb = (int64_t)L.zc * (int64_t)sizeof(S);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            int zd = z + d - 1;
+            zin[d] = true;
+            if (zd < 0 || zd >= nz) {
+                if (PZ) zd = zd < 0 ? zd + nz : zd - nz;
+                else { zin[d] = false; zd = z; }
+            }
+            zo[d] = L.zoff(zd) * (uint32_t)sizeof(S);
+        }
+    }
+    /* any wall test of this wave can succeed (x/y boundary row, or a z end
+     * chunk of a non periodic z) */
+    __device__ __forceinline__ bool near_wall(int chunk, int nzc) const {
+        bool w = false;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) w = w || wall[k];
+        if (!PZ) w = w || chunk == 0 || chunk == nzc - 1;
+        return w;
+    }
+    __device__ __forceinline__ void put(int i, S v) const {
+        const int k = (cx(i) + 1) * 3 + (cy(i) + 1), m = cz(i) + 1;
+        const int64_t back = row[4] + opp(i) * qb + zo[1];
+        const int64_t dst = wall[k] ? back : row[k] + i * qb + zo[m];
+        if (PZ || m == 1 || zin[m]) *(S*)(out + dst) = v;
+        else *(S*)(out + back) = v;
+    }
+    /* no wall test can succeed: the destination row and the lane's z offset */
+    __device__ __forceinline__ void put_inner(int i, S v) const {
+        const int k = (cx(i) + 1) * 3 + (cy(i) + 1), m = cz(i) + 1;
+        *(S*)(out + (row[k] + i * qb) + zo[m]) = v;
     }
 };
 
@@ -144,8 +190,8 @@ struct EqCoef {
 };
 
 template <bool FORCE>
-__device__ __forceinline__ void bgk_all_fast(const float* f, float* fb, float omega, const float* acc) {
-    float rhoBar, j[3];
+__device__ __forceinline__ void bgk_all_fast(const float* f, float* fb, float omega, const float* acc,
+                                             float& rhoBar, float* j) {
     moments<float>(f, rhoBar, j);
     const float rho = rhoBar + 1.0f;
     const float invRho = recip<float>(rho);
@@ -185,13 +231,19 @@ __device__ __forceinline__ void bgk_all_fast(const float* f, float* fb, float om
 
 /* The solid blend in FP32 (same structure as the FP64 path). */
 __device__ __forceinline__ void psm_blend_fast(const float* fp, float* fo, bool full, float B,
-                                               const float* us, float* F) {
+                                               const float* us, float rhoP, const float* jP, float* F) {
     const float oneMinB = 1.0f - B;
-    float post[Q];
-#pragma unroll
-    for (int i = 0; i < Q; ++i) post[i] = full ? fp[i] : fo[i];
+    /* moments of the cell the blend sees: the pre-collision cell (already
+     * summed by the collision) when full, the post-collision cell otherwise.
+     * With B = 1 the (1 - B) terms below vanish, so fo serves both cases. */
     float rhoB, jB[3];
-    moments<float>(post, rhoB, jB);
+    moments<float>(fo, rhoB, jB);
+    if (full) {
+        rhoB = rhoP;
+        jB[0] = jP[0];
+        jB[1] = jP[1];
+        jB[2] = jP[2];
+    }
     const float invRho = recip<float>(rhoB + 1.0f);
     const float jSqr = norm_sqr<float>(jB);
     const float p = 1.5f * invRho;
@@ -202,7 +254,7 @@ __device__ __forceinline__ void psm_blend_fast(const float* fp, float* fo, bool 
     ce.init(1.0f, rhoB, p, jSqr);
     cs.init(1.0f, rhoB, p, uPSqr);
     F[0] = F[1] = F[2] = 0.0f;
-    fo[0] = fmaf(oneMinB, post[0] - fp[0], fp[0]) + B * (ce.base0 - cs.base0);
+    fo[0] = fmaf(oneMinB, fo[0] - fp[0], fp[0]) + B * (ce.base0 - cs.base0);
 #pragma unroll
     for (int i = 1; i <= 9; ++i) {
         const int o = i + 9;
@@ -211,8 +263,8 @@ __device__ __forceinline__ void psm_blend_fast(const float* fp, float* fo, bool 
         cs.pair(i, cdot<float>(i, uP), Si, So);
         const float bb = B * ((fp[o] - Eo) - (fp[i] - Si));
         const float bbo = B * ((fp[i] - Ei) - (fp[o] - So));
-        fo[i] = fmaf(oneMinB, post[i] - fp[i], fp[i]) + bb;
-        fo[o] = fmaf(oneMinB, post[o] - fp[o], fp[o]) + bbo;
+        fo[i] = fmaf(oneMinB, fo[i] - fp[i], fp[i]) + bb;
+        fo[o] = fmaf(oneMinB, fo[o] - fp[o], fp[o]) + bbo;
         const float diff = bb - bbo;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -243,51 +295,75 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     const int y = __builtin_amdgcn_readfirstlane(row - x * a.ny);
     const T omega = T(a.omega);
     const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
-    Pusher<S, PZ> push{fout, a.L, x, y, a.nx, a.ny, a.nz, a.px, a.py};
+    Pusher<S, PZ> push(fout, a.L, x, y, z, a.nx, a.ny, a.nz, a.px, a.py);
     /* this workgroup's (row, chunk) block: 19 contiguous stretches of zc */
     const char* frow = (const char*)(fin + pop_at(row, 0, chunk * a.L.zc, a.L));
     const size_t qstride = (size_t)a.L.zc * sizeof(S);
 
     const int64_t c = rec_index(x, y, z, a.ny, a.nz);
-    const bool solid = a.sflag[c] != 0;
-    const int64_t s = solid ? c : 0;
     const uint32_t zoff = (uint32_t)zl * (uint32_t)sizeof(S);
+    /* Issue order: the flag byte, the 19 populations, then the cell's record.
+     * The flag is back before the populations (loads return in order), so
+     * the record loads go out without a stall of their own and are in flight
+     * during the BGK collision; waiting on the flag before the populations
+     * would add a full memory round trip to every wave. */
+    const uint8_t flag = a.sflag[c];
+    S raw[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) raw[i] = *(const S*)(frow + i * qstride + zoff);
+    __builtin_amdgcn_sched_barrier(0);
+    const bool solid = flag != 0;
+    /* fluid lanes read record 0 (one shared line); their values are unused */
+    const int64_t s = solid ? c : 0;
+    const double rsf = a.rsf[s];
+    const double rus[3] = {a.rus[s], a.rus[a.rcap + s], a.rus[2 * a.rcap + s]};
+    __builtin_amdgcn_sched_barrier(0);
     T fp[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) fp[i] = (T)(*(const S*)(frow + i * qstride + zoff));
+    for (int i = 0; i < Q; ++i) fp[i] = (T)raw[i];
     T fo[Q];
+    double Fc[3] = {0.0, 0.0, 0.0};
     if constexpr (sizeof(T) == 4) {
-        bgk_all_fast<FORCE>(fp, fo, omega, acc);
+        float rhoP, jP[3];
+        bgk_all_fast<FORCE>(fp, fo, omega, acc, rhoP, jP);
         if (solid) {
-            const float sf = (float)a.rsf[s];
-            const bool full = a.rsf[s] > SOLFRAC_MAX;
+            const float sf = (float)rsf;
+            const bool full = rsf > SOLFRAC_MAX;
             const float ooo = (float)a.ooo;
             const float B = full ? 1.0f : (sf * ooo) / ((1.0f - sf) + ooo);
-            const float us[3] = {(float)a.rus[s], (float)a.rus[a.rcap + s], (float)a.rus[2 * a.rcap + s]};
+            const float us[3] = {(float)rus[0], (float)rus[1], (float)rus[2]};
             float F[3];
-            psm_blend_fast(fp, fo, full, B, us, F);
-            a.rF[s] = double(F[0]);
-            a.rF[a.rcap + s] = double(F[1]);
-            a.rF[2 * a.rcap + s] = double(F[2]);
+            psm_blend_fast(fp, fo, full, B, us, rhoP, jP, F);
+            Fc[0] = double(F[0]);
+            Fc[1] = double(F[1]);
+            Fc[2] = double(F[2]);
         }
     } else {
-        bgk_all<T, FORCE>(fp, fo, omega, acc);
+        T rhoP, jP[3];
+        bgk_all<T, FORCE>(fp, fo, omega, acc, rhoP, jP);
         if (solid) {
             /* IBcompositeDynamics::collide (src/ibCompositeDynamics3D.hh:130-203).
              * sf > 0.999 runs as B = 1 on the pre-collision cell: with B = 1
              * and fB = fP the blend (fP + (1-B)(fB - fP)) + B*Omega reduces
              * exactly to fP + Omega, the reference's full-solid branch. */
-            const T sf = T(a.rsf[s]);
-            const bool full = a.rsf[s] > SOLFRAC_MAX;
+            const T sf = T(rsf);
+            const bool full = rsf > SOLFRAC_MAX;
             const T ooo = T(a.ooo);
             const T B = full ? T(1) : (sf * ooo) / ((T(1) - sf) + ooo);
             const T oneMinB = T(1) - B;
-            const T us[3] = {T(a.rus[s]), T(a.rus[a.rcap + s]), T(a.rus[2 * a.rcap + s])};
-            T post[Q];
-#pragma unroll
-            for (int i = 0; i < Q; ++i) post[i] = full ? fp[i] : fo[i];
+            const T us[3] = {T(rus[0]), T(rus[1]), T(rus[2])};
+            /* the reference sums the moments of the cell it blends: the
+             * pre-collision cell when full (the collision summed those
+             * already, in the same order), the post-collision cell otherwise.
+             * With B = 1 the (1 - B) terms below vanish, so fo serves both. */
             T rhoB, jB[3];
-            moments<T>(post, rhoB, jB);
+            moments<T>(fo, rhoB, jB);
+            if (full) {
+                rhoB = rhoP;
+                jB[0] = jP[0];
+                jB[1] = jP[1];
+                jB[2] = jP[2];
+            }
             const T invRho = recip<T>(rhoB + T(1));
             const T jSqr = norm_sqr<T>(jB);
             const T p = T(1.5) * invRho;
@@ -299,7 +375,7 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
             T F[3] = {T(0), T(0), T(0)};
             {
                 const T bb0 = eq_rest<T>(rhoB, p, jSqr) - eq_rest<T>(rhoB, p, uPSqr);
-                fo[0] = (fp[0] + oneMinB * (post[0] - fp[0])) + B * bb0;
+                fo[0] = (fp[0] + oneMinB * (fo[0] - fp[0])) + B * bb0;
             }
 #pragma unroll
             for (int i = 1; i <= 9; ++i) {
@@ -309,8 +385,8 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
                 eq_pair<T>(i, rhoB, p, uP, uPSqr, Si, So);
                 const T bb = B * ((fp[o] - Eo) - (fp[i] - Si));
                 const T bbo = B * ((fp[i] - Ei) - (fp[o] - So));
-                fo[i] = (fp[i] + oneMinB * (post[i] - fp[i])) + bb;
-                fo[o] = (fp[o] + oneMinB * (post[o] - fp[o])) + bbo;
+                fo[i] = (fp[i] + oneMinB * (fo[i] - fp[i])) + bb;
+                fo[o] = (fp[o] + oneMinB * (fo[o] - fp[o])) + bbo;
                 const T diff = bb - bbo;
 #pragma unroll
                 for (int d = 0; d < 3; ++d) {
@@ -318,13 +394,28 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
                     else if (cc(i, d) == -1) F[d] = F[d] + diff;
                 }
             }
-            a.rF[s] = double(F[0]);
-            a.rF[a.rcap + s] = double(F[1]);
-            a.rF[2 * a.rcap + s] = double(F[2]);
+            Fc[0] = double(F[0]);
+            Fc[1] = double(F[1]);
+            Fc[2] = double(F[2]);
         }
     }
+    /* A wave holding a solid cell writes the force of every cell it holds,
+     * zero for the fluid ones (the reference resets the force of every cell
+     * in each collision, src/ibCompositeDynamics3D.hh:137): whole lines
+     * instead of the partial-line writes of the solid cells alone, which
+     * cost a read-modify-write each.  Readers look at solid cells only. */
+    if (__builtin_amdgcn_ballot_w64(solid) != 0) {
+        a.rF[c] = Fc[0];
+        a.rF[a.rcap + c] = Fc[1];
+        a.rF[2 * a.rcap + c] = Fc[2];
+    }
+    if (push.near_wall(chunk, nzc)) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) push.put(i, z, (S)fo[i]);
+        for (int i = 0; i < Q; ++i) push.put(i, (S)fo[i]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) push.put_inner(i, (S)fo[i]);
+    }
 }
 
 /* Launches one collide+stream from fin into fout (row-blocked stores of the
