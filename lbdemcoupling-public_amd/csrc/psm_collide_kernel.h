@@ -284,8 +284,9 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     typedef typename Store<PREC>::T T;
     typedef typename Store<PREC>::S S;
     const int nzc = a.L.nzc;
-    const int row = blockIdx.x / nzc;
-    const int chunk = blockIdx.x - row * nzc;
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int row = blk / nzc;
+    const int chunk = blk - row * nzc;
     const int zl = threadIdx.x;
     const int z = chunk * a.L.zc + zl;
     if (z >= a.nz) return;

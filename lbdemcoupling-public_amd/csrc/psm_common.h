@@ -300,6 +300,18 @@ __host__ __device__ inline void rec_cell(int64_t r, int ny, int nz, int& x, int&
  * neighbouring rows (and, across a chunk boundary, of the next chunk).
  * zc is a multiple of 64; with more than one chunk it is a power of two
  * (zsh = log2 zc). */
+/* Workgroups are handed to the 8 XCDs round-robin (workgroup i to XCD
+ * i % 8), each XCD with its own L2.  Renumbering them so that XCD k sweeps
+ * one contiguous range of the grid keeps neighbouring blocks (the z chunks
+ * of a row, adjacent rows) in one L2: a line that two neighbouring blocks
+ * both write is then assembled in that L2 instead of leaving two partial
+ * lines in two of them.  A bijection of [0, G). */
+__device__ __forceinline__ int xcd_block(int i, int G) {
+    const int xcd = i & 7, local = i >> 3;
+    const int per = G >> 3, rem = G & 7;
+    return xcd * per + (xcd < rem ? xcd : rem) + local;
+}
+
 struct PopLayout {
     int zc;
     int zsh;
