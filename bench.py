@@ -214,15 +214,27 @@ TRAFFIC_JSON = os.path.join(ROOT, "profiles", "collide_stream_traffic.json")
 PROBE_CMD = ("rocprofv3 --pmc {ctr} --output-format csv -d DIR/{tag} -o run -- "
              "python3 bench.py --kernel-probe")
 TRACE_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace -o run -- "
-             "python3 bench.py --steps 10 --warmup 2 --no-extra --cpu-steps 1")
+             "python3 bench.py --steps 10 --warmup 2 --no-extra --no-adjoint --cpu-steps 1")
+TRACE512_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace512 -o run -- "
+                "python3 bench.py --workload channel512 --steps 4 --warmup 1 --no-adjoint --cpu-steps 0")
 
 
 def _short(name):
-    for key, short in (("k_collide_stream<0", "k_collide_stream<FP64>"), ("k_collide_stream<1", "k_collide_stream<FP32_DEV>"),
-                       ("k_collide_stream<2", "k_collide_stream<FP16_STORE>"), ("k_pop_convert<0", "k_pop_convert<FP64> (copy)")):
-        if key in name:
-            return short
-    return name.split("(")[0].replace("void ", "").replace("psm::", "")[:48]
+    """k_collide_stream<PREC, FORCE, PZ> -> k_collide_stream<FP64> (+ body force / z walls noted)."""
+    base = name.split("(")[0].replace("void ", "").replace("psm::", "")
+    precs = {"0": "FP64", "1": "FP32_DEV", "2": "FP16_STORE"}
+    if base.startswith("k_collide_stream<"):
+        args = [t.strip() for t in base[len("k_collide_stream<"):].rstrip(">").split(",")]
+        tag = precs.get(args[0], args[0])
+        extra = []
+        if len(args) > 1 and args[1] == "true":
+            extra.append("body force")
+        if len(args) > 2 and args[2] == "false":
+            extra.append("z walls")
+        return f"k_collide_stream<{tag}>" + (f" ({', '.join(extra)})" if extra else "")
+    if base.startswith("k_pop_convert<0"):
+        return "k_pop_convert<FP64> (copy)"
+    return base[:48]
 
 
 def summarize_profiles(d):
@@ -231,14 +243,22 @@ def summarize_profiles(d):
     import glob
     import sqlite3
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
-    dbs = glob.glob(os.path.join(d, "trace", "*results.db"))
-    if dbs:
+    for sub, fname, title, cmd, what in (
+            ("trace", "kernels_dense256_trace.md",
+             "dense suspension 256^3 (2000 spheres r~U[4,8], tau 0.7, 8^3 samples)", TRACE_CMD,
+             "The run does 1 + 2 + 10 coupled steps per precision (FP64, FP32-dev, FP16-store)."),
+            ("trace512", "kernels_channel512_trace.md",
+             "channel 512x512x768 (100000 spheres r~U[3,6], walls x/y, tau 0.55, 8^3 samples)", TRACE512_CMD,
+             "The run does 1 + 1 + 4 coupled steps per precision (FP64, FP32-dev, FP16-store).")):
+        dbs = glob.glob(os.path.join(d, sub, "*results.db"))
+        if not dbs:
+            continue
         con = sqlite3.connect(dbs[0])
         rows = list(con.execute("select name,total_calls,total_duration,average,percentage from top_kernels"))
-        with open(os.path.join(ROOT, "profiles", "kernels_dense256_trace.md"), "w") as f:
-            f.write("# Kernel durations, dense suspension 256^3 (2000 spheres r~U[4,8], tau 0.7, 8^3 samples)\n\n")
-            f.write("Command: `" + TRACE_CMD + "` (rocprofv3 kernel trace, ROCm 7.2, one MI355X).\n")
-            f.write("The run does 1 + 2 + 10 coupled steps per precision (FP64, FP32-dev, FP16-store).\n\n")
+        with open(os.path.join(ROOT, "profiles", fname), "w") as f:
+            f.write(f"# Kernel durations, {title}\n\n")
+            f.write("Command: `" + cmd + "` (rocprofv3 kernel trace, ROCm 7.2, one MI355X).\n")
+            f.write(what + "\n\n")
             f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
             for name, n, tot, avg, pct in rows[:24]:
                 f.write(f"| {_short(name)} | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |\n")
@@ -403,9 +423,11 @@ def main():
         extra["adjoint"] = {"workload": ADJOINT_CASE["desc"], "fp64": adj["fp64"], "fp32_dev": adj["fp32_dev"],
                             "fp32_grad_rel_err_vs_fp64": {k: rel(g32[k], g64[k]) for k in ("v", "w", "x", "sf")}}
 
-    log(f"cpu baseline: {args.cpu_steps} oracle steps")
-    cpu = cpu_baseline(wl, sph, args.cpu_steps)
-    log(f"cpu baseline: {cpu['value']:.1f} MLUPS")
+    cpu = None
+    if args.cpu_steps > 0:
+        log(f"cpu baseline: {args.cpu_steps} oracle steps")
+        cpu = cpu_baseline(wl, sph, args.cpu_steps)
+        log(f"cpu baseline: {cpu['value']:.1f} MLUPS")
 
     out = {
         "metric": METRIC,
