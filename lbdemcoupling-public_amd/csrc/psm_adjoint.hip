@@ -421,8 +421,9 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
     typedef typename Store<PREC>::T T;
     __shared__ double sh[4];
     const int nzc = a.L.nzc;
-    const int row = blockIdx.x / nzc;
-    const int chunk = blockIdx.x - row * nzc;
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int row = blk / nzc;
+    const int chunk = blk - row * nzc;
     const int z = chunk * a.L.zc + threadIdx.x;
     const int x = __builtin_amdgcn_readfirstlane(row / a.ny);
     const int y = __builtin_amdgcn_readfirstlane(row - x * a.ny);
