@@ -42,9 +42,11 @@ WORKLOADS = {
                     law=L.RADIUS_LOGNORMAL, a=5.0, b=0.25, lo=3.0, hi=10.0, body=(0.0, 0.0, 0.0),
                     desc="periodic polydisperse 384^3, tau 0.6, 20000 spheres lognormal(5,0.25) in [3,10]"),
     "channel512": dict(n=(512, 512, 768), periodic=(0, 0, 1), tau=0.55, count=100000,
-                       law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 1e-6),
-                       desc="channel 512x512x768, walls x/y, body force 1e-6 along z, tau 0.55, "
-                            "100000 spheres r~U[3,6]"),
+                       law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
+                       pressure=dict(dim=2, rho_hi=1.0, rho_lo=1.0 - 1e-4, u_max=0.05),
+                       desc="channel 512x512x768, walls x/y, pressure-driven along z (Zhang-Kwok "
+                            "periodic pressure rho 1 -> 1-1e-4, Poiseuille initial field u_max 0.05), "
+                            "tau 0.55, 100000 spheres r~U[3,6]"),
     "cube512": dict(n=(512, 512, 512), periodic=(1, 1, 1), tau=0.55, count=66667,
                     law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
                     desc="periodic 512^3, tau 0.55, 66667 spheres r~U[3,6] (channel512 density)"),
@@ -69,6 +71,10 @@ def run_gpu(wl, precision, steps, warmup, spheres, slices=8, want_field=False):
     lat = L.Lattice(*n, periodic=wl["periodic"], tau=wl["tau"], precision=precision,
                     slices_per_dim=slices, body_accel=wl["body"])
     try:
+        pr = wl.get("pressure")
+        if pr:
+            lat.init_poiseuille(pr["dim"], pr["rho_hi"], pr["rho_lo"], pr["u_max"])
+            lat.set_periodic_pressure(pr["dim"], pr["rho_hi"], pr["rho_lo"])
         lat.set_spheres(x, v, w, r, ids)
         lat.step(warmup, advance=True)
         lat.synchronize()
@@ -381,10 +387,10 @@ def main():
         wl5 = WORKLOADS["channel512"]
         log("channel512: generating 100000 spheres")
         sph5 = make_spheres(wl5)
-        steps5 = max(10, args.steps // 4)
-        r32 = run_gpu(wl5, "fp32_dev", steps5, max(2, args.warmup // 2), sph5)
+        steps5 = max(10, min(200, 2 * args.steps))
+        r32 = run_gpu(wl5, "fp32_dev", steps5, max(2, args.warmup // 2), sph5, want_field=True)
         log(f"channel512 fp32_dev: {r32['mlups']:.0f} MLUPS, kernel {r32['kernel_gbs']:.0f} GB/s")
-        r64 = run_gpu(wl5, "fp64", steps5, max(2, args.warmup // 2), sph5)
+        r64 = run_gpu(wl5, "fp64", steps5, max(2, args.warmup // 2), sph5, want_field=True)
         log(f"channel512 fp64: {r64['mlups']:.0f} MLUPS")
         e32 = summary(r32, r64)
         extra["channel512"] = {"workload": wl5["desc"], "steps": steps5,

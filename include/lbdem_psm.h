@@ -237,7 +237,14 @@ int lbdem_psm_clear_records(lbdem_psm* h);
 int lbdem_psm_perturb_solid_fraction(lbdem_psm* h, const double* dsf);
 /* Runs the window from the current populations; ft (may be NULL):
  * nsteps*n*6 doubles, (F, T) per particle per step, folded onto map(id)
- * like sum_force_torque.  Leaves the lattice at step nsteps. */
+ * like sum_force_torque.  Leaves the lattice at step nsteps.
+ * checkpoint_every > 0: a population checkpoint every that many steps, the
+ * backward pass holding ceil(nsteps/K) + K - 1 states.  checkpoint_every =
+ * -M < 0: a budget of M states in all (segments of M, M-1, ... steps, each
+ * finished segment's checkpoint reused for recompute; needs M(M+1)/2 >=
+ * nsteps), for windows too large for the first form; such a window allows
+ * one diff_backward.  Every step is recomputed once in both forms, and the
+ * gradients are bit-identical. */
 int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t checkpoint_every, double* ft);
 /* Adjoint of the last diff_forward.  g_ft: nsteps*n*6 (may be NULL = 0);
  * g_fT: 19*Nc host populations layout (may be NULL = 0).  Outputs (each may

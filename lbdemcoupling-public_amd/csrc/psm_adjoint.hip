@@ -721,6 +721,11 @@ static void free_states(std::vector<void*>& v) {
     v.clear();
 }
 
+static size_t state_elem_bytes(const lbdem_psm* h) {
+    return h->precision == LBDEM_PSM_FP64 ? sizeof(double)
+                                          : (h->precision == LBDEM_PSM_FP32_DEV ? sizeof(float) : sizeof(_Float16));
+}
+
 int diff_free(lbdem_psm* h) {
     auto& D = h->diff;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
@@ -731,6 +736,8 @@ int diff_free(lbdem_psm* h) {
     dfree_d(D.d_ft); dfree_d(D.gacc); dfree_d(D.Fscr); dfree_d(D.opart); dfree_d(D.gpart);
     dfree_d(D.cell_out); dfree_d(D.tau_part); dfree_d(D.pb); dfree_d(D.pc);
     dfree_d(D.rec_list); dfree_d(D.d_nrec);
+    dfree_d(D.pp_last); dfree_d(D.pp_next); dfree_d(D.pp_cur);
+    D.pp_last = D.pp_next = D.pp_cur = nullptr;
     D.pb = D.pc = nullptr;
     D.rec_list = nullptr;
     D.d_nrec = nullptr;
@@ -857,20 +864,57 @@ int lbdem_psm_perturb_solid_fraction(lbdem_psm* h, const double* dsf) {
     return LBDEM_PSM_OK;
 }
 
+/* Segment start steps of a window of T steps.  K > 0: a checkpoint every K
+ * steps.  K < 0: a budget of M = -K population states for checkpoints and
+ * recomputed states together, with segments of M, M-1, M-2, ... steps (the
+ * last one cut to what is left).  The backward pass handles segment j
+ * (0-based, from the last) while the checkpoints of segments 0..j are alive
+ * and recomputes len_j - 1 = M - j - 1 states, so at most (j + 1) + (M - j - 1)
+ * = M states are held: the checkpoint of each finished segment becomes
+ * recompute storage.  Every step is recomputed once, as with K > 0.  Needs
+ * M (M + 1) / 2 >= T. */
+static bool window_segments(int T, int K, std::vector<int32_t>& t0) {
+    t0.clear();
+    if (K > 0) {
+        for (int t = 0; t < T; t += K) t0.push_back(t);
+        return true;
+    }
+    const int64_t M = -(int64_t)K;
+    if (M < 1 || M * (M + 1) / 2 < T) return false;
+    int t = 0;
+    for (int len = (int)M; t < T; --len) {
+        t0.push_back(t);
+        t += len;
+    }
+    return true;
+}
+
 int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) {
-    if (!h || nsteps < 1 || K < 1) return LBDEM_PSM_ERR_INVALID_ARG;
+    if (!h || nsteps < 1 || K == 0) return LBDEM_PSM_ERR_INVALID_ARG;
     if (h->precision == LBDEM_PSM_FP16_STORE)
         return set_err(h, LBDEM_PSM_ERR_STATE, "the differentiable window supports FP64 and FP32_DEV storage");
     if (!h->have_particles) return set_err(h, LBDEM_PSM_ERR_STATE, "diff_forward before set_spheres");
-    if (h->pp.enabled)
-        return set_err(h, LBDEM_PSM_ERR_STATE, "the differentiable window does not include the periodic-pressure boundary");
     auto& D = h->diff;
     D.ready = false;
+    std::vector<int32_t> t0;
+    if (!window_segments(nsteps, K, t0))
+        return set_err(h, LBDEM_PSM_ERR_INVALID_ARG,
+                       "a state budget M = -checkpoint_every needs M (M + 1) / 2 >= nsteps");
     PSM_HIP(h, hipStreamSynchronize(h->stream));
     const int64_t np = h->np;
-    const int ncp = (nsteps + K - 1) / K;
+    const int ncp = (int)t0.size();
+    /* checkpoints consumed by a budgeted backward pass were moved to the
+     * recompute pool; drop the empty entries */
+    D.ckpt.erase(std::remove(D.ckpt.begin(), D.ckpt.end(), nullptr), D.ckpt.end());
     int rc = ensure_states(h, D.ckpt, (size_t)ncp);
     if (rc) return rc;
+    if (K < 0) {
+        /* budgeted: the recompute pool starts empty, so that checkpoints
+         * plus pool never exceed the budget */
+        for (void* p : D.seg) dfree_d(p);
+        D.seg.clear();
+    }
+    D.seg_t0 = t0;
     dfree_d(D.d_ft);
     D.d_ft = nullptr;
     PSM_HIP(h, dalloc_d(&D.d_ft, (size_t)nsteps * (size_t)std::max<int64_t>(np, 1) * 6 * sizeof(double)));
@@ -884,9 +928,12 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
                        h->bin_any[h->cur_any], h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2], D.rec_list,
                        D.d_nrec, h->rec_cap);
     PSM_HIP(h, hipGetLastError());
-    for (int t = 0; t < nsteps; ++t) {
-        if (t % K == 0)
-            PSM_HIP(h, hipMemcpyAsync(D.ckpt[t / K], h->f[h->cur_f], h->pop_bytes, hipMemcpyDeviceToDevice, h->stream));
+    for (int t = 0, nextc = 0; t < nsteps; ++t) {
+        if (nextc < ncp && t == t0[nextc]) {
+            PSM_HIP(h, hipMemcpyAsync(D.ckpt[nextc], h->f[h->cur_f], h->pop_bytes, hipMemcpyDeviceToDevice,
+                                      h->stream));
+            ++nextc;
+        }
         rc = launch_collide_stream(h);
         if (rc) return rc;
         if (np > 0) {
@@ -914,9 +961,20 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
             }
         }
     }
+    if (h->pp.enabled) {
+        /* the inlet/outlet planes of f(T), for the adjoint of the last step */
+        const size_t pbytes = (size_t)periodic_pressure_plane_elems(h) * state_elem_bytes(h);
+        dfree_d(D.pp_last);
+        D.pp_last = nullptr;
+        PSM_HIP(h, dalloc_d(&D.pp_last, pbytes));
+        rc = periodic_pressure_extract(h, h->f[h->cur_f], D.pp_last);
+        if (rc) return rc;
+        PSM_HIP(h, hipStreamSynchronize(h->stream));
+    }
     D.T = nsteps;
     D.K = K;
     D.np = np;
+    D.pp = h->pp.enabled;
     D.gen = h->geom_gen;
     D.ready = true;
     return LBDEM_PSM_OK;
@@ -927,13 +985,24 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
     if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
     auto& D = h->diff;
     if (!D.ready) return set_err(h, LBDEM_PSM_ERR_STATE, "diff_backward without a diff_forward");
-    if (D.gen != h->geom_gen || D.np != h->np)
-        return set_err(h, LBDEM_PSM_ERR_STATE, "records, particles or tau changed since diff_forward");
+    if (D.gen != h->geom_gen || D.np != h->np || D.pp != h->pp.enabled)
+        return set_err(h, LBDEM_PSM_ERR_STATE,
+                       "records, particles, tau or the periodic-pressure setting changed since diff_forward");
     const int T = D.T, K = D.K;
     const int64_t np = D.np, nc = h->g.nc, rcap = h->rec_cap;
+    const int nseg = (int)D.seg_t0.size();
+    auto seg_len = [&](int s) { return (s + 1 < nseg ? D.seg_t0[s + 1] : T) - D.seg_t0[s]; };
+    int maxlen = 0;
+    for (int s = 0; s < nseg; ++s) maxlen = std::max(maxlen, seg_len(s));
     PSM_HIP(h, hipStreamSynchronize(h->stream));
-    int rc = ensure_states(h, D.seg, (size_t)std::max(std::min(K, T) - 1, 0));
-    if (rc) return rc;
+    /* a budgeted schedule hands its checkpoints to the recompute pool as it
+     * goes: one backward pass per forward window */
+    if (K < 0) D.ready = false;
+    int rc = LBDEM_PSM_OK;
+    if (K > 0) {
+        rc = ensure_states(h, D.seg, (size_t)std::max(maxlen - 1, 0));
+        if (rc) return rc;
+    }
     for (int k = 0; k < 2; ++k)
         if (!D.lam[k]) PSM_HIP(h, dalloc_d(&D.lam[k], h->pop_bytes));
     if (!D.gacc) PSM_HIP(h, dalloc_d(&D.gacc, 7 * rcap * sizeof(double)));
@@ -975,11 +1044,30 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
     PSM_HIP(h, hipMemsetAsync(D.gacc, 0, 7 * rcap * sizeof(double), h->stream));
     PSM_HIP(h, hipMemsetAsync(D.opart, 0, D.opart_n * sizeof(double), h->stream));
 
-    const int nseg = (T + K - 1) / K;
-    std::vector<const void*> st((size_t)K);
+    const bool pp = D.pp;
+    if (pp) {
+        const size_t pbytes = (size_t)periodic_pressure_plane_elems(h) * state_elem_bytes(h);
+        dfree_d(D.pp_next);
+        dfree_d(D.pp_cur);
+        D.pp_next = D.pp_cur = nullptr;
+        PSM_HIP(h, dalloc_d(&D.pp_next, pbytes));
+        PSM_HIP(h, dalloc_d(&D.pp_cur, pbytes));
+        PSM_HIP(h, hipMemcpyAsync(D.pp_next, D.pp_last, pbytes, hipMemcpyDeviceToDevice, h->stream));
+    }
+    std::vector<const void*> st((size_t)std::max(maxlen, 1));
     for (int s = nseg - 1; s >= 0; --s) {
-        const int t0 = s * K;
-        const int len = std::min(K, T - t0);
+        const int t0 = D.seg_t0[s];
+        const int len = seg_len(s);
+        if (K < 0) {
+            /* budgeted: grow the recompute pool only as far as this
+             * segment needs (freed checkpoints were added to it) */
+            while ((int)D.seg.size() < len - 1) {
+                void* p = nullptr;
+                hipError_t e = dalloc_d(&p, h->pop_bytes);
+                if (e != hipSuccess) { dfree_d(tmp64); return hip_check(h, e, "hipMalloc (window states)"); }
+                D.seg.push_back(p);
+            }
+        }
         st[0] = D.ckpt[s];
         for (int j = 1; j < len; ++j) {
             rc = launch_collide_stream_buf(h, st[j - 1], D.seg[j - 1], D.Fscr);
@@ -988,10 +1076,37 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
         }
         for (int j = len - 1; j >= 0; --j) {
             const int t = t0 + j;
+            if (pp) {
+                /* f(t+1): the next segment's first state, or recomputed */
+                const void* gpl = D.pp_next;
+                if (j < len - 1) {
+                    rc = periodic_pressure_extract(h, st[j + 1], D.pp_cur);
+                    if (rc) { dfree_d(tmp64); return rc; }
+                    gpl = D.pp_cur;
+                }
+                rc = periodic_pressure_adjoint_pre(h, st[j], gpl, D.lam[cur]);
+                if (rc) { dfree_d(tmp64); return rc; }
+            }
             rc = adjoint_step(h, st[j], D.lam[cur], D.lam[1 - cur],
                               have_gft ? D.d_ft + (size_t)t * np * 6 : nullptr);
             if (rc) { dfree_d(tmp64); return rc; }
+            if (pp) {
+                rc = periodic_pressure_adjoint_post(h, D.lam[1 - cur]);
+                if (rc) { dfree_d(tmp64); return rc; }
+            }
             cur = 1 - cur;
+        }
+        if (pp && s > 0) {
+            /* f(t0) is the state after the last step of the segment before */
+            rc = periodic_pressure_extract(h, D.ckpt[s], D.pp_next);
+            if (rc) { dfree_d(tmp64); return rc; }
+        }
+        if (K < 0) {
+            /* this segment's checkpoint is no longer needed: it becomes
+             * recompute storage for the (longer) segment before it; the
+             * stream order keeps the adjoint steps above ahead of its reuse */
+            D.seg.push_back(D.ckpt[s]);
+            D.ckpt[s] = nullptr;
         }
     }
 

@@ -23,11 +23,14 @@
 namespace psm {
 
 /* Cell (i along the axis dim, a, b along the other two axes in increasing
- * order) -> (x, y, z). */
+ * order) -> (x, y, z).  Written as selects: the if / else-if form of the
+ * same mapping was compiled for dim = 2 into code that took y = i (the
+ * generated blocks for dim = 1 and dim = 2 shared the assignment of y),
+ * which the z-axis tests caught. */
 __device__ __forceinline__ void plane_cell(int dim, int i, int a, int b, int& x, int& y, int& z) {
-    if (dim == 0) { x = i; y = a; z = b; }
-    else if (dim == 1) { x = a; y = i; z = b; }
-    else { x = a; y = b; z = i; }
+    x = dim == 0 ? i : a;
+    y = dim == 0 ? a : (dim == 1 ? i : b);
+    z = dim == 2 ? i : b;
 }
 
 struct PlaneArgs {
@@ -126,9 +129,9 @@ static unsigned plane_blocks(const PlaneArgs& p) {
 }
 
 template <int PREC>
-static void pp_pre_t(lbdem_psm* h, const PlaneArgs& p) {
+static void pp_pre_t(lbdem_psm* h, const PlaneArgs& p, const void* fv) {
     typedef typename Store<PREC>::S S;
-    const S* f = (const S*)h->f[h->cur_f];
+    const S* f = (const S*)fv;
     const unsigned nb = plane_blocks(p);
     const int last = h->g.n[p.dim] - 1;
     /* [0] inlet sum, [1] outlet sum, partials from [8] on */
@@ -139,9 +142,9 @@ static void pp_pre_t(lbdem_psm* h, const PlaneArgs& p) {
 }
 
 template <int PREC>
-static void pp_post_t(lbdem_psm* h, const PlaneArgs& p) {
+static void pp_post_t(lbdem_psm* h, const PlaneArgs& p, void* fv) {
     typedef typename Store<PREC>::S S;
-    S* f = (S*)h->f[h->cur_f];
+    S* f = (S*)fv;
     const unsigned nb = plane_blocks(p);
     const int last = h->g.n[p.dim] - 1;
     /* inlet: populations arriving with c_d = +1, against the outlet's mean */
@@ -152,25 +155,236 @@ static void pp_post_t(lbdem_psm* h, const PlaneArgs& p) {
                        (const double*)(h->pp.buf));
 }
 
-int periodic_pressure_pre(lbdem_psm* h) {
+int periodic_pressure_pre_buf(lbdem_psm* h, const void* f) {
     if (!h->pp.enabled) return LBDEM_PSM_OK;
     const PlaneArgs p = plane_args(h);
     switch (h->precision) {
-        case LBDEM_PSM_FP64: pp_pre_t<LBDEM_PSM_FP64>(h, p); break;
-        case LBDEM_PSM_FP32_DEV: pp_pre_t<LBDEM_PSM_FP32_DEV>(h, p); break;
-        default: pp_pre_t<LBDEM_PSM_FP16_STORE>(h, p); break;
+        case LBDEM_PSM_FP64: pp_pre_t<LBDEM_PSM_FP64>(h, p, f); break;
+        case LBDEM_PSM_FP32_DEV: pp_pre_t<LBDEM_PSM_FP32_DEV>(h, p, f); break;
+        default: pp_pre_t<LBDEM_PSM_FP16_STORE>(h, p, f); break;
     }
     PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }
 
-int periodic_pressure_post(lbdem_psm* h) {
+int periodic_pressure_post_buf(lbdem_psm* h, void* f) {
     if (!h->pp.enabled) return LBDEM_PSM_OK;
     const PlaneArgs p = plane_args(h);
     switch (h->precision) {
-        case LBDEM_PSM_FP64: pp_post_t<LBDEM_PSM_FP64>(h, p); break;
-        case LBDEM_PSM_FP32_DEV: pp_post_t<LBDEM_PSM_FP32_DEV>(h, p); break;
-        default: pp_post_t<LBDEM_PSM_FP16_STORE>(h, p); break;
+        case LBDEM_PSM_FP64: pp_post_t<LBDEM_PSM_FP64>(h, p, f); break;
+        case LBDEM_PSM_FP32_DEV: pp_post_t<LBDEM_PSM_FP32_DEV>(h, p, f); break;
+        default: pp_post_t<LBDEM_PSM_FP16_STORE>(h, p, f); break;
+    }
+    PSM_HIP(h, hipGetLastError());
+    return LBDEM_PSM_OK;
+}
+
+int periodic_pressure_pre(lbdem_psm* h) { return periodic_pressure_pre_buf(h, h->f[h->cur_f]); }
+
+int periodic_pressure_post(lbdem_psm* h) { return periodic_pressure_post_buf(h, h->f[h->cur_f]); }
+
+/* ---- adjoint of the periodic-pressure step ---------------------------
+ *
+ * Forward step t: S_in, S_out = plane sums of rhoBar over the inlet and
+ * outlet boxes of f(t); h = collide+stream(f(t)); then g = f(t+1) equals h
+ * except g_i + t_i = a_in (h_i + t_i) for c_i[d] = +1 on the inlet box,
+ * a_in = rho_in / m_out, and g_i + t_i = a_out (h_i + t_i) for c_i[d] = -1 on
+ * the outlet box, a_out = rho_out / m_in, with m = 1 + S / A (A cells per
+ * box).  Given lam' = dJ/dg:
+ *   dJ/dh_i   = a lam'_i on the rescaled populations, lam'_i elsewhere;
+ *   dJ/dm_out = -(sum_inlet lam'_i (g_i + t_i)) / m_out  (= dJ/da_in a_in / -m_out),
+ *   dJ/dm_in  = -(sum_outlet lam'_i (g_i + t_i)) / m_in;
+ * and m depends on every population of its box with weight 1/A, so
+ * dJ/df(t) gains -dot_in / (m_out A) on every population of the outlet box
+ * and -dot_out / (m_in A) on every population of the inlet box, next to the
+ * collide+stream adjoint of dJ/dh. */
+
+/* The inlet (0) and outlet (n-1) boxes of f, all 19 populations, compact:
+ * out[(plane * 19 + i) * A + k]. */
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pp_extract(const typename Store<PREC>::S* __restrict__ f, PlaneArgs p,
+                                                    typename Store<PREC>::S* __restrict__ out) {
+    const int na = p.a1 - p.a0 + 1, nb = p.b1 - p.b0 + 1;
+    const int64_t A = (int64_t)na * nb;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < 2 * A;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int pl = (int)(k / A);
+        const int64_t kk = k - pl * A;
+        const int a = p.a0 + (int)(kk / nb), b = p.b0 + (int)(kk % nb);
+        int x, y, z;
+        plane_cell(p.dim, pl == 0 ? 0 : p.n[p.dim] - 1, a, b, x, y, z);
+        const int64_t row = (int64_t)x * p.n[1] + y;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) out[((int64_t)pl * Q + i) * A + kk] = f[pop_at(row, i, z, p.L)];
+    }
+}
+
+/* Partial sums of lam_i (g_i + t_i) over the populations with c_i[dim] ==
+ * dir on one plane box; g is the compact plane copy of f(t+1). */
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pp_adj_dot(const typename Store<PREC>::S* __restrict__ lam,
+                                                    const typename Store<PREC>::S* __restrict__ g, PlaneArgs p,
+                                                    int pl, int dir, double* __restrict__ part) {
+    __shared__ double sh[256];
+    const int na = p.a1 - p.a0 + 1, nb = p.b1 - p.b0 + 1;
+    const int64_t A = (int64_t)na * nb;
+    double acc = 0.0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < A;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int a = p.a0 + (int)(k / nb), b = p.b0 + (int)(k % nb);
+        int x, y, z;
+        plane_cell(p.dim, pl == 0 ? 0 : p.n[p.dim] - 1, a, b, x, y, z);
+        const int64_t row = (int64_t)x * p.n[1] + y;
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (cc(i, p.dim) != dir) continue;
+            const double gi = (double)Store<PREC>::load(g + ((int64_t)pl * Q + i) * A + k) + wgt<double>(i);
+            s += (double)Store<PREC>::load(lam + pop_at(row, i, z, p.L)) * gi;
+        }
+        acc += s;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+/* v[0] = plane-sum inlet, v[1] outlet (of f(t)); v[2] = dot inlet, v[3]
+ * dot outlet -> v[4] added on the inlet box, v[5] on the outlet box. */
+__global__ void k_pp_adj_coef(double* v, double A) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double m_in = 1.0 + v[0] / A, m_out = 1.0 + v[1] / A;
+    v[4] = -v[3] / (m_in * A);
+    v[5] = -v[2] / (m_out * A);
+}
+
+/* lam_i *= rho_target / <rho> on the populations with c_i[dim] == dir of one
+ * plane box (the rescale factor of the forward step). */
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pp_adj_scale(typename Store<PREC>::S* __restrict__ lam, PlaneArgs p,
+                                                      int plane, int dir, double rho_target,
+                                                      const double* __restrict__ rhobar_sum) {
+    typedef typename Store<PREC>::T T;
+    const int na = p.a1 - p.a0 + 1, nb = p.b1 - p.b0 + 1;
+    const int64_t A = (int64_t)na * nb;
+    const double factor = rho_target / (1.0 + rhobar_sum[0] / (double)A);
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < A;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int a = p.a0 + (int)(k / nb), b = p.b0 + (int)(k % nb);
+        int x, y, z;
+        plane_cell(p.dim, plane, a, b, x, y, z);
+        const int64_t row = (int64_t)x * p.n[1] + y;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (cc(i, p.dim) != dir) continue;
+            typename Store<PREC>::S* q = lam + pop_at(row, i, z, p.L);
+            Store<PREC>::store(q, (T)((double)Store<PREC>::load(q) * factor));
+        }
+    }
+}
+
+/* lam_i += c on every population of one plane box. */
+template <int PREC>
+__global__ __launch_bounds__(256) void k_pp_adj_add(typename Store<PREC>::S* __restrict__ lam, PlaneArgs p,
+                                                    int plane, const double* __restrict__ coef) {
+    typedef typename Store<PREC>::T T;
+    const int na = p.a1 - p.a0 + 1, nb = p.b1 - p.b0 + 1;
+    const int64_t A = (int64_t)na * nb;
+    const double c = coef[0];
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < A;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int a = p.a0 + (int)(k / nb), b = p.b0 + (int)(k % nb);
+        int x, y, z;
+        plane_cell(p.dim, plane, a, b, x, y, z);
+        const int64_t row = (int64_t)x * p.n[1] + y;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            typename Store<PREC>::S* q = lam + pop_at(row, i, z, p.L);
+            Store<PREC>::store(q, (T)((double)Store<PREC>::load(q) + c));
+        }
+    }
+}
+
+int64_t periodic_pressure_plane_elems(const lbdem_psm* h) {
+    const PlaneArgs p = plane_args(h);
+    return 2 * (int64_t)Q * (p.a1 - p.a0 + 1) * (p.b1 - p.b0 + 1);
+}
+
+template <int PREC>
+static void pp_extract_t(lbdem_psm* h, const PlaneArgs& p, const void* f, void* out) {
+    typedef typename Store<PREC>::S S;
+    hipLaunchKernelGGL(k_pp_extract<PREC>, dim3(2 * plane_blocks(p)), dim3(256), 0, h->stream, (const S*)f, p,
+                       (S*)out);
+}
+
+int periodic_pressure_extract(lbdem_psm* h, const void* f, void* out) {
+    if (!h->pp.enabled) return LBDEM_PSM_OK;
+    const PlaneArgs p = plane_args(h);
+    switch (h->precision) {
+        case LBDEM_PSM_FP64: pp_extract_t<LBDEM_PSM_FP64>(h, p, f, out); break;
+        case LBDEM_PSM_FP32_DEV: pp_extract_t<LBDEM_PSM_FP32_DEV>(h, p, f, out); break;
+        default: pp_extract_t<LBDEM_PSM_FP16_STORE>(h, p, f, out); break;
+    }
+    PSM_HIP(h, hipGetLastError());
+    return LBDEM_PSM_OK;
+}
+
+template <int PREC>
+static void pp_adj_pre_t(lbdem_psm* h, const PlaneArgs& p, const void* fin, const void* gplanes, void* lam) {
+    typedef typename Store<PREC>::S S;
+    const unsigned nb = plane_blocks(p);
+    const int last = h->g.n[p.dim] - 1;
+    double* v = h->pp.buf;
+    const double A = (double)(p.a1 - p.a0 + 1) * (double)(p.b1 - p.b0 + 1);
+    /* plane sums of f(t) into v[0], v[1] */
+    pp_pre_t<PREC>(h, p, fin);
+    /* dots with the unscaled lam' into v[2], v[3] */
+    hipLaunchKernelGGL(k_pp_adj_dot<PREC>, dim3(nb), dim3(256), 0, h->stream, (const S*)lam, (const S*)gplanes, p, 0,
+                       1, v + 8);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, h->stream, v + 8, (int)nb, v + 2);
+    hipLaunchKernelGGL(k_pp_adj_dot<PREC>, dim3(nb), dim3(256), 0, h->stream, (const S*)lam, (const S*)gplanes, p, 1,
+                       -1, v + 8);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, h->stream, v + 8, (int)nb, v + 3);
+    hipLaunchKernelGGL(k_pp_adj_coef, dim3(1), dim3(64), 0, h->stream, v, A);
+    /* lam' -> dJ/dh on the rescaled populations */
+    hipLaunchKernelGGL(k_pp_adj_scale<PREC>, dim3(nb), dim3(256), 0, h->stream, (S*)lam, p, 0, 1, h->pp.rho_in,
+                       (const double*)(v + 1));
+    hipLaunchKernelGGL(k_pp_adj_scale<PREC>, dim3(nb), dim3(256), 0, h->stream, (S*)lam, p, last, -1, h->pp.rho_out,
+                       (const double*)v);
+}
+
+template <int PREC>
+static void pp_adj_post_t(lbdem_psm* h, const PlaneArgs& p, void* lam) {
+    typedef typename Store<PREC>::S S;
+    const unsigned nb = plane_blocks(p);
+    const int last = h->g.n[p.dim] - 1;
+    hipLaunchKernelGGL(k_pp_adj_add<PREC>, dim3(nb), dim3(256), 0, h->stream, (S*)lam, p, 0,
+                       (const double*)(h->pp.buf + 4));
+    hipLaunchKernelGGL(k_pp_adj_add<PREC>, dim3(nb), dim3(256), 0, h->stream, (S*)lam, p, last,
+                       (const double*)(h->pp.buf + 5));
+}
+
+int periodic_pressure_adjoint_pre(lbdem_psm* h, const void* fin, const void* gplanes, void* lam) {
+    if (!h->pp.enabled) return LBDEM_PSM_OK;
+    const PlaneArgs p = plane_args(h);
+    switch (h->precision) {
+        case LBDEM_PSM_FP64: pp_adj_pre_t<LBDEM_PSM_FP64>(h, p, fin, gplanes, lam); break;
+        default: pp_adj_pre_t<LBDEM_PSM_FP32_DEV>(h, p, fin, gplanes, lam); break;
+    }
+    PSM_HIP(h, hipGetLastError());
+    return LBDEM_PSM_OK;
+}
+
+int periodic_pressure_adjoint_post(lbdem_psm* h, void* lam) {
+    if (!h->pp.enabled) return LBDEM_PSM_OK;
+    const PlaneArgs p = plane_args(h);
+    switch (h->precision) {
+        case LBDEM_PSM_FP64: pp_adj_post_t<LBDEM_PSM_FP64>(h, p, lam); break;
+        default: pp_adj_post_t<LBDEM_PSM_FP32_DEV>(h, p, lam); break;
     }
     PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
@@ -229,6 +443,7 @@ int lbdem_psm_set_periodic_pressure(lbdem_psm* h, int32_t enable, int32_t dimens
                                     const int32_t* range4) {
     if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
     if (!enable) {
+        if (h->pp.enabled) h->geom_gen += 1;   /* a differentiable window ran with it */
         h->pp.enabled = false;
         return LBDEM_PSM_OK;
     }
@@ -249,6 +464,7 @@ int lbdem_psm_set_periodic_pressure(lbdem_psm* h, int32_t enable, int32_t dimens
         return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "cross-section box outside the lattice or empty");
     if (!h->pp.buf) PSM_HIP(h, hipMalloc((void**)&h->pp.buf, (8 + 1024) * sizeof(double)));
     h->pp.enabled = true;
+    h->geom_gen += 1;   /* the step map changed: a window from before no longer applies */
     h->pp.dim = dimension;
     h->pp.rho_in = rho_in;
     h->pp.rho_out = rho_out;

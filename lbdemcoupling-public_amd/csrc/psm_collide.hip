@@ -203,14 +203,17 @@ int launch_collide_stream(lbdem_psm* h) {
     return periodic_pressure_post(h);
 }
 
-/* One collide+stream between caller-chosen buffers; the per-cell forces go
- * to rF (3 per record slot) instead of the records. */
+/* One collide+stream between caller-chosen buffers (with the periodic-
+ * pressure rescale when enabled); the per-cell forces go to rF (3 per
+ * record slot) instead of the records. */
 int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, double* rF) {
     CollideArgs a = make_args(h);
     a.rF = rF;
+    int rc = periodic_pressure_pre_buf(h, fin);
+    if (rc) return rc;
     launch_any(h, a, fin, fout);
     PSM_HIP(h, hipGetLastError());
-    return LBDEM_PSM_OK;
+    return periodic_pressure_post_buf(h, fout);
 }
 
 static unsigned grid_for(int64_t n) {

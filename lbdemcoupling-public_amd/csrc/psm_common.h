@@ -433,7 +433,8 @@ struct lbdem_psm {
         int64_t gen = -1;
         int32_t T = 0, K = 0;
         int64_t np = 0;
-        std::vector<void*> ckpt;     /* population state at t = 0, K, 2K, ... */
+        std::vector<void*> ckpt;     /* population state at each segment start */
+        std::vector<int32_t> seg_t0; /* segment start steps of the window */
         std::vector<void*> seg;      /* recomputed states of one segment */
         void* lam[2] = {nullptr, nullptr};
         double* d_ft = nullptr;      /* T*np*6, per-particle (F, T) of every step */
@@ -448,6 +449,10 @@ struct lbdem_psm {
         void* pb = nullptr;          /* 19 per record slot: stage-1 adjoint of the record cells */
         void* pc = nullptr;          /* 5 per record slot: coefficients of the BGK adjoint */
         int32_t* rec_list = nullptr; /* record slots of the frozen geometry */
+        bool pp = false;             /* the window ran with the periodic-pressure rescale */
+        void* pp_last = nullptr;     /* periodic pressure: inlet/outlet planes of f(T) */
+        void* pp_next = nullptr;     /* ... of the state after the current segment */
+        void* pp_cur = nullptr;      /* ... of f(t+1) inside a segment */
         unsigned long long* d_nrec = nullptr;
         int64_t nrec = 0;
     } diff;
@@ -474,6 +479,16 @@ int diff_free(lbdem_psm* h);
 int periodic_pressure_pre(lbdem_psm* h);
 int launch_macroscopic(lbdem_psm* h, double* d_rho, double* d_u);
 int periodic_pressure_post(lbdem_psm* h);
+/* the same on caller-chosen population buffers (the differentiable window) */
+int periodic_pressure_pre_buf(lbdem_psm* h, const void* f);
+int periodic_pressure_post_buf(lbdem_psm* h, void* f);
+/* adjoint of the rescale: before the collide+stream adjoint (scales lam =
+ * dJ/df(t+1) in place, gplanes = compact planes of f(t+1)) and after it
+ * (adds the plane-mean terms to lam = dJ/df(t)) */
+int periodic_pressure_extract(lbdem_psm* h, const void* f, void* out);
+int64_t periodic_pressure_plane_elems(const lbdem_psm* h);
+int periodic_pressure_adjoint_pre(lbdem_psm* h, const void* fin, const void* gplanes, void* lam);
+int periodic_pressure_adjoint_post(lbdem_psm* h, void* lam);
 int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
                         int64_t* n_nonfinite);
 int run_voxelize(lbdem_psm* h);

@@ -133,16 +133,32 @@ def test_adjoint_checkpoint_interval_invariance():
     g = gpu_lattice("overlap")
     G = np.random.default_rng(2).normal(size=(nsteps, 3, 6))
     outs = []
-    for K in (1, 3, 7, 10):
+    # K > 0: a checkpoint every K steps; K = -M: a budget of M states with
+    # segments of M, M-1, ... steps (4 + 3, 5 + 2, 7)
+    for K in (1, 3, 7, 10, -4, -5, -7):
         g.set_populations(f0)
         g.clear_records()
         g.set_spheres(x, v, w, r, ids)
         g.diff_forward(nsteps, K)
         outs.append(g.diff_backward(G, want_f0=True))
+        if K < 0:
+            with pytest.raises(L.PSMError):   # the budgeted window is consumed
+                g.diff_backward(G)
     for o in outs[1:]:
         for k in ("x", "v", "w", "sf", "B", "f0"):
             assert np.array_equal(o[k], outs[0][k]), k
         assert o["tau"] == outs[0]["tau"]
+    g.diff_release()
+
+
+def test_adjoint_state_budget_too_small():
+    x, v, w, r, ids = spheres()
+    g = gpu_lattice("overlap")
+    g.set_populations(initial_populations(CASE["n"]))
+    g.set_spheres(x, v, w, r, ids)
+    with pytest.raises(L.PSMError):
+        g.diff_forward(7, -3)      # 3 + 2 + 1 < 7 steps
+    g.diff_forward(6, -3)          # 3 + 2 + 1 = 6
     g.diff_release()
 
 
@@ -169,6 +185,82 @@ def test_adjoint_transpose_identity_one_step():
     lhs = float(np.sum((fp - fm) / (2 * eps) * wt))
     rhs = float(np.sum(u * lam0))
     assert lhs == pytest.approx(rhs, rel=1e-7)
+
+
+def pp_lattice(precision="fp64"):
+    """CASE with a Zhang-Kwok pressure difference along the periodic z axis
+    (large enough that the rescale matters: factors of about 1 +- 0.02)."""
+    g = L.Lattice(*CASE["n"], periodic=CASE["periodic"], tau=CASE["tau"], precision=precision,
+                  body_accel=(0.0, 0.0, 0.0), voxel_mode="overlap")
+    g.set_periodic_pressure(2, 1.02, 0.98)
+    return g
+
+
+@pytest.mark.parametrize("nsteps,K", [(1, 1), (4, 2), (5, -3)])
+def test_adjoint_transpose_identity_periodic_pressure(nsteps, K):
+    """<(S(f0 + e u) - S(f0 - e u)) / 2e, w> = <u, S^T w> for nsteps steps of
+    collide+stream with the periodic-pressure rescale (its plane means make
+    the step nonlinear in f; the central difference is exact to O(e^2))."""
+    n = CASE["n"]
+    f0 = initial_populations(n)
+    x, v, w, r, ids = spheres()
+    g = pp_lattice()
+    g.set_populations(f0)
+    g.set_spheres(x, v, w, r, ids)
+    rng = np.random.default_rng(11)
+    u = rng.normal(size=f0.shape)
+    wt = rng.normal(size=f0.shape)
+    g.diff_forward(nsteps, K)
+    lam0 = g.diff_backward(g_fT=wt, want_f0=True)["f0"]
+    eps = 1e-6
+    g.set_populations(f0 + eps * u)
+    g.diff_forward(nsteps, K)
+    fp = g.get_populations()
+    g.set_populations(f0 - eps * u)
+    g.diff_forward(nsteps, K)
+    fm = g.get_populations()
+    lhs = float(np.sum((fp - fm) / (2 * eps) * wt))
+    rhs = float(np.sum(u * lam0))
+    assert lhs == pytest.approx(rhs, rel=1e-7)
+    g.diff_release()
+
+
+def test_adjoint_periodic_pressure_parameters_fd():
+    """Drag gradients w.r.t. v, w, x, tau and sf through a window with the
+    periodic-pressure rescale, against central differences of the forward."""
+    n = CASE["n"]
+    f0 = initial_populations(n)
+    x, v, w, r, ids = spheres()
+    nsteps = 6
+    G = np.random.default_rng(5).normal(size=(nsteps, 3, 6))
+
+    def J(xx, vv, ww, tau):
+        g = pp_lattice()
+        g.set_tau(tau)
+        g.set_populations(f0)
+        g.set_spheres(xx, vv, ww, r, ids)
+        ft = g.diff_forward(nsteps, 2)
+        g.close()
+        return float(np.sum(ft * G))
+
+    g = pp_lattice()
+    g.set_populations(f0)
+    g.set_spheres(x, v, w, r, ids)
+    g.diff_forward(nsteps, 2)
+    gr = g.diff_backward(G)
+    g.close()
+    eps = 1e-6
+    checks = [("v", (0, 2)), ("v", (2, 0)), ("w", (1, 1)), ("x", (0, 0)), ("x", (2, 2))]
+    for name, (p_, d_) in checks:
+        arrs = {"x": x.copy(), "v": v.copy(), "w": w.copy()}
+        arrs[name][p_, d_] += eps
+        jp = J(arrs["x"], arrs["v"], arrs["w"], CASE["tau"])
+        arrs[name][p_, d_] -= 2 * eps
+        jm = J(arrs["x"], arrs["v"], arrs["w"], CASE["tau"])
+        fd = (jp - jm) / (2 * eps)
+        assert gr[name][p_, d_] == pytest.approx(fd, rel=1e-5, abs=1e-9), (name, p_, d_)
+    fd = (J(x, v, w, CASE["tau"] + eps) - J(x, v, w, CASE["tau"] - eps)) / (2 * eps)
+    assert gr["tau"] == pytest.approx(fd, rel=1e-5)
 
 
 def channel_case(precision, nsteps=40):

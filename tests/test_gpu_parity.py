@@ -337,7 +337,7 @@ def test_poiseuille_init_matches_oracle():
     for prec, tol in (("fp64", 0.0), ("fp32_dev", 1e-7)):
         g = L.Lattice(*n, periodic=(1, 0, 0), tau=0.8, precision=prec)
         o = O.OracleLattice(*n, periodic=(1, 0, 0), tau=0.8)
-        for dim in (0, 2):
+        for dim in (0, 1, 2):
             n3 = list(n)
             n3[dim] -= 1
             g.init_poiseuille(dim, 1.002, 0.998, 0.03, n3=n3)
@@ -380,6 +380,32 @@ def test_periodic_pressure_channel_matches_oracle():
     assert rel_err(g.get_populations(), o.get_populations()) < 1e-12
 
 
+@pytest.mark.parametrize("dim", [0, 1, 2])
+def test_periodic_pressure_every_axis_matches_oracle(dim):
+    """The Zhang-Kwok rescale along each axis of a fully periodic box with
+    spheres, 6 steps, against the oracle (the inlet/outlet boxes are x, y
+    or z planes; each axis maps the box onto the lattice differently)."""
+    n = (14, 12, 16)
+    g, o = make_pair(n, periodic=(1, 1, 1), tau=0.8, N=5)
+    for lat in (g, o):
+        lat.set_periodic_pressure(dim, 1.02, 0.98)
+    x = np.array([[6.3, 5.2, 3.1], [7.6, 4.1, 10.4], [4.2, 8.7, 14.6]])
+    v = np.array([[0.01, -0.004, 0.003], [-0.006, 0.002, 0.008], [0.003, 0.005, -0.007]])
+    w = np.zeros((3, 3))
+    r = np.array([3.1, 2.7, 2.4])
+    ids = np.array([1, 2, 3])
+    g.set_spheres(x, v, w, r, ids)
+    o.set_spheres(x, v, w, r, ids)
+    for _ in range(6):
+        g.collide_stream(1)
+        o.collide_stream(1, 1)
+    Fg, Tg = g.sum_force_torque(x)
+    Fo, To = o.sum_force_torque(x)
+    assert rel_err(g.get_populations(), o.get_populations()) < 1e-12
+    assert rel_err(Fg, Fo) < 1e-12
+    assert torque_err(Tg, To, Fo) < 1e-12
+
+
 def test_periodic_pressure_errors():
     g = L.Lattice(8, 8, 8, periodic=(1, 0, 1), tau=0.8)
     with pytest.raises(L.PSMError):
@@ -391,8 +417,10 @@ def test_periodic_pressure_errors():
     g.set_periodic_pressure(0, 1.01, 0.99)
     g.set_spheres(np.array([[4.0, 4.0, 4.0]]), np.zeros((1, 3)), np.zeros((1, 3)), np.array([2.0]),
                   np.array([1]))
+    g.diff_forward(2, 1)
+    g.set_periodic_pressure(0, 1.0, 1.0, enable=False)
     with pytest.raises(L.PSMError):
-        g.diff_forward(2, 1)                             # not part of the differentiable window
+        g.diff_backward(np.zeros((2, 1, 6)))             # the step map changed since the forward
 
 
 def test_macroscopic_fields_match_populations():
