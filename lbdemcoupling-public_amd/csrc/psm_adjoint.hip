@@ -249,11 +249,16 @@ struct AdjArgs {
     const int32_t* id_map;
     int64_t id_map_size;
     const double* xs;           /* positions of the torque arms */
-    double* gacc;               /* 7 per record slot (SoA): gB, gus[3], F x gT[3] */
+    /* gacc, pb and pc hold one entry per record at its position in the
+     * window's record list (stride pcap), so the record stage writes them
+     * contiguously; rpos maps a record cell to that position */
+    double* gacc;               /* 7 per record (SoA): gB, gus[3], F x gT[3] */
     double* opart;              /* one omega-gradient partial per workgroup */
     const uint32_t* bin_live;   /* live records per bin */
-    void* pb;                   /* 19 per record slot (SoA, arithmetic type): stage-1 adjoint of fP */
-    void* pc;                   /* 5 per record slot (SoA): coef, gr, gj[3] */
+    void* pb;                   /* 19 per record (SoA, arithmetic type): stage-1 adjoint of fP */
+    void* pc;                   /* 5 per record (SoA): coef, gr, gj[3] */
+    const int32_t* rpos;        /* per cell: position in the record list */
+    int64_t pcap;
 };
 
 /* Adjoint of the pushed populations at (x, y, z): pull from x + c_i, or
@@ -307,7 +312,7 @@ __device__ __forceinline__ void pull_adjoint_row(const typename Store<PREC>::S* 
 template <int PREC, bool FORCE>
 __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::S* __restrict__ fin,
                                                     const typename Store<PREC>::S* __restrict__ lin,
-                                                    const AdjArgs& a, int64_t r) {
+                                                    const AdjArgs& a, int64_t r, int64_t p) {
     typedef typename Store<PREC>::T T;
     int x, y, z;
     rec_cell(r, a.ny, a.nz, x, y, z);
@@ -346,23 +351,24 @@ __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::
     blend_vjp<T, FORCE>(fP, lam, pb, omega, acc, full, B, us, gF, Bb, usb, F, coef, gr, gj);
     T* pbo = (T*)a.pb;
     T* pco = (T*)a.pc;
+    const int64_t P = a.pcap;
 #pragma unroll
-    for (int i = 0; i < Q; ++i) pbo[i * a.rcap + r] = pb[i];
-    pco[r] = coef;
-    pco[a.rcap + r] = gr;
-    pco[2 * a.rcap + r] = gj[0];
-    pco[3 * a.rcap + r] = gj[1];
-    pco[4 * a.rcap + r] = gj[2];
+    for (int i = 0; i < Q; ++i) pbo[i * P + p] = pb[i];
+    pco[p] = coef;
+    pco[P + p] = gr;
+    pco[2 * P + p] = gj[0];
+    pco[3 * P + p] = gj[1];
+    pco[4 * P + p] = gj[2];
     double* g = a.gacc;
-    if (!full) g[r] += (double)Bb;
-    g[a.rcap + r] += (double)usb[0];
-    g[2 * a.rcap + r] += (double)usb[1];
-    g[3 * a.rcap + r] += (double)usb[2];
+    if (!full) g[p] += (double)Bb;
+    g[P + p] += (double)usb[0];
+    g[2 * P + p] += (double)usb[1];
+    g[3 * P + p] += (double)usb[2];
     if (obj) {
         const double Fx = F[0], Fy = F[1], Fz = F[2];
-        g[4 * a.rcap + r] += Fy * gT[2] - Fz * gT[1];
-        g[5 * a.rcap + r] += Fz * gT[0] - Fx * gT[2];
-        g[6 * a.rcap + r] += Fx * gT[1] - Fy * gT[0];
+        g[4 * P + p] += Fy * gT[2] - Fz * gT[1];
+        g[5 * P + p] += Fz * gT[0] - Fx * gT[2];
+        g[6 * P + p] += Fx * gT[1] - Fy * gT[0];
     }
 }
 
@@ -374,7 +380,7 @@ __global__ __launch_bounds__(256) void k_adjoint_records(const typename Store<PR
                                                          int64_t n) {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    adjoint_record_cell<PREC, FORCE>(fin, lin, a, (int64_t)list[k]);
+    adjoint_record_cell<PREC, FORCE>(fin, lin, a, (int64_t)list[k], k);
 }
 
 /* The record slots of the frozen geometry, appended per wavefront (one
@@ -383,6 +389,7 @@ __global__ __launch_bounds__(256) void k_adjoint_records(const typename Store<PR
 __global__ __launch_bounds__(256) void k_list_records(const uint8_t* __restrict__ sflag,
                                                       const uint32_t* __restrict__ bin_live, int n0, int n1,
                                                       int n2, int nb1, int nb2, int32_t* __restrict__ list,
+                                                      int32_t* __restrict__ rpos,
                                                       unsigned long long* __restrict__ count, int64_t cap) {
     const int64_t b = blockIdx.x;
     if (bin_live[b] == 0u) return;
@@ -401,7 +408,11 @@ __global__ __launch_bounds__(256) void k_list_records(const uint8_t* __restrict_
         base = __shfl(base, leader);
         if (live) {
             const unsigned long long pos = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
-            if ((int64_t)pos < cap) list[pos] = (int32_t)rec_index(X, Y, Z, n1, n2);
+            if ((int64_t)pos < cap) {
+                const int64_t c = rec_index(X, Y, Z, n1, n2);
+                list[pos] = (int32_t)c;
+                rpos[c] = (int32_t)pos;
+            }
         }
     }
 }
@@ -439,17 +450,18 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
         const uint32_t zoff = (uint32_t)threadIdx.x * (uint32_t)sizeof(S);
 #pragma unroll
         for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load((const S*)(frow + i * qstride + zoff));
-        const bool rec = a.sflag[(uint32_t)row * (uint32_t)a.nz + (uint32_t)z] != 0;
-        int64_t r = 0;
+        /* a record cell of the window's list (its position r) */
+        int64_t r = -1;
+        if (a.sflag[(uint32_t)row * (uint32_t)a.nz + (uint32_t)z] != 0) r = a.rpos[rec_index(x, y, z, a.ny, a.nz)];
+        const bool rec = r >= 0;
         T coef = T(1), gr = T(0), gj[3] = {T(0), T(0), T(0)};
         if (rec) {
-            r = rec_index(x, y, z, a.ny, a.nz);
             const T* pc = (const T*)a.pc;
             coef = pc[r];
-            gr = pc[a.rcap + r];
-            gj[0] = pc[2 * a.rcap + r];
-            gj[1] = pc[3 * a.rcap + r];
-            gj[2] = pc[4 * a.rcap + r];
+            gr = pc[a.pcap + r];
+            gj[0] = pc[2 * a.pcap + r];
+            gj[1] = pc[3 * a.pcap + r];
+            gj[2] = pc[4 * a.pcap + r];
         }
         BgkCell<T, FORCE> s;
         s.init(fP, omega, acc);
@@ -506,7 +518,7 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
             /* the record cell's direct part, all loads in flight before the adds */
             const T* pb = (const T*)a.pb;
 #pragma unroll
-            for (int i = 0; i < Q; ++i) fP[i] = pb[i * a.rcap + r];
+            for (int i = 0; i < Q; ++i) fP[i] = pb[i * a.pcap + r];
 #pragma unroll
             for (int i = 0; i < Q; ++i) lam[i] = fP[i] + lam[i];
         }
@@ -535,7 +547,8 @@ __global__ __launch_bounds__(256) void k_sum_vec(const double* __restrict__ in, 
 __global__ __launch_bounds__(256) void k_grad_cells(
                                                     int64_t rcap, const uint8_t* __restrict__ sflag,
                                                     const double* __restrict__ rsf,
-                                                    const double* __restrict__ gacc, double ooo,
+                                                    const double* __restrict__ gacc,
+                                                    const int32_t* __restrict__ rpos, double ooo,
                                                     double* __restrict__ cell_out, int64_t nc,
                                                     double* __restrict__ tau_part) {
     __shared__ double sh[4];
@@ -544,10 +557,10 @@ __global__ __launch_bounds__(256) void k_grad_cells(
          r += (int64_t)gridDim.x * blockDim.x) {
         const int64_t c = r;   /* record slot = cell index */
         double gB = 0.0, gsf = 0.0;
-        if (sflag[c]) {
+        if (sflag[c] && rpos[c] >= 0) {
             const double sf = rsf[r];
             if (!(sf > SOLFRAC_MAX)) {
-                gB = gacc[r];
+                gB = gacc[rpos[c]];
                 const double den = (1.0 - sf) + ooo;
                 gsf = gB * (ooo * (1.0 + ooo)) / (den * den);
                 tp += gB * (sf * (1.0 - sf)) / (den * den);
@@ -568,7 +581,9 @@ struct GradArgs {
     const int32_t* rcand;
     const int32_t* rid;
     int64_t rcap;
-    const double* gacc;
+    const double* gacc;         /* at the record's list position (stride pcap) */
+    const int32_t* rpos;
+    int64_t pcap;
     const double* xs;
     const double* pw;
     const double* pr;
@@ -590,8 +605,10 @@ __device__ __forceinline__ void record_grad(const GradArgs& a, int64_t r, int X,
     const double d[3] = {min_image((double)X - a.xs[3 * o], a.n[0]),
                          min_image((double)Y - a.xs[3 * o + 1], a.n[1]),
                          min_image((double)Z - a.xs[3 * o + 2], a.n[2])};
-    const double g[3] = {a.gacc[a.rcap + r], a.gacc[2 * a.rcap + r], a.gacc[3 * a.rcap + r]};
-    const double fxt[3] = {a.gacc[4 * a.rcap + r], a.gacc[5 * a.rcap + r], a.gacc[6 * a.rcap + r]};
+    const int64_t p = a.rpos[r], P = a.pcap;
+    if (p < 0) return;   /* not in the window's record list */
+    const double g[3] = {a.gacc[P + p], a.gacc[2 * P + p], a.gacc[3 * P + p]};
+    const double fxt[3] = {a.gacc[4 * P + p], a.gacc[5 * P + p], a.gacc[6 * P + p]};
     const double w[3] = {a.pw[3 * o], a.pw[3 * o + 1], a.pw[3 * o + 2]};
     /* us = v + w x d, d = cell - x */
     acc[0] += g[0];
@@ -607,7 +624,7 @@ __device__ __forceinline__ void record_grad(const GradArgs& a, int64_t r, int X,
         const double sf = a.rsf[r];
         if (!(sf > SOLFRAC_MAX)) {
             const double den = (1.0 - sf) + a.ooo;
-            const double gsf = a.gacc[r] * (a.ooo * (1.0 + a.ooo)) / (den * den);
+            const double gsf = a.gacc[p] * (a.ooo * (1.0 + a.ooo)) / (den * den);
             const double dist = sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
             double der = 0.0;
             int br;
@@ -735,7 +752,9 @@ int diff_free(lbdem_psm* h) {
     D.lam[0] = D.lam[1] = nullptr;
     dfree_d(D.d_ft); dfree_d(D.gacc); dfree_d(D.Fscr); dfree_d(D.opart); dfree_d(D.gpart);
     dfree_d(D.cell_out); dfree_d(D.tau_part); dfree_d(D.pb); dfree_d(D.pc);
-    dfree_d(D.rec_list); dfree_d(D.d_nrec);
+    dfree_d(D.rec_list); dfree_d(D.d_nrec); dfree_d(D.rpos);
+    D.rpos = nullptr;
+    D.pcap = 0;
     dfree_d(D.pp_last); dfree_d(D.pp_next); dfree_d(D.pp_cur);
     D.pp_last = D.pp_next = D.pp_cur = nullptr;
     D.pb = D.pc = nullptr;
@@ -776,6 +795,8 @@ static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lo
     a.bin_live = h->bin_any[h->cur_any];
     a.pb = h->diff.pb;
     a.pc = h->diff.pc;
+    a.rpos = h->diff.rpos;
+    a.pcap = h->diff.pcap;
     const int64_t nrec = h->diff.nrec;
     dim3 gridr((unsigned)std::max<int64_t>(1, (nrec + 255) / 256)), blockr(256);
     const int32_t* list = h->diff.rec_list;
@@ -906,14 +927,21 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
     /* checkpoints consumed by a budgeted backward pass were moved to the
      * recompute pool; drop the empty entries */
     D.ckpt.erase(std::remove(D.ckpt.begin(), D.ckpt.end(), nullptr), D.ckpt.end());
-    int rc = ensure_states(h, D.ckpt, (size_t)ncp);
-    if (rc) return rc;
+    int rc = LBDEM_PSM_OK;
     if (K < 0) {
-        /* budgeted: the recompute pool starts empty, so that checkpoints
-         * plus pool never exceed the budget */
-        for (void* p : D.seg) dfree_d(p);
+        /* budgeted: all state buffers (checkpoints and recompute pool) are
+         * one pool of at most M; the first ncp become checkpoints, the rest
+         * stay in the recompute pool, so a repeated window allocates nothing */
+        const size_t M = (size_t)(-(int64_t)K);
+        std::vector<void*> pool(D.ckpt);
+        pool.insert(pool.end(), D.seg.begin(), D.seg.end());
+        D.ckpt.clear();
         D.seg.clear();
+        while (pool.size() > M) { dfree_d(pool.back()); pool.pop_back(); }
+        for (size_t k = 0; k < pool.size(); ++k) ((k < (size_t)ncp) ? D.ckpt : D.seg).push_back(pool[k]);
     }
+    rc = ensure_states(h, D.ckpt, (size_t)ncp);
+    if (rc) return rc;
     D.seg_t0 = t0;
     dfree_d(D.d_ft);
     D.d_ft = nullptr;
@@ -922,11 +950,13 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
     PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
     /* record slots of the frozen geometry, for the adjoint's first stage */
     if (!D.rec_list) PSM_HIP(h, dalloc_d(&D.rec_list, h->rec_cap * sizeof(int32_t)));
+    if (!D.rpos) PSM_HIP(h, dalloc_d(&D.rpos, h->g.nc * sizeof(int32_t)));
     if (!D.d_nrec) PSM_HIP(h, dalloc_d(&D.d_nrec, sizeof(unsigned long long)));
     PSM_HIP(h, hipMemsetAsync(D.d_nrec, 0, sizeof(unsigned long long), h->stream));
+    PSM_HIP(h, hipMemsetAsync(D.rpos, 0xff, h->g.nc * sizeof(int32_t), h->stream));   /* -1: no record */
     hipLaunchKernelGGL(k_list_records, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, h->sflag,
                        h->bin_any[h->cur_any], h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2], D.rec_list,
-                       D.d_nrec, h->rec_cap);
+                       D.rpos, D.d_nrec, h->rec_cap);
     PSM_HIP(h, hipGetLastError());
     for (int t = 0, nextc = 0; t < nsteps; ++t) {
         if (nextc < ncp && t == t0[nextc]) {
@@ -1005,11 +1035,20 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
     }
     for (int k = 0; k < 2; ++k)
         if (!D.lam[k]) PSM_HIP(h, dalloc_d(&D.lam[k], h->pop_bytes));
-    if (!D.gacc) PSM_HIP(h, dalloc_d(&D.gacc, 7 * rcap * sizeof(double)));
     if (!D.Fscr) PSM_HIP(h, dalloc_d(&D.Fscr, 3 * rcap * sizeof(double)));
+    /* per-record adjoint arrays at the records' list positions: sized by the
+     * window's record count */
     const size_t tsz = h->precision == LBDEM_PSM_FP64 ? sizeof(double) : sizeof(float);
-    if (!D.pb) PSM_HIP(h, dalloc_d(&D.pb, (size_t)Q * rcap * tsz));
-    if (!D.pc) PSM_HIP(h, dalloc_d(&D.pc, (size_t)5 * rcap * tsz));
+    const int64_t P = std::max<int64_t>(D.nrec, 1);
+    if (D.pcap != P || D.ptsz != tsz) {
+        dfree_d(D.gacc); dfree_d(D.pb); dfree_d(D.pc);
+        D.gacc = nullptr; D.pb = D.pc = nullptr;
+        PSM_HIP(h, dalloc_d(&D.gacc, 7 * (size_t)P * sizeof(double)));
+        PSM_HIP(h, dalloc_d(&D.pb, (size_t)Q * P * tsz));
+        PSM_HIP(h, dalloc_d(&D.pc, (size_t)5 * P * tsz));
+        D.pcap = P;
+        D.ptsz = tsz;
+    }
     const int64_t nblk = (int64_t)h->g.n[0] * h->g.n[1] * h->g.L.nzc;
     if (D.opart_n < nblk + 8) {
         dfree_d(D.opart);
@@ -1041,7 +1080,7 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
         rc = launch_pop_convert_buf(h, D.lam[cur], tmp64, true);
         if (rc) { dfree_d(tmp64); return rc; }
     }
-    PSM_HIP(h, hipMemsetAsync(D.gacc, 0, 7 * rcap * sizeof(double), h->stream));
+    PSM_HIP(h, hipMemsetAsync(D.gacc, 0, 7 * (size_t)D.pcap * sizeof(double), h->stream));
     PSM_HIP(h, hipMemsetAsync(D.opart, 0, D.opart_n * sizeof(double), h->stream));
 
     const bool pp = D.pp;
@@ -1114,7 +1153,7 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
     double* red = h->red_tmp;   /* [0] omega adjoint, [1] tau adjoint through B */
     hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, h->stream, D.opart, nblk, red);
     hipLaunchKernelGGL(k_grad_cells, dim3((unsigned)gblocks), dim3(256), 0, h->stream, rcap, h->sflag, h->rec.sf,
-                       D.gacc, h->ooo, D.cell_out, nc, D.tau_part);
+                       D.gacc, D.rpos, h->ooo, D.cell_out, nc, D.tau_part);
     hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, h->stream, D.tau_part, (int64_t)gblocks, red + 1);
     if (np > 0) {
         GradArgs ga;
@@ -1125,6 +1164,8 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
         ga.rid = h->rec.id;
         ga.rcap = rcap;
         ga.gacc = D.gacc;
+        ga.rpos = D.rpos;
+        ga.pcap = D.pcap;
         ga.xs = h->psum_x;
         ga.pw = h->pw;
         ga.pr = h->pr;

@@ -438,7 +438,7 @@ struct lbdem_psm {
         std::vector<void*> seg;      /* recomputed states of one segment */
         void* lam[2] = {nullptr, nullptr};
         double* d_ft = nullptr;      /* T*np*6, per-particle (F, T) of every step */
-        double* gacc = nullptr;      /* 7 per record slot: gB, gus[3], F x gT[3] */
+        double* gacc = nullptr;      /* 7 per record (list position): gB, gus[3], F x gT[3] */
         double* Fscr = nullptr;      /* 3 per record slot: forces of recomputed steps */
         double* opart = nullptr;     /* omega-gradient partial per workgroup */
         int64_t opart_n = 0;
@@ -446,9 +446,12 @@ struct lbdem_psm {
         double* cell_out = nullptr;  /* 2*Nc dense gsf, gB */
         double* tau_part = nullptr;
         int64_t tau_part_n = 0;
-        void* pb = nullptr;          /* 19 per record slot: stage-1 adjoint of the record cells */
-        void* pc = nullptr;          /* 5 per record slot: coefficients of the BGK adjoint */
+        void* pb = nullptr;          /* 19 per record (list position): stage-1 adjoint of the record cells */
+        void* pc = nullptr;          /* 5 per record (list position): coefficients of the BGK adjoint */
         int32_t* rec_list = nullptr; /* record slots of the frozen geometry */
+        int32_t* rpos = nullptr;     /* per cell: position in rec_list, -1 without a record */
+        int64_t pcap = 0;            /* entries of gacc, pb, pc (the record count) */
+        size_t ptsz = 0;             /* arithmetic-type bytes of pb, pc */
         bool pp = false;             /* the window ran with the periodic-pressure rescale */
         void* pp_last = nullptr;     /* periodic pressure: inlet/outlet planes of f(T) */
         void* pp_next = nullptr;     /* ... of the state after the current segment */

@@ -475,8 +475,9 @@ class Lattice:
                                                 _ptr(ft)))
         return ft
 
-    def diff_backward(self, g_ft=None, g_fT=None, want_f0=False):
-        """Adjoint of the last diff_forward: dict of gradients."""
+    def diff_backward(self, g_ft=None, g_fT=None, want_f0=False, want_cells=True):
+        """Adjoint of the last diff_forward: dict of gradients.  want_cells=False
+        skips the dense per-cell sf and B gradients (2 Nc doubles)."""
         n = self.np_
         gft = None if g_ft is None else _f64(g_ft)
         gfT = None if g_fT is None else _f64(g_fT).reshape(Q, -1)
@@ -484,8 +485,8 @@ class Lattice:
         gv = np.zeros((n, 3))
         gw = np.zeros((n, 3))
         gtau = ctypes.c_double()
-        gsf = np.zeros(self.nc)
-        gB = np.zeros(self.nc)
+        gsf = np.zeros(self.nc) if want_cells else None
+        gB = np.zeros(self.nc) if want_cells else None
         gf0 = np.zeros((Q, self.nc)) if want_f0 else None
         self._c(self.lib.lbdem_psm_diff_backward(self.h, _ptr(gft), _ptr(gfT), _ptr(gx), _ptr(gv),
                                                  _ptr(gw), ctypes.byref(gtau), _ptr(gsf), _ptr(gB),
