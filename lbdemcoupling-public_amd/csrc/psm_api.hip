@@ -558,7 +558,14 @@ int lbdem_psm_get_cell_records(lbdem_psm* h, double* sf, int64_t* id, double* us
     PSM_HIP(h, hipMemcpy(rsf.data(), h->rec.sf, cap * sizeof(double), hipMemcpyDeviceToHost));
     PSM_HIP(h, hipMemcpy(rid.data(), h->rec.id, cap * sizeof(int32_t), hipMemcpyDeviceToHost));
     PSM_HIP(h, hipMemcpy(rus.data(), h->rec.us, 3 * cap * sizeof(double), hipMemcpyDeviceToHost));
-    PSM_HIP(h, hipMemcpy(rF.data(), h->rec.F, 3 * cap * sizeof(double), hipMemcpyDeviceToHost));
+    if (h->precision == LBDEM_PSM_FP64) {
+        PSM_HIP(h, hipMemcpy(rF.data(), h->rec.F, 3 * cap * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        /* FP32/FP16 storage keeps the per-cell force as float */
+        std::vector<float> f32(3 * cap);
+        PSM_HIP(h, hipMemcpy(f32.data(), h->rec.F, 3 * cap * sizeof(float), hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < 3 * cap; ++k) rF[k] = (double)f32[k];
+    }
     for (int X = 0; X < n0; ++X)
         for (int Y = 0; Y < n1; ++Y)
             for (int Z = 0; Z < n2; ++Z) {

@@ -206,7 +206,7 @@ int launch_collide_stream(lbdem_psm* h) {
 /* One collide+stream between caller-chosen buffers (with the periodic-
  * pressure rescale when enabled); the per-cell forces go to rF (3 per
  * record slot) instead of the records. */
-int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, double* rF) {
+int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, void* rF) {
     CollideArgs a = make_args(h);
     a.rF = rF;
     int rc = periodic_pressure_pre_buf(h, fin);

@@ -26,7 +26,7 @@ struct CollideArgs {
     const uint8_t* sflag;
     const double* rsf;
     const double* rus;
-    double* rF;
+    void* rF;        /* per-cell force: double for FP64 storage, float otherwise */
     int64_t rcap;
 };
 
@@ -412,9 +412,19 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
      * instead of the partial-line writes of the solid cells alone, which
      * cost a read-modify-write each.  Readers look at solid cells only. */
     if (__builtin_amdgcn_ballot_w64(solid) != 0) {
-        a.rF[c] = Fc[0];
-        a.rF[a.rcap + c] = Fc[1];
-        a.rF[2 * a.rcap + c] = Fc[2];
+        if constexpr (sizeof(T) == 4) {
+            /* the FP32 arithmetic's force, stored exactly in float: half the
+             * bytes here and in the force sum */
+            float* rF = (float*)a.rF;
+            rF[c] = (float)Fc[0];
+            rF[a.rcap + c] = (float)Fc[1];
+            rF[2 * a.rcap + c] = (float)Fc[2];
+        } else {
+            double* rF = (double*)a.rF;
+            rF[c] = Fc[0];
+            rF[a.rcap + c] = Fc[1];
+            rF[2 * a.rcap + c] = Fc[2];
+        }
     }
     if (push.near_wall(chunk, nzc)) {
 #pragma unroll

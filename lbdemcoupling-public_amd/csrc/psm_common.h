@@ -261,7 +261,8 @@ struct Records {
     double* sf;      /* solid fraction */
     int32_t* id;     /* particle tag */
     double* us;      /* 3*cap: particle velocity at the cell, component-major */
-    double* F;       /* 3*cap: hydrodynamic force of the last collision */
+    double* F;       /* 3*cap: hydrodynamic force of the last collision (FP64 storage; with
+                      * FP32/FP16 storage the first 3*cap floats of the buffer hold it) */
     int32_t* cand;   /* candidate entry of the setter this step, -1 = none */
 };
 
@@ -477,7 +478,7 @@ int launch_collide_stream(lbdem_psm* h);
 int launch_init_equilibrium(lbdem_psm* h, const double* d_rho, const double* d_u);
 int launch_pop_convert(lbdem_psm* h, double* d_f64, bool to_device_store);
 int launch_pop_convert_buf(lbdem_psm* h, void* buf, double* d_f64, bool to_device_store);
-int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, double* rF);
+int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, void* rF);
 int diff_free(lbdem_psm* h);
 int periodic_pressure_pre(lbdem_psm* h);
 int launch_macroscopic(lbdem_psm* h, double* d_rho, double* d_u);

@@ -30,6 +30,7 @@ struct ForceArgs {
     const uint8_t* sflag;
     Records rec;
     int64_t rcap;
+    int f32;               /* per-cell forces stored as float (FP32/FP16 storage) */
     const int32_t* powner;
     const double* xs;      /* positions for the torque arms */
     double* partial;
@@ -56,13 +57,20 @@ __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
     for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) {
         const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
         const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
-        int e = -1;
         const int64_t r = valid ? rec_index(X, Y, Z, a.n[1], a.n[2]) : 0;
+        int e = -1;
         if (valid && a.sflag[r]) {
             e = a.rec.cand[r];
-            sF[l][0] = a.rec.F[r];
-            sF[l][1] = a.rec.F[a.rcap + r];
-            sF[l][2] = a.rec.F[2 * a.rcap + r];
+            if (a.f32) {
+                const float* F32 = (const float*)a.rec.F;
+                sF[l][0] = (double)F32[r];
+                sF[l][1] = (double)F32[a.rcap + r];
+                sF[l][2] = (double)F32[2 * a.rcap + r];
+            } else {
+                sF[l][0] = a.rec.F[r];
+                sF[l][1] = a.rec.F[a.rcap + r];
+                sF[l][2] = a.rec.F[2 * a.rcap + r];
+            }
         }
         sE[l] = e;
     }
@@ -75,7 +83,11 @@ __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
         const double xo = a.xs[3 * o], yo = a.xs[3 * o + 1], zo = a.xs[3 * o + 2];
         double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int l = lane; l < BIN_CELLS; l += 64) {
-            if (sE[l] != e) continue;
+            /* an x-slice of the bin with no cell of this entry is skipped by
+             * the whole wavefront */
+            const bool mine = sE[l] == e;
+            if (__ballot(mine) == 0ull) continue;
+            if (!mine) continue;
             const double dx = min_image((double)(bx * BIN + (l >> 6)) - xo, a.n[0]);
             const double dy = min_image((double)(by * BIN + ((l >> 3) & 7)) - yo, a.n[1]);
             const double dz = min_image((double)(bz * BIN + (l & 7)) - zo, a.n[2]);
@@ -142,7 +154,13 @@ __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_
         const double dx = min_image((double)X - a.xs[3 * o], a.n[0]);
         const double dy = min_image((double)Y - a.xs[3 * o + 1], a.n[1]);
         const double dz = min_image((double)Z - a.xs[3 * o + 2], a.n[2]);
-        const double fx = a.rec.F[r], fy = a.rec.F[a.rcap + r], fz = a.rec.F[2 * a.rcap + r];
+        double fx, fy, fz;
+        if (a.f32) {
+            const float* F32 = (const float*)a.rec.F;
+            fx = F32[r]; fy = F32[a.rcap + r]; fz = F32[2 * a.rcap + r];
+        } else {
+            fx = a.rec.F[r]; fy = a.rec.F[a.rcap + r]; fz = a.rec.F[2 * a.rcap + r];
+        }
         atomicAdd(&pft[6 * o + 0], fx);
         atomicAdd(&pft[6 * o + 1], fy);
         atomicAdd(&pft[6 * o + 2], fz);
@@ -163,6 +181,7 @@ int run_force_sum(lbdem_psm* h) {
     a.sflag = h->sflag;
     a.rec = h->rec;
     a.rcap = h->rec_cap;
+    a.f32 = h->precision == LBDEM_PSM_FP64 ? 0 : 1;
     a.powner = h->powner;
     a.xs = h->psum_x;
     a.partial = h->partial;
