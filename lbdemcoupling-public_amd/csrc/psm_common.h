@@ -273,6 +273,7 @@ struct Cand {
     double v[3];
     double w[3];
     double rp2, rm2, rsq;
+    double ri2, ro2;    /* squared radii of the exact no-count band (sub-cell rule) */
     int lo[3], hi[3];   /* clipped bounding box (cells) */
     int id;
     int e;              /* entry index */

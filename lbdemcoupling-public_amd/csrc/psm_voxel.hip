@@ -206,10 +206,21 @@ __global__ __launch_bounds__(256) void k_owner_entries(VoxArgs a, const int32_t*
 
 /* Candidate data in key order, one entry per thread: the image box and the
  * squared radii used by the sub-cell test. */
+/* Squared radii of the exact no-count band of coop_solid_fraction: every
+ * sub-cell sample lies within e of the cell centre, so a centre within
+ * r - e - tol (beyond r + e + tol) of the sphere centre has all samples
+ * inside (outside); tol is far above the sample test's rounding. */
+__device__ __forceinline__ void exact_band(double r, double e, double& ri2, double& ro2) {
+    const double tol = 1e-9 * (1.0 + r);
+    const double ri = r - e - tol, ro = r + e + tol;
+    ri2 = ri > 0.0 ? ri * ri : -1.0;
+    ro2 = ro * ro;
+}
+
 __global__ __launch_bounds__(256) void k_cand_gather(VoxArgs a, const uint32_t* __restrict__ key,
                                                      const double* __restrict__ pv,
                                                      const double* __restrict__ pw,
-                                                     const int32_t* __restrict__ pid, double h,
+                                                     const int32_t* __restrict__ pid, double h, double esamp,
                                                      const int32_t* __restrict__ bin_start,
                                                      const int32_t* __restrict__ bin_count,
                                                      int64_t nbins, Cand* __restrict__ out, int64_t cap) {
@@ -239,6 +250,7 @@ __global__ __launch_bounds__(256) void k_cand_gather(VoxArgs a, const uint32_t* 
         cd.rp2 = r_p * r_p;
         cd.rm2 = r_m * r_m;
         cd.rsq = r * r;
+        exact_band(r, esamp, cd.ri2, cd.ro2);
         cd.id = pid[q];
         cd.e = (int)e;
         out[e] = cd;
@@ -369,31 +381,21 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
     return sf;
 }
 
-/* Squared radii of the exact no-count band of coop_solid_fraction. */
-__device__ __forceinline__ void exact_band(double r, double e, double& ri2, double& ro2) {
-    const double tol = 1e-9 * (1.0 + r);
-    const double ri = r - e - tol, ro = r + e + tol;
-    ri2 = ri > 0.0 ? ri * ri : -1.0;
-    ro2 = ro * ro;
-}
-
 /* The voxelization rule: NT >= 0 the reference's sub-cell count (NT = N
  * when specialized, 0 = runtime N), NT < 0 the analytic overlap of the
  * differentiable mode (per lane, no wavefront cooperation).  Called by all
  * 64 lanes together. */
 template <int NT>
 __device__ __forceinline__ double solid_fraction(bool active, double dx, double dy, double dz,
-                                                 double rp2, double rm2, double rsq, double r, int Nrt,
-                                                 double sw, double frac, double rc, double vc, double esamp,
-                                                 int& n, int& br) {
+                                                 double rp2, double rm2, double rsq, double r, double ri2,
+                                                 double ro2, int Nrt, double sw, double frac, double rc,
+                                                 double vc, int& n, int& br) {
     if constexpr (NT < 0) {
         n = 0;
         br = 0;
         if (!active) return 0.0;
         return overlap_fraction(sqrt((dx * dx + dy * dy) + dz * dz), r, rc, vc, nullptr, &br);
     } else {
-        double ri2, ro2;
-        exact_band(r, esamp, ri2, ro2);
         return coop_solid_fraction<NT>(active, dx, dy, dz, rp2, rm2, rsq, ri2, ro2, Nrt, sw, frac, n, br);
     }
 }
@@ -439,9 +441,11 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     {
         for (int j = 0; j < ncand; ++j) {
             const Cand& cd = cands[j];
-            /* whole-wavefront rejection against the candidate's box */
-            if (wX < cd.lo[0] || wX > cd.hi[0] || y0 + BIN - 1 < cd.lo[1] || y0 > cd.hi[1] ||
-                z0 + BIN - 1 < cd.lo[2] || z0 > cd.hi[2])
+            /* whole-wavefront rejection against the candidate's box; the six
+             * bounds are read together (no short circuit between the loads) */
+            const int lo0 = cd.lo[0], lo1 = cd.lo[1], lo2 = cd.lo[2];
+            const int hi0 = cd.hi[0], hi1 = cd.hi[1], hi2 = cd.hi[2];
+            if ((wX < lo0) | (wX > hi0) | (y0 + BIN - 1 < lo1) | (y0 > hi1) | (z0 + BIN - 1 < lo2) | (z0 > hi2))
                 continue;
             const bool inb = valid && X >= cd.lo[0] && X <= cd.hi[0] && Y >= cd.lo[1] && Y <= cd.hi[1] &&
                              Z >= cd.lo[2] && Z <= cd.hi[2];
@@ -459,8 +463,8 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
                 if (__ballot(near || mine) == 0ull) continue;
             }
             int nn, br;
-            const double sf = solid_fraction<NT>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, a.N, a.sw,
-                                                 a.frac, a.rc, a.vc, a.esamp, nn, br);
+            const double sf = solid_fraction<NT>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, cd.ri2, cd.ro2,
+                                                 a.N, a.sw, a.frac, a.rc, a.vc, nn, br);
             if (!inb) continue;
             const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (ssf > SOLFRAC_MIN ? 1 : 0);
             bool set = false, zero = false;
@@ -540,8 +544,10 @@ __global__ __launch_bounds__(256) void k_subcell(int64_t n, const double* __rest
     const double r_p = rr + h, r_m = rr - h;
     const double dx = act ? d3[3 * k] : 0.0, dy = act ? d3[3 * k + 1] : 0.0, dz = act ? d3[3 * k + 2] : 0.0;
     int nn, bb;
-    const double v = solid_fraction<NT>(act, dx, dy, dz, r_p * r_p, r_m * r_m, rr * rr, rr, N, sw, frac, rc, vc,
-                                        esamp, nn, bb);
+    double ri2, ro2;
+    exact_band(rr, esamp, ri2, ro2);
+    const double v = solid_fraction<NT>(act, dx, dy, dz, r_p * r_p, r_m * r_m, rr * rr, rr, ri2, ro2, N, sw, frac,
+                                        rc, vc, nn, bb);
     if (act) {
         sf[k] = v;
         cnt[k] = nn;
@@ -601,8 +607,10 @@ int run_voxelize(lbdem_psm* h) {
     if (h->np > 0)
         hipLaunchKernelGGL(k_owner_entries, dim3(pg), dim3(256), 0, h->stream, va, h->bin_cand_start, h->bin_cand_count, h->cand_key, h->p_ent_start, h->owner_ent, h->ent_cap);
 
+    /* farthest sub-cell sample from the cell centre, sqrt(3) (1/2 - 1/(2N)) */
+    const double esamp = std::sqrt(3.0) * (0.5 - 0.5 / (double)h->desc.slices_per_dim) + 1e-12;
     hipLaunchKernelGGL(k_cand_gather, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, (h->ent_cap + 255) / 256))),
-                       dim3(256), 0, h->stream, va, h->cand_key, h->pv, h->pw, h->pid, h->sqrt3half,
+                       dim3(256), 0, h->stream, va, h->cand_key, h->pv, h->pw, h->pid, h->sqrt3half, esamp,
                        h->bin_cand_start, h->bin_cand_count, nbins, h->cand_data, h->ent_cap);
     uint32_t* cnt_new = h->bin_any[1 - h->cur_any];
     PSM_HIP(h, hipMemsetAsync(cnt_new, 0, nbins * sizeof(uint32_t), h->stream));
@@ -614,7 +622,7 @@ int run_voxelize(lbdem_psm* h) {
     fa.frac = 1.0 / (double)(fa.N * fa.N * fa.N);
     fa.rc = h->cell_rc;
     fa.vc = h->cell_vc;
-    fa.esamp = std::sqrt(3.0) * (0.5 - 0.5 / (double)fa.N) + 1e-12;
+    fa.esamp = esamp;
     const bool overlap = h->desc.voxel_mode == LBDEM_PSM_VOXEL_OVERLAP;
     const bool init_vel = (h->flags & LBDEM_PSM_INIT_VEL) != 0;
     fa.bin_cand_start = h->bin_cand_start;
