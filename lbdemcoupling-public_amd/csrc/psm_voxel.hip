@@ -263,7 +263,6 @@ struct FoldArgs {
     int N;
     double sw, frac;
     double rc, vc;             /* cell sphere of the analytic overlap */
-    double esamp;              /* sqrt(3) (1/2 - 1/(2N)): farthest sub-cell sample from the centre */
     const int32_t* bin_cand_start;
     const int32_t* bin_cand_count;
     const Cand* cand;
@@ -622,7 +621,6 @@ int run_voxelize(lbdem_psm* h) {
     fa.frac = 1.0 / (double)(fa.N * fa.N * fa.N);
     fa.rc = h->cell_rc;
     fa.vc = h->cell_vc;
-    fa.esamp = esamp;
     const bool overlap = h->desc.voxel_mode == LBDEM_PSM_VOXEL_OVERLAP;
     const bool init_vel = (h->flags & LBDEM_PSM_INIT_VEL) != 0;
     fa.bin_cand_start = h->bin_cand_start;
