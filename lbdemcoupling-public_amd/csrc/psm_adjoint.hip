@@ -391,7 +391,9 @@ __global__ __launch_bounds__(256) void k_list_records(const uint8_t* __restrict_
                                                       int n2, int nb1, int nb2, int32_t* __restrict__ list,
                                                       int32_t* __restrict__ rpos,
                                                       unsigned long long* __restrict__ count, int64_t cap) {
-    const int64_t b = blockIdx.x;
+    /* neighbouring bins (z fastest) share the lines of the cell-ordered
+     * arrays: keep them on one XCD's L2 */
+    const int64_t b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     if (bin_live[b] == 0u) return;
     const int bz = (int)(b % nb2);
     const int by = (int)((b / nb2) % nb1);

@@ -250,22 +250,32 @@ __device__ __forceinline__ void psm_blend_fast(const float* fp, float* fo, bool 
     const float onePlus = 1.0f + rhoB;
     const float uP[3] = {us[0] * onePlus, us[1] * onePlus, us[2] * onePlus};
     const float uPSqr = norm_sqr<float>(uP);
-    EqCoef ce, cs;
-    ce.init(1.0f, rhoB, p, jSqr);
-    cs.init(1.0f, rhoB, p, uPSqr);
+    /* Pair (i, o = i+9) with cj = c_i.j, cu = c_i.uP, per weight class W:
+     * E_i/o = base_j +- K1 cj + K2 cj^2, S_i/o = base_u +- K1 cu + K2 cu^2
+     * (K1 = 3W, K2 = 3pW, base = W (rhoB - p |.|^2)).  The solid terms
+     *   bb  = B ((f_o - E_o) - (f_i - S_i)) = B (sym + asym),
+     *   bbo = B ((f_i - E_i) - (f_o - S_o)) = B (sym - asym),
+     * with asym = (f_o - f_i) + K1 (cj + cu) and
+     * sym = (base_u - base_j) + K2 (cu - cj)(cu + cj); bb - bbo = 2 B asym. */
+    const float w1 = 1.0f / 18.0f, w2 = 1.0f / 36.0f;
+    const float dq = p * (jSqr - uPSqr);          /* (base_u - base_j) / W */
+    const float twoB = B + B;
     F[0] = F[1] = F[2] = 0.0f;
-    fo[0] = fmaf(oneMinB, fo[0] - fp[0], fp[0]) + B * (ce.base0 - cs.base0);
+    /* rest population: B (E_0 - S_0) = -B W0 dq */
+    fo[0] = fmaf(oneMinB, fo[0] - fp[0], fp[0]) - B * ((1.0f / 3.0f) * dq);
 #pragma unroll
     for (int i = 1; i <= 9; ++i) {
         const int o = i + 9;
-        float Ei, Eo, Si, So;
-        ce.pair(i, cdot<float>(i, jB), Ei, Eo);
-        cs.pair(i, cdot<float>(i, uP), Si, So);
-        const float bb = B * ((fp[o] - Eo) - (fp[i] - Si));
-        const float bbo = B * ((fp[i] - Ei) - (fp[o] - So));
-        fo[i] = fmaf(oneMinB, fo[i] - fp[i], fp[i]) + bb;
-        fo[o] = fmaf(oneMinB, fo[o] - fp[o], fp[o]) + bbo;
-        const float diff = bb - bbo;
+        const float W = i <= 3 ? w1 : w2;
+        const float K1 = 3.0f * W, K2 = K1 * p, dbase = W * dq;
+        const float cj = cdot<float>(i, jB), cu = cdot<float>(i, uP);
+        const float sum = cj + cu;
+        const float sym = fmaf(K2, (cu - cj) * sum, dbase);
+        const float asym = fmaf(K1, sum, fp[o] - fp[i]);
+        /* f + (1 - B)(fB - f): exactly f for a full cell (B = 1) */
+        fo[i] = fmaf(oneMinB, fo[i] - fp[i], fp[i]) + B * (sym + asym);
+        fo[o] = fmaf(oneMinB, fo[o] - fp[o], fp[o]) + B * (sym - asym);
+        const float diff = twoB * asym;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (cc(i, d) == 1) F[d] = F[d] - diff;

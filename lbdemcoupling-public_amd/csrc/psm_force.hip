@@ -46,7 +46,9 @@ __device__ __forceinline__ double wave_sum(double v) {
 __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
     __shared__ double sF[BIN_CELLS][3];
     __shared__ int sE[BIN_CELLS];
-    const int64_t b = blockIdx.x;
+    /* neighbouring bins (z fastest) share the lines of the cell-ordered
+     * arrays: keep them on one XCD's L2 */
+    const int64_t b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int ncand = a.bin_cand_count[b];
     if (ncand == 0) return;
     const int cstart = a.bin_cand_start[b];

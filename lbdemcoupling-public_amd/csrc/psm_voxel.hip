@@ -408,7 +408,9 @@ enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
  * back in place. */
 template <int NT, bool INITVEL>
 __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
-    const int64_t b = blockIdx.x;
+    /* neighbouring bins (z fastest) share the lines of the cell-ordered
+     * arrays: keep them on one XCD's L2 */
+    const int64_t b = xcd_block((int)blockIdx.x, (int)gridDim.x);
     const int ncand = a.bin_cand_count[b];
     if (ncand == 0 && a.cnt_old[b] == 0u) return;   /* nothing here, nothing was here */
     const int cstart = a.bin_cand_start[b];
