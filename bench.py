@@ -325,21 +325,71 @@ def cpu_baseline(wl, spheres, nsteps):
         o.collide_stream(1, threads)
         o.sum_force_torque(x)
     dt = time.perf_counter() - t0
+    # one more step on a single core, for the per-core figure
+    t1 = time.perf_counter()
+    x = L.advance_positions(x, v, n, wl["periodic"])
+    o.set_spheres(x, v, w, r, ids)
+    o.collide_stream(1, 1)
+    o.sum_force_torque(x)
+    dt1 = time.perf_counter() - t1
     o.close()
     return {"value": nc * nsteps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "single_core_value": round(nc / dt1 / 1e6, 2),
             "sample": f"{nsteps} full coupled FP64 steps (voxelize + collide/stream + force sum) of the "
                       f"same {n[0]}x{n[1]}x{n[2]} workload by the CPU restatement of the reference "
                       f"algorithm; collide/stream on {threads} OpenMP threads, voxelization and force "
-                      f"sum single-threaded as in the reference; {dt:.1f} s"}
+                      f"sum single-threaded as in the reference; {dt:.1f} s (single_core_value: one "
+                      f"more step on one core, {dt1:.1f} s)"}
+
+
+def stream_copy_gbs(nbytes=2 << 30, reps=10):
+    """Device-to-device hipMemcpy of nbytes, timed with HIP events: a copy
+    ceiling (read + write bytes per second) for context."""
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        return None
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
+    hip.hipFree.argtypes = [vp]
+    hip.hipMemset.argtypes = [vp, ctypes.c_int, sz]
+    hip.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
+    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
+    hip.hipEventRecord.argtypes = [vp, vp]
+    hip.hipEventSynchronize.argtypes = [vp]
+    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
+    hip.hipEventDestroy.argtypes = [vp]
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
+        return None
+    try:
+        hip.hipMemset(a, 0, nbytes)
+        hip.hipEventCreate(ctypes.byref(e0))
+        hip.hipEventCreate(ctypes.byref(e1))
+        for _ in range(2):
+            hip.hipMemcpyAsync(b, a, nbytes, 3, None)
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(b, a, nbytes, 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        ms = ctypes.c_float()
+        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        return round(2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9, 1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
 
 
 TRAFFIC_JSON = os.path.join(ROOT, "profiles", "collide_stream_traffic.json")
 PROBE_CMD = ("rocprofv3 --pmc {ctr} --output-format csv -d DIR/{tag} -o run -- "
              "python3 bench.py --kernel-probe")
 TRACE_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace -o run -- "
-             "python3 bench.py --steps 10 --warmup 2 --no-extra --no-adjoint --cpu-steps 1")
+             "python3 bench.py --steps 10 --warmup 2 --repeats 1 --no-extra --no-adjoint --cpu-steps 1")
 TRACE512_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace512 -o run -- "
-                "python3 bench.py --workload channel512 --steps 4 --warmup 1 --no-adjoint --cpu-steps 0")
+                "python3 bench.py --workload channel512 --steps 4 --warmup 1 --repeats 1 --no-adjoint --cpu-steps 0")
 
 
 def _short(name):
@@ -453,6 +503,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--repeats", type=int, default=5,
+                    help="repetitions of the FP64 line (each: warmup + steps); value is their median")
     ap.add_argument("--workload", default="dense256", choices=sorted(WORKLOADS))
     ap.add_argument("--no-extra", action="store_true", help="skip the 512^3-class FP32 line")
     ap.add_argument("--cpu-steps", type=int, default=4)
@@ -481,11 +533,21 @@ def main():
         kernel_probe(wl, sph)
         return
     res = {}
-    for prec in ("fp64", "fp32_dev", "fp16_store"):
+    runs64 = []
+    for k in range(max(1, args.repeats)):
+        log(f"fp64 run {k + 1}/{max(1, args.repeats)}: {args.warmup} warmup + {args.steps} timed steps")
+        runs64.append(run_gpu(wl, "fp64", args.steps, args.warmup, sph, want_field=(k == 0)))
+        log(f"fp64: {runs64[-1]['mlups']:.0f} MLUPS full step, kernel {runs64[-1]['kernel_gbs']:.0f} GB/s")
+    # the median repetition; fields and forces (identical in every repetition) from the first
+    med = sorted(runs64, key=lambda rr: rr["mlups"])[len(runs64) // 2]
+    res["fp64"] = dict(med, u=runs64[0].get("u"), F=runs64[0]["F"], T=runs64[0]["T"])
+    for prec in ("fp32_dev", "fp16_store"):
         log(f"{prec}: {args.warmup} warmup + {args.steps} timed steps")
         res[prec] = run_gpu(wl, prec, args.steps, args.warmup, sph, want_field=True)
         log(f"{prec}: {res[prec]['mlups']:.0f} MLUPS full step, kernel {res[prec]['kernel_gbs']:.0f} GB/s")
     f64 = res["fp64"]
+    copy_gbs = stream_copy_gbs()
+    log(f"device copy: {copy_gbs} GB/s")
 
     def summary(rr, ref=f64):
         return {"mlups": round(rr["mlups"], 1), "ms_per_step": round(rr["ms_per_step"], 4),
@@ -571,6 +633,9 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "repeats": max(1, args.repeats),
+        "value_is": "median over the repetitions of the FP64 line (each times exactly `steps` steps)",
+        "fp64_runs_mlups": [round(rr["mlups"], 1) for rr in runs64],
         "ms_per_step": round(f64["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -586,7 +651,9 @@ def main():
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a known-size copy; "
                                        "profiles/collide_stream_dense256_hbm_traffic.md",
                      "kernel": "k_collide_stream<FP64>",
-                     "algorithmic_bytes_per_launch": 304 * nc},
+                     "algorithmic_bytes_per_launch": 304 * nc,
+                     "device_copy_gbs": copy_gbs,
+                     "device_copy": "hipMemcpy device to device of 2 GiB, read + write bytes per second"},
         "cpu_baseline": cpu,
         "fp64": summary(f64),
         "fp32_dev": summary(res["fp32_dev"]),
