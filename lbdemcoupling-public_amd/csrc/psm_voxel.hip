@@ -332,19 +332,65 @@ __device__ __forceinline__ double readlane_d(double v, int L) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+/* Sub-cell counts of the pending lanes of one fold wavefront (N = 8).  The
+ * wavefront is one x-slice of a bin with lane = 8 y + z, so the offset dx
+ * and the radius are the same on every lane, and lanes of one z-row share
+ * dz bit for bit.  The pending lanes are taken z-row by z-row: the row's
+ * eight dz_sq[k] are formed once (wave-uniform); each query then forms
+ * dx_sq[i] + dy_sq[j] on lane 8 i + j and tests the eight k.  Every sample
+ * test is the same FP64 expression as wave_count's,
+ * (dx_sq[i] + dy_sq[j]) + dz_sq[k] < r^2, so the counts are the same. */
+__device__ __forceinline__ void slice_counts8(unsigned long long pend, double dx, double dy, double dz,
+                                              double rsq, double sw, double frac, int& n, double& sf) {
+    const int lane = (int)__lane_id();
+    const double di = -0.5 + ((double)(lane >> 3) + 0.5) * sw;
+    const double dj = -0.5 + ((double)(lane & 7) + 0.5) * sw;
+    const double ax = readlane_d(dx, __ffsll(pend) - 1) + di;
+    const double xsq = ax * ax;
+    while (pend) {
+        const int L = __ffsll(pend) - 1;
+        const unsigned long long row = pend & (0x0101010101010101ull << (L & 7));
+        pend &= ~row;
+        const double bz = readlane_d(dz, L);
+        double zsq[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const double az = bz + (-0.5 + ((double)k + 0.5) * sw);
+            zsq[k] = az * az;
+        }
+        unsigned long long q = row;
+        while (q) {
+            const int M = __ffsll(q) - 1;
+            q &= q - 1;
+            const double ay = readlane_d(dy, M) + dj;
+            const double sxy = xsq + ay * ay;
+            unsigned cnt = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) cnt += (unsigned)__popcll(__ballot(sxy + zsq[k] < rsq));
+            if (lane == M) { n = (int)cnt; sf = frac * (double)cnt; }
+        }
+    }
+}
+
 /* calcSolidFraction (src/ibProcessors3D.hh:116-158) for one query per lane,
  * called by all 64 lanes together: the two distance shortcuts per lane, then
- * every lane that needs a count gets it from wave_count in turn.
+ * every lane that needs a count gets it from wave_count in turn (SLICE: from
+ * slice_counts8, whose lane layout the caller guarantees).
  *
  * Between the reference's shortcut radii r -+ sqrt(3.1)/2, a cell whose
- * centre lies within ri (resp. beyond ro) of the sphere centre has every
- * sub-cell sample strictly inside (resp. outside) the sphere: the samples
- * lie within e = sqrt(3) (1/2 - 1/(2N)) of the cell centre, and
- * ri = r - e - tol, ro = r + e + tol with a margin tol far above the
- * rounding error of the sample test.  Their count (N^3, resp. 0) and
- * solid fraction (fraction * count) are then known without counting; the
- * result is the same as the full count's, bit for bit. */
-template <int NT>
+ * sub-cell samples all lie strictly inside (resp. outside) the sphere has
+ * count N^3 (resp. 0) and solid fraction fraction * count, known without
+ * counting.  Two bounds find such cells, each with a margin tol far above
+ * the rounding error of the sample test, so the result is the same as the
+ * full count's, bit for bit:
+ *  - the samples lie within e = sqrt(3) (1/2 - 1/(2N)) of the cell centre:
+ *    a centre within ri = r - e - tol (beyond ro = r + e + tol) of the
+ *    sphere centre settles the cell;
+ *  - tighter, the samples lie in the box of half-width s = 1/2 - 1/(2N)
+ *    about the cell centre: the box's farthest point from the sphere centre
+ *    inside r (nearest point outside r) settles it.  With the box, exactly
+ *    the cells the sphere surface cuts are left to count. */
+template <int NT, bool SLICE>
 __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, double dy, double dz,
                                                       double rp2, double rm2, double rsq, double ri2,
                                                       double ro2, int Nrt, double sw, double frac, int& n,
@@ -367,7 +413,24 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
             else need = true;
         }
     }
+    if (__ballot(need) != 0ull && need) {
+        const double s = -0.5 + ((double)(N - 1) + 0.5) * sw;   /* outermost sample offset */
+        const double tol = 1e-9 * (1.0 + rsq);
+        const double ex = fabs(dx), ey = fabs(dy), ez = fabs(dz);
+        const double fx = ex + s, fy = ey + s, fz = ez + s;
+        const double nx = fmax(ex - s, 0.0), ny = fmax(ey - s, 0.0), nz = fmax(ez - s, 0.0);
+        if ((fx * fx + fy * fy) + fz * fz < rsq - tol) {
+            n = N * N * N; sf = frac * (double)(N * N * N); need = false;
+        } else if ((nx * nx + ny * ny) + nz * nz > rsq + tol) {
+            n = 0; sf = frac * 0.0; need = false;
+        }
+    }
     unsigned long long pend = __ballot(need);
+    if constexpr (SLICE) {
+        static_assert(NT == 8, "one sample column per lane");
+        if (pend) slice_counts8(pend, dx, dy, dz, rsq, sw, frac, n, sf);
+        return sf;
+    }
     const int lane = (int)__lane_id();
     while (pend) {
         const int L = __ffsll(pend) - 1;
@@ -384,7 +447,7 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
  * when specialized, 0 = runtime N), NT < 0 the analytic overlap of the
  * differentiable mode (per lane, no wavefront cooperation).  Called by all
  * 64 lanes together. */
-template <int NT>
+template <int NT, bool SLICE = false>
 __device__ __forceinline__ double solid_fraction(bool active, double dx, double dy, double dz,
                                                  double rp2, double rm2, double rsq, double r, double ri2,
                                                  double ro2, int Nrt, double sw, double frac, double rc,
@@ -395,7 +458,7 @@ __device__ __forceinline__ double solid_fraction(bool active, double dx, double 
         if (!active) return 0.0;
         return overlap_fraction(sqrt((dx * dx + dy * dy) + dz * dz), r, rc, vc, nullptr, &br);
     } else {
-        return coop_solid_fraction<NT>(active, dx, dy, dz, rp2, rm2, rsq, ri2, ro2, Nrt, sw, frac, n, br);
+        return coop_solid_fraction<NT, SLICE>(active, dx, dy, dz, rp2, rm2, rsq, ri2, ro2, Nrt, sw, frac, n, br);
     }
 }
 
@@ -464,7 +527,7 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
                 if (__ballot(near || mine) == 0ull) continue;
             }
             int nn, br;
-            const double sf = solid_fraction<NT>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, cd.ri2, cd.ro2,
+            const double sf = solid_fraction<NT, NT == 8>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, cd.ri2, cd.ro2,
                                                  a.N, a.sw, a.frac, a.rc, a.vc, nn, br);
             if (!inb) continue;
             const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (ssf > SOLFRAC_MIN ? 1 : 0);
