@@ -314,6 +314,14 @@ __device__ __forceinline__ int xcd_block(int i, int G) {
     return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
 
+/* Broadcast of lane L's double (L wave-uniform) into scalar registers. */
+__device__ __forceinline__ double readlane_d(double v, int L) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, L);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), L);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 struct PopLayout {
     int zc;
     int zsh;

@@ -35,6 +35,7 @@ struct ForceArgs {
     const double* xs;      /* positions for the torque arms */
     double* partial;
     int64_t ent_cap;
+    int64_t nbins;
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -43,70 +44,127 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+/* Sum of v[0..7] over the 64 lanes of a wavefront, reduce-scatter style:
+ * three halving exchanges (lane bits 5, 4, 3) leave each lane one value,
+ * index m = 4 b5 + 2 b4 + b3 of the lane, summed over the eight lanes that
+ * share those bits; three butterflies over bits 2, 1, 0 finish it.  Ten
+ * shuffles of a double in place of 48, and the order of every addition is
+ * fixed (a + b and b + a round alike), so the sum is deterministic. */
+__device__ __forceinline__ double wave_sum8_scatter(const double (&v)[8], int lane) {
+    double w4[4], w2[2];
+    const bool u5 = (lane & 32) != 0, u4 = (lane & 16) != 0, u3 = (lane & 8) != 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const double keep = u5 ? v[4 + m] : v[m];
+        const double send = u5 ? v[m] : v[4 + m];
+        w4[m] = keep + __shfl_xor(send, 32, 64);
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const double keep = u4 ? w4[2 + m] : w4[m];
+        const double send = u4 ? w4[m] : w4[2 + m];
+        w2[m] = keep + __shfl_xor(send, 16, 64);
+    }
+    const double keep = u3 ? w2[1] : w2[0];
+    const double send = u3 ? w2[0] : w2[1];
+    double w = keep + __shfl_xor(send, 8, 64);
+    w += __shfl_xor(w, 4, 64);
+    w += __shfl_xor(w, 2, 64);
+    w += __shfl_xor(w, 1, 64);
+    return w;
+}
+
+/* One wavefront per bin, four bins per workgroup.  Lane (y, z) of the bin
+ * holds its eight cells (x, y, z), x = 0..7, in registers: the liveness
+ * flag, the candidate entry that set the record and the force.  The
+ * particle positions of the bin's candidate entries are fetched one entry
+ * per lane, overlapping the cell loads.  Then, entry by entry, each lane
+ * adds F and d x F of its cells set by that entry (in x order) and the
+ * wavefront reduces the six sums with wave_sum8_scatter. */
+template <typename FT>
 __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
-    __shared__ double sF[BIN_CELLS][3];
-    __shared__ int sE[BIN_CELLS];
+    const int lane = (int)(threadIdx.x & 63);
     /* neighbouring bins (z fastest) share the lines of the cell-ordered
-     * arrays: keep them on one XCD's L2 */
-    const int64_t b = xcd_block((int)blockIdx.x, (int)gridDim.x);
+     * arrays: keep them in one workgroup and on one XCD's L2 */
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t b = (int64_t)xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + wv;
+    if (b >= a.nbins) return;
     const int ncand = a.bin_cand_count[b];
     if (ncand == 0) return;
     const int cstart = a.bin_cand_start[b];
     if ((int64_t)cstart + ncand > a.ent_cap) return;
-    const int bz = (int)(b % a.nb[2]);
-    const int by = (int)((b / a.nb[2]) % a.nb[1]);
-    const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
-    for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) {
-        const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
-        const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
-        const int64_t r = valid ? rec_index(X, Y, Z, a.n[1], a.n[2]) : 0;
-        int e = -1;
-        if (valid && a.sflag[r]) {
-            e = a.rec.cand[r];
-            if (a.f32) {
-                const float* F32 = (const float*)a.rec.F;
-                sF[l][0] = (double)F32[r];
-                sF[l][1] = (double)F32[a.rcap + r];
-                sF[l][2] = (double)F32[2 * a.rcap + r];
-            } else {
-                sF[l][0] = a.rec.F[r];
-                sF[l][1] = a.rec.F[a.rcap + r];
-                sF[l][2] = a.rec.F[2 * a.rcap + r];
+    const int bi = (int)b;
+    const int bz = bi % a.nb[2];
+    const int by = (bi / a.nb[2]) % a.nb[1];
+    const int bx = bi / (a.nb[2] * a.nb[1]);
+    const int Y = by * BIN + (lane >> 3), Z = bz * BIN + (lane & 7);
+    const bool yz = Y < a.n[1] && Z < a.n[2];
+    /* Three rounds of loads, each issued at once: the eight liveness
+     * flags and the bin's candidate keys (one entry per lane); the records
+     * of the live cells and the entries' owners; the owners' positions.
+     * The empty asm statements pin the loaded values in registers after
+     * the last round, so no conversion or test waits inside a round. */
+    const int64_t plane = (int64_t)a.n[1] * a.n[2];
+    const int64_t lo = yz ? (int64_t)Y * a.n[2] + Z : 0;
+    const int tl = lane < ncand ? lane : 0;
+    const uint32_t kk = a.key[cstart + tl];
+    int fl[BIN];
+#pragma unroll
+    for (int x = 0; x < BIN; ++x) {
+        const int X = bx * BIN + x;
+        const int64_t xb = (int64_t)(X < a.n[0] ? X : a.n[0] - 1) * plane;
+        fl[x] = a.sflag[xb + lo];
+        if (!yz || X >= a.n[0]) fl[x] = 0;
+    }
+    const int po = a.powner[(int)(kk & QMASK)];
+    const FT* __restrict__ F = (const FT*)a.rec.F;
+    int ce[BIN];
+    FT fx[BIN], fy[BIN], fz[BIN];
+#pragma unroll
+    for (int x = 0; x < BIN; ++x) {
+        ce[x] = -1;
+        fx[x] = fy[x] = fz[x] = (FT)0;
+        if (fl[x]) {
+            const int64_t r = (int64_t)(bx * BIN + x) * plane + lo;
+            ce[x] = a.rec.cand[r];
+            fx[x] = F[r];
+            fy[x] = F[a.rcap + r];
+            fz[x] = F[2 * a.rcap + r];
+        }
+    }
+    double px = a.xs[3 * po], py = a.xs[3 * po + 1], pz = a.xs[3 * po + 2];
+#pragma unroll
+    for (int x = 0; x < BIN; ++x) asm volatile("" : "+v"(ce[x]), "+v"(fx[x]), "+v"(fy[x]), "+v"(fz[x]));
+    asm volatile("" : "+v"(px), "+v"(py), "+v"(pz));
+    const double dy0 = (double)Y, dz0 = (double)Z;
+    for (int t = 0; t < ncand; ++t) {
+        const int e = cstart + t;
+        double xo, yo, zo;
+        if (t < 64) {
+            xo = readlane_d(px, t); yo = readlane_d(py, t); zo = readlane_d(pz, t);
+        } else {
+            const int o = a.powner[(int)(a.key[e] & QMASK)];
+            xo = a.xs[3 * o]; yo = a.xs[3 * o + 1]; zo = a.xs[3 * o + 2];
+        }
+        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        const double dy = min_image(dy0 - yo, a.n[1]);
+        const double dz = min_image(dz0 - zo, a.n[2]);
+#pragma unroll
+        for (int x = 0; x < BIN; ++x) {
+            if (ce[x] == e) {
+                const double dx = min_image((double)(bx * BIN + x) - xo, a.n[0]);
+                const double gx = (double)fx[x], gy = (double)fy[x], gz = (double)fz[x];
+                acc[0] += gx;
+                acc[1] += gy;
+                acc[2] += gz;
+                acc[3] += dy * gz - dz * gy;
+                acc[4] += -dx * gz + dz * gx;
+                acc[5] += dx * gy - dy * gx;
             }
         }
-        sE[l] = e;
-    }
-    __syncthreads();
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int j = wave; j < ncand; j += 4) {
-        const int e = cstart + j;
-        const int q = (int)(a.key[e] & QMASK);
-        const int o = a.powner[q];
-        const double xo = a.xs[3 * o], yo = a.xs[3 * o + 1], zo = a.xs[3 * o + 2];
-        double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-        for (int l = lane; l < BIN_CELLS; l += 64) {
-            /* an x-slice of the bin with no cell of this entry is skipped by
-             * the whole wavefront */
-            const bool mine = sE[l] == e;
-            if (__ballot(mine) == 0ull) continue;
-            if (!mine) continue;
-            const double dx = min_image((double)(bx * BIN + (l >> 6)) - xo, a.n[0]);
-            const double dy = min_image((double)(by * BIN + ((l >> 3) & 7)) - yo, a.n[1]);
-            const double dz = min_image((double)(bz * BIN + (l & 7)) - zo, a.n[2]);
-            const double fx = sF[l][0], fy = sF[l][1], fz = sF[l][2];
-            acc[0] += fx;
-            acc[1] += fy;
-            acc[2] += fz;
-            acc[3] += dy * fz - dz * fy;
-            acc[4] += -dx * fz + dz * fx;
-            acc[5] += dx * fy - dy * fx;
-        }
-#pragma unroll
-        for (int m = 0; m < 6; ++m) acc[m] = wave_sum(acc[m]);
-        if (lane == 0) {
-#pragma unroll
-            for (int m = 0; m < 6; ++m) a.partial[6 * (int64_t)e + m] = acc[m];
-        }
+        const double s = wave_sum8_scatter(acc, lane);
+        const int m = lane >> 3;
+        if ((lane & 7) == 0 && m < 6) a.partial[6 * (int64_t)e + m] = s;
     }
 }
 
@@ -188,7 +246,10 @@ int run_force_sum(lbdem_psm* h) {
     a.xs = h->psum_x;
     a.partial = h->partial;
     a.ent_cap = h->ent_cap;
-    hipLaunchKernelGGL(k_force_bin, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, a);
+    a.nbins = h->nbins;
+    const dim3 grid((unsigned)((h->nbins + 3) / 4));
+    if (a.f32) hipLaunchKernelGGL(k_force_bin<float>, grid, dim3(256), 0, h->stream, a);
+    else hipLaunchKernelGGL(k_force_bin<double>, grid, dim3(256), 0, h->stream, a);
     if (h->np > 0) {
         hipLaunchKernelGGL(k_force_particle, dim3((unsigned)((h->np + 3) / 4)), dim3(256), 0, h->stream,
                            h->np, h->p_ent_start, h->p_nent, h->owner_ent, h->partial, h->pft, h->ent_cap);

@@ -324,14 +324,6 @@ __device__ __forceinline__ unsigned wave_count(double dx, double dy, double dz, 
     return total;
 }
 
-/* Broadcast of lane L's double (L wave-uniform) into scalar registers. */
-__device__ __forceinline__ double readlane_d(double v, int L) {
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, L);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), L);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
-
 /* Sub-cell counts of the pending lanes of one fold wavefront (N = 8).  The
  * wavefront is one x-slice of a bin with lane = 8 y + z, so the offset dx
  * and the radius are the same on every lane, and lanes of one z-row share
