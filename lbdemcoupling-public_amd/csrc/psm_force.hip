@@ -152,8 +152,13 @@ __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
 #pragma unroll
         for (int x = 0; x < BIN; ++x) {
             if (ce[x] == e) {
-                const double dx = min_image((double)(bx * BIN + x) - xo, a.n[0]);
-                const double gx = (double)fx[x], gy = (double)fy[x], gz = (double)fz[x];
+                /* converted here, not hoisted out of the entry loop, so the
+                 * cells stay in registers at their stored width */
+                FT h0 = fx[x], h1 = fy[x], h2 = fz[x];
+                int X = bx * BIN + x;
+                asm volatile("" : "+v"(h0), "+v"(h1), "+v"(h2), "+s"(X));
+                const double dx = min_image((double)X - xo, a.n[0]);
+                const double gx = (double)h0, gy = (double)h1, gz = (double)h2;
                 acc[0] += gx;
                 acc[1] += gy;
                 acc[2] += gz;
