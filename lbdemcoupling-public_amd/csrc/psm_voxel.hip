@@ -388,34 +388,28 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
                                                       double ro2, int Nrt, double sw, double frac, int& n,
                                                       int& br) {
     const int N = NT > 0 ? NT : Nrt;
-    double sf = 0.0;
-    n = 0;
-    br = 0;
-    bool need = false;
-    if (active) {
-        const double dist = dx * dx + dy * dy + dz * dz;
-        if (dist > rp2) {
-            br = 0;
-        } else if (dist < rm2) {
-            br = 1; n = N * N * N; sf = 1.0;
-        } else {
-            br = 2;
-            if (dist < ri2) { n = N * N * N; sf = frac * (double)(N * N * N); }
-            else if (dist > ro2) { n = 0; sf = frac * 0.0; }
-            else need = true;
-        }
-    }
-    if (__ballot(need) != 0ull && need) {
+    const int N3 = N * N * N;
+    /* branch-free: every lane evaluates every test */
+    const double dist = dx * dx + dy * dy + dz * dz;
+    const bool far = dist > rp2;
+    const bool core = !far && dist < rm2;
+    const bool band = active && !far && !core;
+    br = !active || far ? 0 : (core ? 1 : 2);
+    const bool allin = band && dist < ri2;
+    bool need = band && !allin && !(dist > ro2);
+    n = (active && core) || allin ? N3 : 0;
+    double sf = active && core ? 1.0 : (allin ? frac * (double)N3 : 0.0);
+    if (__ballot(need) != 0ull) {
         const double s = -0.5 + ((double)(N - 1) + 0.5) * sw;   /* outermost sample offset */
         const double tol = 1e-9 * (1.0 + rsq);
         const double ex = fabs(dx), ey = fabs(dy), ez = fabs(dz);
         const double fx = ex + s, fy = ey + s, fz = ez + s;
         const double nx = fmax(ex - s, 0.0), ny = fmax(ey - s, 0.0), nz = fmax(ez - s, 0.0);
-        if ((fx * fx + fy * fy) + fz * fz < rsq - tol) {
-            n = N * N * N; sf = frac * (double)(N * N * N); need = false;
-        } else if ((nx * nx + ny * ny) + nz * nz > rsq + tol) {
-            n = 0; sf = frac * 0.0; need = false;
-        }
+        const bool hin = need && (fx * fx + fy * fy) + fz * fz < rsq - tol;
+        const bool hout = (nx * nx + ny * ny) + nz * nz > rsq + tol;
+        n = hin ? N3 : n;
+        sf = hin ? frac * (double)N3 : sf;
+        need = need && !hin && !hout;
     }
     unsigned long long pend = __ballot(need);
     if constexpr (SLICE) {
@@ -521,27 +515,24 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
             int nn, br;
             const double sf = solid_fraction<NT, NT == 8>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, cd.ri2, cd.ro2,
                                                  a.N, a.sw, a.frac, a.rc, a.vc, nn, br);
-            if (!inb) continue;
-            const int flag = (sf > SOLFRAC_MIN ? 1 : 0) + 2 * (ssf > SOLFRAC_MIN ? 1 : 0);
-            bool set = false, zero = false;
-            if (flag == 0) zero = true;
-            else if (flag == 1) set = true;
-            else if (flag == 2) zero = (sid == cd.id);
-            else set = (sf > ssf || sid == cd.id);
-            if (zero) {
-                kind = ST_ZERO; ssf = 0.0; sid = 0; se = -1;
-                sus[0] = sus[1] = sus[2] = 0.0;
-            }
-            if (set) {   /* setValues, src/ibProcessors3D.hh:162-174 */
-                kind = ST_SET;
-                ssf = sf;
-                sid = cd.id;
-                se = cd.e;
-                sus[0] = cd.v[0] + (cd.w[1] * dz - cd.w[2] * dy);
-                sus[1] = cd.v[1] + (-cd.w[0] * dz + cd.w[2] * dx);
-                sus[2] = cd.v[2] + (cd.w[0] * dy - cd.w[1] * dx);
-            }
-            if (INITVEL && sf > SOLFRAC_MAX) {
+            /* the four-case rule, branch-free: flag = new + 2 old;
+             * 0 -> zero, 1 -> set, 2 -> zero if own record, 3 -> set if
+             * larger or own record */
+            const bool pnew = sf > SOLFRAC_MIN, pold = ssf > SOLFRAC_MIN, own = sid == cd.id;
+            const bool zero = inb && !pnew && (!pold || own);
+            const bool set = inb && pnew && (!pold || sf > ssf || own);
+            /* setValues, src/ibProcessors3D.hh:162-174 */
+            const double u0 = cd.v[0] + (cd.w[1] * dz - cd.w[2] * dy);
+            const double u1 = cd.v[1] + (-cd.w[0] * dz + cd.w[2] * dx);
+            const double u2 = cd.v[2] + (cd.w[0] * dy - cd.w[1] * dx);
+            kind = set ? ST_SET : (zero ? ST_ZERO : kind);
+            ssf = set ? sf : (zero ? 0.0 : ssf);
+            sid = set ? cd.id : (zero ? 0 : sid);
+            se = set ? cd.e : (zero ? -1 : se);
+            sus[0] = set ? u0 : (zero ? 0.0 : sus[0]);
+            sus[1] = set ? u1 : (zero ? 0.0 : sus[1]);
+            sus[2] = set ? u2 : (zero ? 0.0 : sus[2]);
+            if (INITVEL && inb && sf > SOLFRAC_MAX) {
                 dv = true;
                 if (kind == ST_OLD) {
                     dvus[0] = a.rec.us[ridx];
