@@ -969,7 +969,7 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
         rc = launch_collide_stream(h);
         if (rc) return rc;
         if (np > 0) {
-            rc = run_force_sum(h);
+            rc = run_force_sum(h, h->psum_x);
             if (rc) return rc;
             PSM_HIP(h, hipMemcpyAsync(D.d_ft + (size_t)t * np * 6, h->pft, np * 6 * sizeof(double),
                                       hipMemcpyDeviceToDevice, h->stream));

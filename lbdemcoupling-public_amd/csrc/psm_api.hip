@@ -504,7 +504,7 @@ int lbdem_psm_sum_force_torque(lbdem_psm* h, int64_t n, const double* x3, double
         if (x3) PSM_HIP(h, hipMemcpy(h->psum_x, x3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
         else PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->px, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     }
-    int rc = run_force_sum(h);
+    int rc = run_force_sum(h, h->psum_x);
     if (rc) return rc;
     return fold_forces_to_host(h, n, F3, T3);
 }
@@ -524,9 +524,7 @@ int lbdem_psm_step(lbdem_psm* h, int32_t nsteps, int32_t flags) {
         if (rc) return rc;
         rc = launch_collide_stream(h);
         if (rc) return rc;
-        if (h->np > 0)
-            PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->px, 3 * h->np * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
-        rc = run_force_sum(h);
+        rc = run_force_sum(h, h->px);
         if (rc) return rc;
     }
     return check_overflow(h);

@@ -507,7 +507,7 @@ int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
 int run_voxelize(lbdem_psm* h);
 int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_r, double* d_sf,
                    int32_t* d_cnt, int32_t* d_br);
-int run_force_sum(lbdem_psm* h);
+int run_force_sum(lbdem_psm* h, const double* xs);   /* xs: positions for the torque arms */
 int count_records(lbdem_psm* h);
 int launch_advance(lbdem_psm* h);
 int launch_define_velocity(lbdem_psm* h);

@@ -235,7 +235,7 @@ __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_
     }
 }
 
-int run_force_sum(lbdem_psm* h) {
+int run_force_sum(lbdem_psm* h, const double* xs) {
     hipEvent_t ev = nullptr;
     prof_begin(h, 2, &ev);
     ForceArgs a;
@@ -248,7 +248,7 @@ int run_force_sum(lbdem_psm* h) {
     a.rcap = h->rec_cap;
     a.f32 = h->precision == LBDEM_PSM_FP64 ? 0 : 1;
     a.powner = h->powner;
-    a.xs = h->psum_x;
+    a.xs = xs;
     a.partial = h->partial;
     a.ent_cap = h->ent_cap;
     a.nbins = h->nbins;

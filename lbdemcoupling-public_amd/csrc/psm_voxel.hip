@@ -621,6 +621,21 @@ __global__ __launch_bounds__(256) void k_advance(int64_t np, double* __restrict_
     }
 }
 
+/* The per-step clears of the voxelization in one launch: counters 0 and
+ * 2..7 (1 holds the overflow flags), the per-bin candidate counts and
+ * fill cursors, and the live-record counts the fold is about to write. */
+__global__ __launch_bounds__(256) void k_clear_step(int64_t* __restrict__ counters, int32_t* __restrict__ bcount,
+                                                    int32_t* __restrict__ bfill, uint32_t* __restrict__ cnt_new,
+                                                    int64_t nbins) {
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i0 < 8 && i0 != 1) counters[i0] = 0;
+    for (int64_t i = i0; i < nbins; i += (int64_t)gridDim.x * blockDim.x) {
+        bcount[i] = 0;
+        bfill[i] = 0;
+        cnt_new[i] = 0u;
+    }
+}
+
 static VoxArgs vox_args(lbdem_psm* h) {
     VoxArgs a;
     for (int d = 0; d < 3; ++d) { a.n[d] = h->g.n[d]; a.per[d] = h->g.periodic[d]; a.nb[d] = h->g.nb[d]; }
@@ -636,10 +651,9 @@ int run_voxelize(lbdem_psm* h) {
     const int64_t nbins = h->nbins;
     VoxArgs va = vox_args(h);
     /* counters[1] (overflow flags) is cleared by the public entry points only */
-    PSM_HIP(h, hipMemsetAsync(h->counters, 0, sizeof(int64_t), h->stream));
-    PSM_HIP(h, hipMemsetAsync(h->counters + 2, 0, 6 * sizeof(int64_t), h->stream));
-    PSM_HIP(h, hipMemsetAsync(h->bin_cand_count, 0, nbins * sizeof(int32_t), h->stream));
-    PSM_HIP(h, hipMemsetAsync(h->bin_fill, 0, nbins * sizeof(int32_t), h->stream));
+    uint32_t* cnt_new = h->bin_any[1 - h->cur_any];
+    hipLaunchKernelGGL(k_clear_step, dim3((unsigned)std::min<int64_t>(2048, std::max<int64_t>(1, (nbins + 255) / 256))),
+                       dim3(256), 0, h->stream, h->counters, h->bin_cand_count, h->bin_fill, cnt_new, nbins);
     const unsigned pg = (unsigned)std::max<int64_t>(1, (h->np + 3) / 4);   /* a wave per particle */
     if (h->np > 0)
         hipLaunchKernelGGL(k_bin_count, dim3(pg), dim3(256), 0, h->stream, va, h->bin_cand_count, h->p_nent);
@@ -659,8 +673,6 @@ int run_voxelize(lbdem_psm* h) {
     hipLaunchKernelGGL(k_cand_gather, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, (h->ent_cap + 255) / 256))),
                        dim3(256), 0, h->stream, va, h->cand_key, h->pv, h->pw, h->pid, h->sqrt3half, esamp,
                        h->bin_cand_start, h->bin_cand_count, nbins, h->cand_data, h->ent_cap);
-    uint32_t* cnt_new = h->bin_any[1 - h->cur_any];
-    PSM_HIP(h, hipMemsetAsync(cnt_new, 0, nbins * sizeof(uint32_t), h->stream));
 
     FoldArgs fa;
     for (int d = 0; d < 3; ++d) { fa.n[d] = h->g.n[d]; fa.nb[d] = h->g.nb[d]; }
