@@ -435,6 +435,8 @@ def summarize_profiles(d):
             f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
             for name, n, tot, avg, pct in rows[:24]:
                 f.write(f"| {_short(name)} | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |\n")
+            f.write("\n`__amd_rocclr_copyBuffer` is the run's device-to-device copy ceiling "
+                    "(hipMemcpy of 2 GiB, the `device_copy_gbs` figure), not part of a step.\n")
     res = {}
     for tag, ctr in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
         path = os.path.join(d, tag, "run_counter_collection.csv")
