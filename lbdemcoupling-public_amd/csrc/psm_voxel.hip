@@ -477,7 +477,7 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     const int y0 = by * BIN, z0 = bz * BIN;
 
     const bool had = valid && a.sflag[cidx] != 0;
-    int kind = had ? ST_OLD : ST_ZERO, sid = 0, se = -1;
+    int kind = had ? ST_OLD : ST_ZERO, sid = 0, se = -1, sj = -1;
     double ssf = 0.0, sus[3] = {0.0, 0.0, 0.0};
     bool dv = false;
     double dvus[3] = {0.0, 0.0, 0.0};
@@ -485,6 +485,8 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
         ssf = a.rec.sf[ridx];
         sid = a.rec.id[ridx];
     }
+    const double osf = ssf;
+    const int oid = sid;
 
     /* candidates in key order; their data are wave-uniform */
     const Cand* __restrict__ cands = a.cand + cstart;
@@ -521,17 +523,21 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
             const bool pnew = sf > SOLFRAC_MIN, pold = ssf > SOLFRAC_MIN, own = sid == cd.id;
             const bool zero = inb && !pnew && (!pold || own);
             const bool set = inb && pnew && (!pold || sf > ssf || own);
-            /* setValues, src/ibProcessors3D.hh:162-174 */
-            const double u0 = cd.v[0] + (cd.w[1] * dz - cd.w[2] * dy);
-            const double u1 = cd.v[1] + (-cd.w[0] * dz + cd.w[2] * dx);
-            const double u2 = cd.v[2] + (cd.w[0] * dy - cd.w[1] * dx);
             kind = set ? ST_SET : (zero ? ST_ZERO : kind);
             ssf = set ? sf : (zero ? 0.0 : ssf);
             sid = set ? cd.id : (zero ? 0 : sid);
             se = set ? cd.e : (zero ? -1 : se);
-            sus[0] = set ? u0 : (zero ? 0.0 : sus[0]);
-            sus[1] = set ? u1 : (zero ? 0.0 : sus[1]);
-            sus[2] = set ? u2 : (zero ? 0.0 : sus[2]);
+            sj = set ? j : (zero ? -1 : sj);
+            if constexpr (INITVEL) {
+                /* setValues, src/ibProcessors3D.hh:162-174, here because
+                 * initVel reads the velocity of the record in force */
+                const double u0 = cd.v[0] + (cd.w[1] * dz - cd.w[2] * dy);
+                const double u1 = cd.v[1] + (-cd.w[0] * dz + cd.w[2] * dx);
+                const double u2 = cd.v[2] + (cd.w[0] * dy - cd.w[1] * dx);
+                sus[0] = set ? u0 : (zero ? 0.0 : sus[0]);
+                sus[1] = set ? u1 : (zero ? 0.0 : sus[1]);
+                sus[2] = set ? u2 : (zero ? 0.0 : sus[2]);
+            }
             if (INITVEL && inb && sf > SOLFRAC_MAX) {
                 dv = true;
                 if (kind == ST_OLD) {
@@ -552,8 +558,21 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     if (__lane_id() == 0 && m) atomicAdd(&a.cnt_new[b], (uint32_t)__popcll(m));
     if (!valid) return;
     if (kind == ST_SET) {
-        a.rec.sf[ridx] = ssf;
-        a.rec.id[ridx] = sid;
+        if constexpr (!INITVEL) {
+            /* setValues (src/ibProcessors3D.hh:162-174) of the candidate
+             * that set the record last, once per cell: the same FP64
+             * expressions as in the candidate loop */
+            const Cand& cw = cands[sj];
+            const double dx = (double)X - cw.x[0];
+            const double dy = (double)Y - cw.x[1];
+            const double dz = (double)Z - cw.x[2];
+            sus[0] = cw.v[0] + (cw.w[1] * dz - cw.w[2] * dy);
+            sus[1] = cw.v[1] + (-cw.w[0] * dz + cw.w[2] * dx);
+            sus[2] = cw.v[2] + (cw.w[0] * dy - cw.w[1] * dx);
+        }
+        /* a live record whose fraction and owner did not change keeps them */
+        if (!had || ssf != osf) a.rec.sf[ridx] = ssf;
+        if (!had || sid != oid) a.rec.id[ridx] = sid;
         a.rec.us[ridx] = sus[0];
         a.rec.us[a.rcap + ridx] = sus[1];
         a.rec.us[2 * a.rcap + ridx] = sus[2];
