@@ -139,6 +139,15 @@ __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
     const double dy0 = (double)Y, dz0 = (double)Z;
     for (int t = 0; t < ncand; ++t) {
         const int e = cstart + t;
+        /* an entry whose candidate set no cell of this bin (its box only
+         * grazes the bin) contributes zeros, without the reduction */
+        bool any = false;
+#pragma unroll
+        for (int x = 0; x < BIN; ++x) any |= ce[x] == e;
+        if (__ballot(any) == 0ull) {
+            if (lane < 6) a.partial[6 * (int64_t)e + lane] = 0.0;
+            continue;
+        }
         double xo, yo, zo;
         if (t < 64) {
             xo = readlane_d(px, t); yo = readlane_d(py, t); zo = readlane_d(pz, t);
