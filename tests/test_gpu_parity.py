@@ -51,7 +51,7 @@ def random_state(n, seed, u0=0.02):
     return rho, u
 
 
-@pytest.mark.parametrize("N", [5, 8])
+@pytest.mark.parametrize("N", [5, 8, 6])
 def test_subcell_counts_bitexact(N):
     rng = np.random.default_rng(10 + N)
     m = 20000
@@ -137,7 +137,7 @@ def run_coupled(n, periodic, N, count, steps, seed, tau=0.7, init_vel=False, bod
     return worst, st
 
 
-@pytest.mark.parametrize("N", [5, 8])
+@pytest.mark.parametrize("N", [5, 8, 6])
 def test_coupled_steps_periodic(N):
     worst, st = run_coupled((32, 28, 30), (1, 1, 1), N, 14, 4, seed=3 + N)
     assert st["images"] > 0 and st["records"] > 0
@@ -206,6 +206,39 @@ def test_stale_records_are_kept_and_summed():
     Fo, To = o.sum_force_torque(x)
     assert g.stats()["orphans"] >= 1
     assert rel_err(Fg, Fo) < FORCE_RTOL and torque_err(Tg, To, Fo) < FORCE_RTOL
+    g.close()
+
+
+def test_many_candidates_in_one_bin():
+    """More than 64 small spheres whose boxes meet one 8^3 bin: the force sum
+    takes the positions of entries past the 64th from memory, and the fold
+    walks a long candidate list.  Overlapping spheres go through the
+    four-case rule like any other."""
+    n = (24, 24, 24)
+    g, o = make_pair(n, (1, 1, 1), tau=0.7, N=8)
+    rng = np.random.default_rng(77)
+    m = 90
+    x = rng.uniform(7.6, 15.4, (m, 3))
+    v = rng.normal(0, 0.01, (m, 3))
+    w = rng.normal(0, 1e-3, (m, 3))
+    r = rng.uniform(0.6, 1.1, m)
+    # every sphere's box (r + sqrt(3.1)/2 around the centre) meets the bin of cells 8..15
+    assert np.all(x - r[:, None] - 0.89 < 16) and np.all(x + r[:, None] + 0.89 >= 8)
+    ids = list(range(1, m + 1))
+    rho, u = random_state(n, 9, 0.01)
+    for lat in (g, o):
+        lat.init_equilibrium(rho, u)
+    for s in range(2):
+        if s > 0:
+            x = L.advance_positions(x, v, n, (1, 1, 1))
+        for lat in (g, o):
+            lat.set_spheres(x, v, w, r, ids)
+        assert_records_equal(g.cell_records(), o.cell_records())
+        g.collide_stream(1)
+        o.collide_stream(1)
+        Fg, Tg = g.sum_force_torque(x)
+        Fo, To = o.sum_force_torque(x)
+        assert rel_err(Fg, Fo) < FORCE_RTOL and torque_err(Tg, To, Fo) < FORCE_RTOL
     g.close()
 
 
