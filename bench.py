@@ -522,7 +522,7 @@ def main():
         summarize_profiles(args.summarize_profiles)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
 
