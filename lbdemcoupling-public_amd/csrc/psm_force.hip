@@ -38,12 +38,6 @@ struct ForceArgs {
     int64_t nbins;
 };
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-    return v;
-}
-
 /* Sum of v[0..7] over the 64 lanes of a wavefront, reduce-scatter style:
  * three halving exchanges (lane bits 5, 4, 3) leave each lane one value,
  * index m = 4 b5 + 2 b4 + b3 of the lane, summed over the eight lanes that
@@ -183,7 +177,7 @@ __global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
 }
 
 /* One wavefront per particle: lane-strided partial sums of its candidate
- * entries, then the fixed 64-lane shuffle tree. */
+ * entries, then the fixed reduce-scatter tree of wave_sum8_scatter. */
 __global__ __launch_bounds__(256) void k_force_particle(int64_t np, const int32_t* __restrict__ p_ent_start,
                                                         const int32_t* __restrict__ p_nent,
                                                         const int32_t* __restrict__ owner_ent,
@@ -191,7 +185,7 @@ __global__ __launch_bounds__(256) void k_force_particle(int64_t np, const int32_
                                                         double* __restrict__ pft, int64_t ent_cap) {
     const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = (int)__lane_id();
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (q < np) {
         const int64_t s = p_ent_start[q];
         const int n = p_nent[q];
@@ -203,12 +197,9 @@ __global__ __launch_bounds__(256) void k_force_particle(int64_t np, const int32_
             for (int m = 0; m < 6; ++m) acc[m] += partial[6 * (int64_t)e + m];
         }
     }
-#pragma unroll
-    for (int m = 0; m < 6; ++m) acc[m] = wave_sum(acc[m]);
-    if (q < np && lane == 0) {
-#pragma unroll
-        for (int m = 0; m < 6; ++m) pft[6 * q + m] = acc[m];
-    }
+    const double v = wave_sum8_scatter(acc, lane);
+    const int m = lane >> 3;
+    if (q < np && (lane & 7) == 0 && m < 6) pft[6 * q + m] = v;
 }
 
 __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_t* __restrict__ orphan,
