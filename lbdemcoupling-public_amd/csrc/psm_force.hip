@@ -8,10 +8,10 @@
  *
  * Here the sum runs over the bins that hold candidates, in two
  * deterministic stages:
- *  1. per bin (one workgroup): the bin's records are staged in LDS; each
- *     wavefront takes some of the bin's candidate entries and sums, for
- *     each, the records that candidate set this step, with a fixed 64-lane
- *     shuffle tree in FP64.  One 6-vector partial per candidate entry.
+ *  1. per bin (one wavefront): each lane holds eight cells of the bin in
+ *     registers; for each of the bin's candidate entries the wavefront sums
+ *     the records that candidate set this step, with a fixed reduce-scatter
+ *     tree in FP64.  One 6-vector partial per candidate entry.
  *  2. per particle: its candidate entries are added in the fixed
  *     (image rank, bin) order.
  * Records whose owner set no candidate of their bin this step (stale
@@ -42,8 +42,9 @@ struct ForceArgs {
  * three halving exchanges (lane bits 5, 4, 3) leave each lane one value,
  * index m = 4 b5 + 2 b4 + b3 of the lane, summed over the eight lanes that
  * share those bits; three butterflies over bits 2, 1, 0 finish it.  Ten
- * shuffles of a double in place of 48, and the order of every addition is
- * fixed (a + b and b + a round alike), so the sum is deterministic. */
+ * shuffles of a double in place of 36 for six separate trees, and the order
+ * of every addition is fixed (a + b and b + a round alike), so the sum is
+ * deterministic. */
 __device__ __forceinline__ double wave_sum8_scatter(const double (&v)[8], int lane) {
     double w4[4], w2[2];
     const bool u5 = (lane & 32) != 0, u4 = (lane & 16) != 0, u3 = (lane & 8) != 0;
