@@ -13,11 +13,12 @@
  * previous step.  Here every particle image is binned into 8^3-cell bins
  * (key = image rank << 27 | particle index, which is the canonical order:
  * the given particles by index, then periodic images by shift rank), each
- * bin's candidate list is sorted, and one workgroup per bin stages the
- * candidates in LDS and folds them, per cell and in key order, through the
- * reference's four-case rule.  Each cell's record lives at a fixed,
- * bin-permuted index and is updated in place (the fold of a cell reads
- * and writes only that cell's record).
+ * bin's candidate list is sorted, and one workgroup per bin (one lane per
+ * cell, one wavefront per x-slice) folds them, per cell and in key order,
+ * through the reference's four-case rule; the candidates' data are
+ * wave-uniform and come in through scalar loads.  Each cell's record lives
+ * at the cell's own index and is updated in place (the fold of a cell
+ * reads and writes only that cell's record).
  *
  * The sub-cell count is evaluated in FP64 with the reference's expression
  * order and without contraction (this file is compiled with
