@@ -274,7 +274,8 @@ struct Cand {
     double w[3];
     double rp2, rm2, rsq;
     double ri2, ro2;    /* squared radii of the exact no-count band (sub-cell rule) */
-    int lo[3], hi[3];   /* clipped bounding box (cells) */
+    uint64_t yzmask;    /* clipped bounding box within the entry's bin: bit 8 y + z */
+    uint32_t xmask;     /* ... and bit x (bin-local coordinates) */
     int id;
     int e;              /* entry index */
 };

@@ -224,6 +224,7 @@ __global__ __launch_bounds__(256) void k_cand_gather(VoxArgs a, const uint32_t* 
                                                      const int32_t* __restrict__ pid, double h, double esamp,
                                                      const int32_t* __restrict__ bin_start,
                                                      const int32_t* __restrict__ bin_count,
+                                                     const int32_t* __restrict__ cand_bin,
                                                      int64_t nbins, Cand* __restrict__ out, int64_t cap) {
     const int64_t nent = nbins > 0 ? (int64_t)bin_start[nbins - 1] + bin_count[nbins - 1] : 0;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nent && e < cap;
@@ -242,8 +243,25 @@ __global__ __launch_bounds__(256) void k_cand_gather(VoxArgs a, const uint32_t* 
             xi[0] = xi[1] = xi[2] = 0.0;
         }
         for (int d = 0; d < 3; ++d) {
-            cd.x[d] = xi[d]; cd.lo[d] = lo[d]; cd.hi[d] = hi[d];
+            cd.x[d] = xi[d];
             cd.v[d] = pv[3 * q + d]; cd.w[d] = pw[3 * q + d];
+        }
+        /* the box clipped to the entry's bin, as masks over the bin's cells */
+        {
+            const int b = cand_bin[e];
+            const int bz = b % a.nb[2], by = (b / a.nb[2]) % a.nb[1], bx = b / (a.nb[2] * a.nb[1]);
+            const int o[3] = {bx * BIN, by * BIN, bz * BIN};
+            uint32_t m[3];
+            for (int d = 0; d < 3; ++d) {
+                const int s = lo[d] - o[d] > 0 ? lo[d] - o[d] : 0;
+                const int t = hi[d] - o[d] < BIN - 1 ? hi[d] - o[d] : BIN - 1;
+                m[d] = s <= t ? ((2u << t) - 1u) & ~((1u << s) - 1u) : 0u;
+            }
+            uint64_t yz = 0;
+            for (int yl = 0; yl < BIN; ++yl)
+                if ((m[1] >> yl) & 1u) yz |= (uint64_t)m[2] << (BIN * yl);
+            cd.xmask = m[0];
+            cd.yzmask = yz;
         }
         cd.r = r;
         const double r_p = r + h;
@@ -483,8 +501,7 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     const int64_t cidx = valid ? ((int64_t)X * a.n[1] + Y) * a.n[2] + Z : 0;
     const int64_t ridx = cidx;   /* record slot = cell index */
     /* a wavefront is the x-slice X of the bin: 8 y by 8 z cells */
-    const int wX = __builtin_amdgcn_readfirstlane(X);
-    const int y0 = by * BIN, z0 = bz * BIN;
+    const int wl = __builtin_amdgcn_readfirstlane(l >> 6);
 
     const bool had = valid && a.sflag[cidx] != 0;
     int kind = had ? ST_OLD : ST_ZERO, sid = 0, se = -1, sj = -1;
@@ -503,14 +520,11 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     {
         for (int j = 0; j < ncand; ++j) {
             const Cand& cd = cands[j];
-            /* whole-wavefront rejection against the candidate's box; the six
-             * bounds are read together (no short circuit between the loads) */
-            const int lo0 = cd.lo[0], lo1 = cd.lo[1], lo2 = cd.lo[2];
-            const int hi0 = cd.hi[0], hi1 = cd.hi[1], hi2 = cd.hi[2];
-            if ((wX < lo0) | (wX > hi0) | (y0 + BIN - 1 < lo1) | (y0 > hi1) | (z0 + BIN - 1 < lo2) | (z0 > hi2))
-                continue;
-            const bool inb = valid && X >= cd.lo[0] && X <= cd.hi[0] && Y >= cd.lo[1] && Y <= cd.hi[1] &&
-                             Z >= cd.lo[2] && Z <= cd.hi[2];
+            /* whole-wavefront rejection against the candidate's box (its
+             * x-slices within the bin), then the lane's cell against its
+             * (y, z) cells within the bin */
+            if (((cd.xmask >> wl) & 1u) == 0u) continue;
+            const bool inb = valid && ((cd.yzmask >> (l & 63)) & 1ull) != 0ull;
             const double dx = (double)X - cd.x[0];
             const double dy = (double)Y - cd.x[1];
             const double dz = (double)Z - cd.x[2];
@@ -701,7 +715,7 @@ int run_voxelize(lbdem_psm* h) {
     const double esamp = std::sqrt(3.0) * (0.5 - 0.5 / (double)h->desc.slices_per_dim) + 1e-12;
     hipLaunchKernelGGL(k_cand_gather, dim3((unsigned)std::max<int64_t>(1, std::min<int64_t>(8192, (h->ent_cap + 255) / 256))),
                        dim3(256), 0, h->stream, va, h->cand_key, h->pv, h->pw, h->pid, h->sqrt3half, esamp,
-                       h->bin_cand_start, h->bin_cand_count, nbins, h->cand_data, h->ent_cap);
+                       h->bin_cand_start, h->bin_cand_count, h->cand_bin, nbins, h->cand_data, h->ent_cap);
 
     FoldArgs fa;
     for (int d = 0; d < 3; ++d) { fa.n[d] = h->g.n[d]; fa.nb[d] = h->g.nb[d]; }
