@@ -352,7 +352,8 @@ __device__ __forceinline__ unsigned wave_count(double dx, double dy, double dz, 
  * test is the same FP64 expression as wave_count's,
  * (dx_sq[i] + dy_sq[j]) + dz_sq[k] < r^2, so the counts are the same. */
 __device__ __forceinline__ void slice_counts8(unsigned long long pend, double dx, double dy, double dz,
-                                              double rsq, double sw, double frac, int& n, double& sf) {
+                                              double rsq, double sw, double frac, double di, double dj, int& n,
+                                              double& sf) {
     /* dz_sq[k] of every z-row of the slice, one value per lane (lane
      * 8 r + k), in LDS: a z-row then reads its eight values on the LDS pipe
      * instead of forming them with 16 FP64 instructions.  One region per
@@ -363,11 +364,9 @@ __device__ __forceinline__ void slice_counts8(unsigned long long pend, double dx
     double* zt = ztab[threadIdx.x >> 6];
     {
         const double dzr = __shfl(dz, lane >> 3, 64);   /* dz of z-row lane / 8 */
-        const double az = dzr + (-0.5 + ((double)(lane & 7) + 0.5) * sw);
+        const double az = dzr + dj;
         zt[lane] = az * az;
     }
-    const double di = -0.5 + ((double)(lane >> 3) + 0.5) * sw;
-    const double dj = -0.5 + ((double)(lane & 7) + 0.5) * sw;
     const double ax = readlane_d(dx, __ffsll(pend) - 1) + di;
     const double xsq = ax * ax;
     while (pend) {
@@ -414,7 +413,7 @@ template <int NT, bool SLICE>
 __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, double dy, double dz,
                                                       double rp2, double rm2, double rsq, double ri2,
                                                       double ro2, int Nrt, double sw, double frac, int& n,
-                                                      int& br) {
+                                                      int& br, double di = 0.0, double dj = 0.0) {
     const int N = NT > 0 ? NT : Nrt;
     const int N3 = N * N * N;
     /* branch-free: every lane evaluates every test */
@@ -433,7 +432,7 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
         const double ex = fabs(dx), ey = fabs(dy), ez = fabs(dz);
         const double fx = ex + s, fy = ey + s, fz = ez + s;
         const double nx = fmax(ex - s, 0.0), ny = fmax(ey - s, 0.0), nz = fmax(ez - s, 0.0);
-        const bool hin = need && (fx * fx + fy * fy) + fz * fz < rsq - tol;
+        const bool hin = need & ((fx * fx + fy * fy) + fz * fz < rsq - tol);
         const bool hout = (nx * nx + ny * ny) + nz * nz > rsq + tol;
         n = hin ? N3 : n;
         sf = hin ? frac * (double)N3 : sf;
@@ -442,7 +441,7 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
     unsigned long long pend = __ballot(need);
     if constexpr (SLICE) {
         static_assert(NT == 8, "one sample column per lane");
-        if (pend) slice_counts8(pend, dx, dy, dz, rsq, sw, frac, n, sf);
+        if (pend) slice_counts8(pend, dx, dy, dz, rsq, sw, frac, di, dj, n, sf);
         return sf;
     }
     const int lane = (int)__lane_id();
@@ -465,14 +464,16 @@ template <int NT, bool SLICE = false>
 __device__ __forceinline__ double solid_fraction(bool active, double dx, double dy, double dz,
                                                  double rp2, double rm2, double rsq, double r, double ri2,
                                                  double ro2, int Nrt, double sw, double frac, double rc,
-                                                 double vc, int& n, int& br) {
+                                                 double vc, int& n, int& br, double di = 0.0,
+                                                 double dj = 0.0) {
     if constexpr (NT < 0) {
         n = 0;
         br = 0;
         if (!active) return 0.0;
         return overlap_fraction(sqrt((dx * dx + dy * dy) + dz * dz), r, rc, vc, nullptr, &br);
     } else {
-        return coop_solid_fraction<NT, SLICE>(active, dx, dy, dz, rp2, rm2, rsq, ri2, ro2, Nrt, sw, frac, n, br);
+        return coop_solid_fraction<NT, SLICE>(active, dx, dy, dz, rp2, rm2, rsq, ri2, ro2, Nrt, sw, frac, n, br, di,
+                                              dj);
     }
 }
 
@@ -508,6 +509,10 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
     double ssf = 0.0, sus[3] = {0.0, 0.0, 0.0};
     bool dv = false;
     double dvus[3] = {0.0, 0.0, 0.0};
+    /* sample offsets of the lane's column (i, j) = (lane / 8, lane % 8) in
+     * the N = 8 count, the reference's delta expression */
+    const double sdi = -0.5 + ((double)((l & 63) >> 3) + 0.5) * a.sw;
+    const double sdj = -0.5 + ((double)(l & 7) + 0.5) * a.sw;
     if (had) {
         ssf = a.rec.sf[ridx];
         sid = a.rec.id[ridx];
@@ -524,7 +529,7 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
              * x-slices within the bin), then the lane's cell against its
              * (y, z) cells within the bin */
             if (((cd.xmask >> wl) & 1u) == 0u) continue;
-            const bool inb = valid && ((cd.yzmask >> (l & 63)) & 1ull) != 0ull;
+            const bool inb = valid & (((cd.yzmask >> (l & 63)) & 1ull) != 0ull);
             const double dx = (double)X - cd.x[0];
             const double dy = (double)Y - cd.x[1];
             const double dz = (double)Z - cd.x[2];
@@ -534,13 +539,15 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
              * rule); a lane with no record stays empty.  Skip the candidate
              * when that holds for every lane of the wavefront. */
             {
-                const bool near = inb && dx * dx + dy * dy + dz * dz <= cd.rp2;
-                const bool mine = inb && ssf > SOLFRAC_MIN && sid == cd.id;
+                /* non-short-circuit: the candidate's reach and id load
+                 * with its centre, not behind a branch each */
+                const bool near = inb & (dx * dx + dy * dy + dz * dz <= cd.rp2);
+                const bool mine = inb & (ssf > SOLFRAC_MIN) & (sid == cd.id);
                 if (__ballot(near || mine) == 0ull) continue;
             }
             int nn, br;
             const double sf = solid_fraction<NT, NT == 8>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, cd.ri2, cd.ro2,
-                                                 a.N, a.sw, a.frac, a.rc, a.vc, nn, br);
+                                                 a.N, a.sw, a.frac, a.rc, a.vc, nn, br, sdi, sdj);
             /* the four-case rule, branch-free: flag = new + 2 old;
              * 0 -> zero, 1 -> set, 2 -> zero if own record, 3 -> set if
              * larger or own record */
