@@ -197,9 +197,10 @@ __device__ __forceinline__ void psm_solid(const T* fPre, T* f, T omega, bool ful
 /* Minimum image of a torque arm component over the lattice size n
  * (src/ibProcessors3D.hh:238-243, integer halves of the block size). */
 __device__ __forceinline__ double min_image(double d, int n) {
-    if (d > (double)(n / 2)) d -= (double)n;
-    else if (d < (double)(-n / 2)) d += (double)n;
-    return d;
+    /* selects, not branches: the same two candidate values either way */
+    const double h = (double)(n / 2), l = (double)(-n / 2);
+    const double up = d - (double)n, dn = d + (double)n;
+    return d > h ? up : (d < l ? dn : d);
 }
 
 /* Analytic sphere-cell overlap, the differentiable voxelization: the
