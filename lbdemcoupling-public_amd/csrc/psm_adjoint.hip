@@ -880,6 +880,10 @@ int lbdem_psm_perturb_solid_fraction(lbdem_psm* h, const double* dsf) {
     dfree_d(d);
     PSM_HIP(h, e);
     h->geom_gen += 1;
+    {
+        const int mr = mirror_records(h);   /* the valid entries were applied */
+        if (mr) return mr;
+    }
     if (flags & 16)
         return set_err(h, LBDEM_PSM_ERR_INVALID_ARG,
                        "solid-fraction perturbation leaves (0.001, 1] or touches a cell without a record "

@@ -74,11 +74,13 @@ static int alloc_records(lbdem_psm* h, Records& r) {
     PSM_HIP(h, dalloc(&r.us, 3 * cap * sizeof(double)));
     PSM_HIP(h, dalloc(&r.F, 3 * cap * sizeof(double)));
     PSM_HIP(h, dalloc(&r.cand, cap * sizeof(int32_t)));
+    r.m4 = nullptr;
+    if (h->precision != LBDEM_PSM_FP64) PSM_HIP(h, dalloc(&r.m4, 4 * cap * sizeof(float)));
     return LBDEM_PSM_OK;
 }
 
 static void free_records(Records& r) {
-    dfree(r.sf); dfree(r.id); dfree(r.us); dfree(r.F); dfree(r.cand);
+    dfree(r.sf); dfree(r.id); dfree(r.us); dfree(r.F); dfree(r.cand); dfree(r.m4);
     memset(&r, 0, sizeof(r));
 }
 
@@ -624,7 +626,7 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
     PSM_HIP(h, hipMemcpy(h->counters, hc, sizeof(hc), hipMemcpyHostToDevice));
     h->records_from_host = true;
     h->geom_gen += 1;
-    return LBDEM_PSM_OK;
+    return mirror_records(h);
 }
 
 int lbdem_psm_subcell_counts(lbdem_psm* h, int64_t n, const double* d3, const double* r, double* sf,

@@ -164,6 +164,7 @@ static CollideArgs make_args(lbdem_psm* h) {
     for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
     a.sflag = h->sflag;
     a.rsf = h->rec.sf; a.rus = h->rec.us; a.rF = h->rec.F; a.rcap = h->rec_cap;
+    a.rm4 = h->rec.m4;
     return a;
 }
 

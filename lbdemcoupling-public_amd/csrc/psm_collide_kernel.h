@@ -28,6 +28,7 @@ struct CollideArgs {
     const double* rus;
     void* rF;        /* per-cell force: double for FP64 storage, float otherwise */
     int64_t rcap;
+    const float* rm4;  /* float view of the records (FP32/FP16 storage) */
 };
 
 /* Equilibria of a population pair (i, i+9): c_{i+9} = -c_i, so the pair
@@ -332,8 +333,18 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     const bool solid = flag != 0;
     /* fluid lanes read record 0 (one shared line); their values are unused */
     const int64_t s = solid ? c : 0;
-    const double rsf = a.rsf[s];
-    const double rus[3] = {a.rus[s], a.rus[a.rcap + s], a.rus[2 * a.rcap + s]};
+    /* FP64: the record itself; FP32/FP16: its 16-byte float view (half the
+     * bytes of the four doubles, one load) */
+    double rsf = 0.0, rus[3] = {0.0, 0.0, 0.0};
+    float4 rm = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if constexpr (sizeof(T) == 4) {
+        rm = *(const float4*)(a.rm4 + 4 * s);
+    } else {
+        rsf = a.rsf[s];
+        rus[0] = a.rus[s];
+        rus[1] = a.rus[a.rcap + s];
+        rus[2] = a.rus[2 * a.rcap + s];
+    }
     __builtin_amdgcn_sched_barrier(0);
     T fp[Q];
 #pragma unroll
@@ -344,11 +355,11 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
         float rhoP, jP[3];
         bgk_all_fast<FORCE>(fp, fo, omega, acc, rhoP, jP);
         if (solid) {
-            const float sf = (float)rsf;
-            const bool full = rsf > SOLFRAC_MAX;
+            const float sf = rm.x;
+            const bool full = rm.x > 1.5f;   /* the view's mark of sf > SOLFRAC_MAX */
             const float ooo = (float)a.ooo;
             const float B = full ? 1.0f : (sf * ooo) / ((1.0f - sf) + ooo);
-            const float us[3] = {(float)rus[0], (float)rus[1], (float)rus[2]};
+            const float us[3] = {rm.y, rm.z, rm.w};
             float F[3];
             psm_blend_fast(fp, fo, full, B, us, rhoP, jP, F);
             Fc[0] = double(F[0]);

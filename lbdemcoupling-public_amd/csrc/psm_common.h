@@ -265,7 +265,18 @@ struct Records {
     double* F;       /* 3*cap: hydrodynamic force of the last collision (FP64 storage; with
                       * FP32/FP16 storage the first 3*cap floats of the buffer hold it) */
     int32_t* cand;   /* candidate entry of the setter this step, -1 = none */
+    float* m4;       /* FP32/FP16 storage only, 4*cap: per cell (s, us) in float for the
+                      * collide, s = 2 for sf > SOLFRAC_MAX, else (float)sf; valid where the
+                      * cell holds a live record (null with FP64 storage) */
 };
+
+/* The float view of a live record that the FP32/FP16 collide reads. */
+__host__ __device__ inline void record_mirror(double sf, const double* us, float* out4) {
+    out4[0] = sf > SOLFRAC_MAX ? 2.0f : (float)sf;
+    out4[1] = (float)us[0];
+    out4[2] = (float)us[1];
+    out4[3] = (float)us[2];
+}
 
 /* A candidate particle image of one bin, gathered in key order. */
 struct Cand {
@@ -507,6 +518,7 @@ int periodic_pressure_adjoint_post(lbdem_psm* h, void* lam);
 int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
                         int64_t* n_nonfinite);
 int run_voxelize(lbdem_psm* h);
+int mirror_records(lbdem_psm* h);   /* float view of the live records (FP32/FP16 storage) */
 int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_r, double* d_sf,
                    int32_t* d_cnt, int32_t* d_br);
 int run_force_sum(lbdem_psm* h, const double* xs);   /* xs: positions for the torque arms */

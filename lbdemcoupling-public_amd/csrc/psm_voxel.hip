@@ -608,6 +608,11 @@ __global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
         a.rec.us[a.rcap + ridx] = sus[1];
         a.rec.us[2 * a.rcap + ridx] = sus[2];
         a.rec.cand[ridx] = se;
+        if (a.rec.m4) {
+            float m[4];
+            record_mirror(ssf, sus, m);
+            *(float4*)(a.rec.m4 + 4 * ridx) = make_float4(m[0], m[1], m[2], m[3]);
+        }
     } else if (kind == ST_OLD) {
         /* untouched this step: values stay, summed on the orphan path */
         a.rec.cand[ridx] = -1;
@@ -768,6 +773,31 @@ int run_voxelize(lbdem_psm* h) {
         int rc = launch_define_velocity(h);
         if (rc) return rc;
     }
+    return LBDEM_PSM_OK;
+}
+
+/* The float view of every live record, after a writer other than the fold
+ * (host records, a checkpoint, the sf perturbation of the adjoint checks). */
+__global__ __launch_bounds__(256) void k_mirror_records(int64_t nc, const uint8_t* __restrict__ sflag,
+                                                        const double* __restrict__ sf,
+                                                        const double* __restrict__ us, int64_t rcap,
+                                                        float* __restrict__ m4) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (!sflag[c]) continue;
+        const double u[3] = {us[c], us[rcap + c], us[2 * rcap + c]};
+        float m[4];
+        record_mirror(sf[c], u, m);
+        *(float4*)(m4 + 4 * c) = make_float4(m[0], m[1], m[2], m[3]);
+    }
+}
+
+int mirror_records(lbdem_psm* h) {
+    if (!h->rec.m4) return LBDEM_PSM_OK;
+    const int64_t nc = h->g.nc;
+    hipLaunchKernelGGL(k_mirror_records, dim3((unsigned)std::min<int64_t>(8192, (nc + 255) / 256)), dim3(256), 0,
+                       h->stream, nc, h->sflag, h->rec.sf, h->rec.us, h->rec_cap, h->rec.m4);
+    PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }
 
