@@ -410,6 +410,52 @@ def _short(name):
     return base[:48]
 
 
+ISSUE_CTRS = "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE"
+ISSUE_CMD = ("rocprofv3 --pmc " + ISSUE_CTRS + " --output-format csv -d DIR/{tag} -o run -- "
+             "python3 bench.py {args}--kernel-probe")
+
+
+def summarize_issue(d):
+    """Instruction counters of the step's kernels -> profiles/kernel_issue_counters.md."""
+    import csv
+    out = ["# Instructions per wavefront and issue share, the coupled step's kernels\n",
+           "Counters: `" + ISSUE_CMD.format(tag="issue", args="") + "` (256^3 dense suspension) and `"
+           + ISSUE_CMD.format(tag="issue512", args="--workload channel512 ") + "` (512x512x768 channel);",
+           "one MI355X, ROCm 7.2.  Counter values are summed over the device.  The issue share is",
+           "instructions x 4 cycles (a wave64 instruction on a 16-lane SIMD) over 1024 SIMDs x the",
+           "kernel's cycles (GRBM_GUI_ACTIVE / 8, one count per XCD); VALU and SALU issue from",
+           "different wavefronts in the same cycle.\n"]
+    found = False
+    for tag, title in (("issue", "256^3 dense suspension"), ("issue512", "512x512x768 channel")):
+        path = os.path.join(d, tag, "run_counter_collection.csv")
+        if not os.path.exists(path):
+            continue
+        found = True
+        acc = {}
+        for r in csv.DictReader(open(path)):
+            acc.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+        out += [f"## {title}\n", "| kernel | waves | VALU / wave | SALU / wave | SMEM / wave | LDS / wave | "
+                "VALU issue share | SALU issue share |", "|---|---|---|---|---|---|---|---|"]
+        rows = []
+        for k, v in acc.items():
+            sk = _short(k)
+            if not (sk.startswith("k_collide_stream") or sk.startswith("k_fold") or sk.startswith("k_force_bin")):
+                continue
+            m = {c: sum(x) / len(x) for c, x in v.items()}
+            waves = max(m.get("SQ_WAVES", 0.0), 1.0)
+            cyc = m.get("GRBM_GUI_ACTIVE", 0.0) / 8.0
+            share = (lambda n: n * 4.0 / (1024.0 * cyc)) if cyc > 0 else (lambda n: 0.0)
+            rows.append((sk, waves, m.get("SQ_INSTS_VALU", 0) / waves, m.get("SQ_INSTS_SALU", 0) / waves,
+                         m.get("SQ_INSTS_SMEM", 0) / waves, m.get("SQ_INSTS_LDS", 0) / waves,
+                         share(m.get("SQ_INSTS_VALU", 0)), share(m.get("SQ_INSTS_SALU", 0))))
+        for sk, waves, va, sa, sm, ld, vs, ss in sorted(rows):
+            out.append(f"| {sk} | {waves:.0f} | {va:.0f} | {sa:.0f} | {sm:.1f} | {ld:.1f} | {vs:.2f} | {ss:.2f} |")
+        out.append("")
+    if found:
+        with open(os.path.join(ROOT, "profiles", "kernel_issue_counters.md"), "w") as f:
+            f.write("\n".join(out) + "\n")
+
+
 def summarize_profiles(d):
     """Kernel-trace stats and FETCH/WRITE counters -> profiles/*.md + traffic json."""
     import csv
@@ -437,6 +483,7 @@ def summarize_profiles(d):
                 f.write(f"| {_short(name)} | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |\n")
             f.write("\n`__amd_rocclr_copyBuffer` is the run's device-to-device copy ceiling "
                     "(hipMemcpy of 2 GiB, the `device_copy_gbs` figure), not part of a step.\n")
+    summarize_issue(d)
     res = {}
     for tag, ctr in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
         path = os.path.join(d, tag, "run_counter_collection.csv")
