@@ -378,44 +378,72 @@ __global__ __launch_bounds__(256) void k_adjoint_records(const typename Store<PR
                                                          const typename Store<PREC>::S* __restrict__ lin,
                                                          AdjArgs a, const int32_t* __restrict__ list,
                                                          int64_t n) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    /* consecutive list positions are neighbouring cells: keep a range of
+     * them on one XCD's L2, where their gathers share lines */
+    const int64_t k = (int64_t)xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
     if (k >= n) return;
     adjoint_record_cell<PREC, FORCE>(fin, lin, a, (int64_t)list[k], k);
 }
 
-/* The record slots of the frozen geometry, appended per wavefront (one
- * atomic per wavefront; the order is irrelevant, every record writes only
- * its own slots). */
+/* Live record cells of bin b (one workgroup per bin). */
+__global__ __launch_bounds__(256) void k_count_records(const uint8_t* __restrict__ sflag,
+                                                       const uint32_t* __restrict__ bin_live, int n0, int n1,
+                                                       int n2, int nb1, int nb2, int64_t* __restrict__ cnt) {
+    const int64_t b = blockIdx.x;
+    __shared__ uint32_t s_n;
+    if (threadIdx.x == 0) s_n = 0u;
+    __syncthreads();
+    if (bin_live[b] != 0u) {
+        const int bz = (int)(b % nb2);
+        const int by = (int)((b / nb2) % nb1);
+        const int bx = (int)(b / ((int64_t)nb2 * nb1));
+        for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) {
+            const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
+            const bool live = X < n0 && Y < n1 && Z < n2 && sflag[((int64_t)X * n1 + Y) * n2 + Z] != 0;
+            const unsigned long long m = __ballot(live);
+            if (__lane_id() == 0 && m != 0ull) atomicAdd(&s_n, (uint32_t)__popcll(m));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) cnt[b] = (int64_t)s_n;
+}
+
+/* The record slots of the frozen geometry in bin order (bins z fastest),
+ * and within a bin in cell order (x slice, then y, then z), from the bin's
+ * first position off[b].  Consecutive list positions are then neighbouring
+ * cells, so the record stage's gathers of one workgroup share lines. */
 __global__ __launch_bounds__(256) void k_list_records(const uint8_t* __restrict__ sflag,
                                                       const uint32_t* __restrict__ bin_live, int n0, int n1,
-                                                      int n2, int nb1, int nb2, int32_t* __restrict__ list,
-                                                      int32_t* __restrict__ rpos,
-                                                      unsigned long long* __restrict__ count, int64_t cap) {
-    /* neighbouring bins (z fastest) share the lines of the cell-ordered
-     * arrays: keep them on one XCD's L2 */
-    const int64_t b = xcd_block((int)blockIdx.x, (int)gridDim.x);
+                                                      int n2, int nb1, int nb2, const int64_t* __restrict__ off,
+                                                      int32_t* __restrict__ list, int32_t* __restrict__ rpos) {
+    const int64_t b = blockIdx.x;
     if (bin_live[b] == 0u) return;
     const int bz = (int)(b % nb2);
     const int by = (int)((b / nb2) % nb1);
     const int bx = (int)(b / ((int64_t)nb2 * nb1));
-    const int lane = (int)__lane_id();
-    for (int l = threadIdx.x; l < BIN_CELLS; l += blockDim.x) {
+    const int lane = (int)__lane_id(), wave = (int)(threadIdx.x >> 6), nw = (int)(blockDim.x >> 6);
+    __shared__ uint32_t s_wn[16];
+    int64_t base = off[b];
+    for (int l0 = 0; l0 < BIN_CELLS; l0 += blockDim.x) {
+        const int l = l0 + (int)threadIdx.x;
         const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
-        const bool live = X < n0 && Y < n1 && Z < n2 && sflag[((int64_t)X * n1 + Y) * n2 + Z] != 0;
+        const bool live = l < BIN_CELLS && X < n0 && Y < n1 && Z < n2 &&
+                          sflag[((int64_t)X * n1 + Y) * n2 + Z] != 0;
         const unsigned long long m = __ballot(live);
-        if (m == 0ull) continue;
-        unsigned long long base = 0;
-        const int leader = __ffsll((long long)m) - 1;
-        if (lane == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
-        base = __shfl(base, leader);
+        if (lane == 0) s_wn[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        int64_t wb = base;
+        for (int w = 0; w < wave; ++w) wb += s_wn[w];
+        int64_t tot = 0;
+        for (int w = 0; w < nw; ++w) tot += s_wn[w];
         if (live) {
-            const unsigned long long pos = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
-            if ((int64_t)pos < cap) {
-                const int64_t c = rec_index(X, Y, Z, n1, n2);
-                list[pos] = (int32_t)c;
-                rpos[c] = (int32_t)pos;
-            }
+            const int64_t pos = wb + __popcll(m & ((1ull << lane) - 1ull));
+            const int64_t c = rec_index(X, Y, Z, n1, n2);
+            list[pos] = (int32_t)c;
+            rpos[c] = (int32_t)pos;
         }
+        base += tot;
+        __syncthreads();
     }
 }
 
@@ -754,14 +782,14 @@ int diff_free(lbdem_psm* h) {
     D.lam[0] = D.lam[1] = nullptr;
     dfree_d(D.d_ft); dfree_d(D.gacc); dfree_d(D.Fscr); dfree_d(D.opart); dfree_d(D.gpart);
     dfree_d(D.cell_out); dfree_d(D.tau_part); dfree_d(D.pb); dfree_d(D.pc);
-    dfree_d(D.rec_list); dfree_d(D.d_nrec); dfree_d(D.rpos);
+    dfree_d(D.rec_list); dfree_d(D.bin_off); dfree_d(D.rpos);
     D.rpos = nullptr;
     D.pcap = 0;
     dfree_d(D.pp_last); dfree_d(D.pp_next); dfree_d(D.pp_cur);
     D.pp_last = D.pp_next = D.pp_cur = nullptr;
     D.pb = D.pc = nullptr;
     D.rec_list = nullptr;
-    D.d_nrec = nullptr;
+    D.bin_off = nullptr;
     D.nrec = 0;
     D.d_ft = D.gacc = D.Fscr = D.opart = D.gpart = D.cell_out = D.tau_part = nullptr;
     D.opart_n = D.tau_part_n = 0;
@@ -957,13 +985,30 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
     /* record slots of the frozen geometry, for the adjoint's first stage */
     if (!D.rec_list) PSM_HIP(h, dalloc_d(&D.rec_list, h->rec_cap * sizeof(int32_t)));
     if (!D.rpos) PSM_HIP(h, dalloc_d(&D.rpos, h->g.nc * sizeof(int32_t)));
-    if (!D.d_nrec) PSM_HIP(h, dalloc_d(&D.d_nrec, sizeof(unsigned long long)));
-    PSM_HIP(h, hipMemsetAsync(D.d_nrec, 0, sizeof(unsigned long long), h->stream));
+    if (!D.bin_off) PSM_HIP(h, dalloc_d(&D.bin_off, h->nbins * sizeof(int64_t)));
     PSM_HIP(h, hipMemsetAsync(D.rpos, 0xff, h->g.nc * sizeof(int32_t), h->stream));   /* -1: no record */
-    hipLaunchKernelGGL(k_list_records, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, h->sflag,
-                       h->bin_any[h->cur_any], h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2], D.rec_list,
-                       D.rpos, D.d_nrec, h->rec_cap);
-    PSM_HIP(h, hipGetLastError());
+    {
+        hipLaunchKernelGGL(k_count_records, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, h->sflag,
+                           h->bin_any[h->cur_any], h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2],
+                           D.bin_off);
+        PSM_HIP(h, hipGetLastError());
+        std::vector<int64_t> off((size_t)h->nbins);
+        PSM_HIP(h, hipMemcpyAsync(off.data(), D.bin_off, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost,
+                                  h->stream));
+        PSM_HIP(h, hipStreamSynchronize(h->stream));
+        int64_t tot = 0;
+        for (auto& o : off) { const int64_t c = o; o = tot; tot += c; }
+        if (tot > h->rec_cap) return LBDEM_PSM_ERR_CAPACITY;
+        D.nrec = tot;
+        PSM_HIP(h, hipMemcpyAsync(D.bin_off, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                                  h->stream));
+        hipLaunchKernelGGL(k_list_records, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, h->sflag,
+                           h->bin_any[h->cur_any], h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2],
+                           (const int64_t*)D.bin_off, D.rec_list, D.rpos);
+        PSM_HIP(h, hipGetLastError());
+        /* the host copy of the offsets is released only after the upload */
+        PSM_HIP(h, hipStreamSynchronize(h->stream));
+    }
     for (int t = 0, nextc = 0; t < nsteps; ++t) {
         if (nextc < ncp && t == t0[nextc]) {
             PSM_HIP(h, hipMemcpyAsync(D.ckpt[nextc], h->f[h->cur_f], h->pop_bytes, hipMemcpyDeviceToDevice,
@@ -979,10 +1024,7 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
                                       hipMemcpyDeviceToDevice, h->stream));
         }
     }
-    unsigned long long nrec = 0;
-    PSM_HIP(h, hipMemcpyAsync(&nrec, D.d_nrec, sizeof(nrec), hipMemcpyDeviceToHost, h->stream));
     PSM_HIP(h, hipStreamSynchronize(h->stream));
-    D.nrec = std::min<int64_t>((int64_t)nrec, h->rec_cap);
     if (ft && np > 0) {
         std::vector<double> tmp((size_t)nsteps * np * 6);
         PSM_HIP(h, hipMemcpy(tmp.data(), D.d_ft, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));

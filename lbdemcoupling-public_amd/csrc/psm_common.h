@@ -479,7 +479,7 @@ struct lbdem_psm {
         void* pp_last = nullptr;     /* periodic pressure: inlet/outlet planes of f(T) */
         void* pp_next = nullptr;     /* ... of the state after the current segment */
         void* pp_cur = nullptr;      /* ... of f(t+1) inside a segment */
-        unsigned long long* d_nrec = nullptr;
+        int64_t* bin_off = nullptr;  /* per bin: record count, then the bin's first list position */
         int64_t nrec = 0;
     } diff;
 
