@@ -258,6 +258,7 @@ struct AdjArgs {
     void* pb;                   /* 19 per record (SoA, arithmetic type): stage-1 adjoint of fP */
     void* pc;                   /* 5 per record (SoA): coef, gr, gj[3] */
     const int32_t* rpos;        /* per cell: position in the record list */
+    const int32_t* rown;        /* per list position: the record's particle index, or -1 */
     int64_t pcap;
 };
 
@@ -314,10 +315,17 @@ __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::
                                                     const typename Store<PREC>::S* __restrict__ lin,
                                                     const AdjArgs& a, int64_t r, int64_t p) {
     typedef typename Store<PREC>::T T;
+    /* everything that depends only on the list position is loaded first,
+     * in flight with the gathers: the particle index and the accumulators */
+    const int o = a.rown[p];
+    const int64_t P = a.pcap;
+    double* g = a.gacc;
+    double gacc[7];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) gacc[m] = g[m * P + p];
     int x, y, z;
     rec_cell(r, a.ny, a.nz, x, y, z);
     const int64_t row = (int64_t)x * a.ny + y;
-    if (!a.sflag[r]) return;
     const T omega = T(a.omega);
     const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};
     T lam[Q], fP[Q], pb[Q];
@@ -332,8 +340,6 @@ __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::
     double gT[3] = {0.0, 0.0, 0.0};
     bool obj = false;
     if (a.gft) {
-        const int id = a.rid[r];
-        const int o = (id >= 1 && id < a.id_map_size) ? a.id_map[id] : -1;
         if (o >= 0) {
             obj = true;
             const double d[3] = {min_image((double)x - a.xs[3 * o], a.nx),
@@ -351,7 +357,6 @@ __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::
     blend_vjp<T, FORCE>(fP, lam, pb, omega, acc, full, B, us, gF, Bb, usb, F, coef, gr, gj);
     T* pbo = (T*)a.pb;
     T* pco = (T*)a.pc;
-    const int64_t P = a.pcap;
 #pragma unroll
     for (int i = 0; i < Q; ++i) pbo[i * P + p] = pb[i];
     pco[p] = coef;
@@ -359,16 +364,15 @@ __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::
     pco[2 * P + p] = gj[0];
     pco[3 * P + p] = gj[1];
     pco[4 * P + p] = gj[2];
-    double* g = a.gacc;
-    if (!full) g[p] += (double)Bb;
-    g[P + p] += (double)usb[0];
-    g[2 * P + p] += (double)usb[1];
-    g[3 * P + p] += (double)usb[2];
+    if (!full) g[p] = gacc[0] + (double)Bb;
+    g[P + p] = gacc[1] + (double)usb[0];
+    g[2 * P + p] = gacc[2] + (double)usb[1];
+    g[3 * P + p] = gacc[3] + (double)usb[2];
     if (obj) {
         const double Fx = F[0], Fy = F[1], Fz = F[2];
-        g[4 * P + p] += Fy * gT[2] - Fz * gT[1];
-        g[5 * P + p] += Fz * gT[0] - Fx * gT[2];
-        g[6 * P + p] += Fx * gT[1] - Fy * gT[0];
+        g[4 * P + p] = gacc[4] + (Fy * gT[2] - Fz * gT[1]);
+        g[5 * P + p] = gacc[5] + (Fz * gT[0] - Fx * gT[2]);
+        g[6 * P + p] = gacc[6] + (Fx * gT[1] - Fy * gT[0]);
     }
 }
 
@@ -415,7 +419,10 @@ __global__ __launch_bounds__(256) void k_count_records(const uint8_t* __restrict
 __global__ __launch_bounds__(256) void k_list_records(const uint8_t* __restrict__ sflag,
                                                       const uint32_t* __restrict__ bin_live, int n0, int n1,
                                                       int n2, int nb1, int nb2, const int64_t* __restrict__ off,
-                                                      int32_t* __restrict__ list, int32_t* __restrict__ rpos) {
+                                                      const int32_t* __restrict__ rid,
+                                                      const int32_t* __restrict__ id_map, int64_t id_map_size,
+                                                      int32_t* __restrict__ list, int32_t* __restrict__ rpos,
+                                                      int32_t* __restrict__ rown) {
     const int64_t b = blockIdx.x;
     if (bin_live[b] == 0u) return;
     const int bz = (int)(b % nb2);
@@ -441,6 +448,8 @@ __global__ __launch_bounds__(256) void k_list_records(const uint8_t* __restrict_
             const int64_t c = rec_index(X, Y, Z, n1, n2);
             list[pos] = (int32_t)c;
             rpos[c] = (int32_t)pos;
+            const int id = rid[c];
+            rown[pos] = (id >= 1 && id < id_map_size) ? id_map[id] : -1;
         }
         base += tot;
         __syncthreads();
@@ -782,7 +791,7 @@ int diff_free(lbdem_psm* h) {
     D.lam[0] = D.lam[1] = nullptr;
     dfree_d(D.d_ft); dfree_d(D.gacc); dfree_d(D.Fscr); dfree_d(D.opart); dfree_d(D.gpart);
     dfree_d(D.cell_out); dfree_d(D.tau_part); dfree_d(D.pb); dfree_d(D.pc);
-    dfree_d(D.rec_list); dfree_d(D.bin_off); dfree_d(D.rpos);
+    dfree_d(D.rec_list); dfree_d(D.bin_off); dfree_d(D.rpos); dfree_d(D.rown);
     D.rpos = nullptr;
     D.pcap = 0;
     dfree_d(D.pp_last); dfree_d(D.pp_next); dfree_d(D.pp_cur);
@@ -790,6 +799,7 @@ int diff_free(lbdem_psm* h) {
     D.pb = D.pc = nullptr;
     D.rec_list = nullptr;
     D.bin_off = nullptr;
+    D.rown = nullptr;
     D.nrec = 0;
     D.d_ft = D.gacc = D.Fscr = D.opart = D.gpart = D.cell_out = D.tau_part = nullptr;
     D.opart_n = D.tau_part_n = 0;
@@ -826,6 +836,7 @@ static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lo
     a.pb = h->diff.pb;
     a.pc = h->diff.pc;
     a.rpos = h->diff.rpos;
+    a.rown = h->diff.rown;
     a.pcap = h->diff.pcap;
     const int64_t nrec = h->diff.nrec;
     dim3 gridr((unsigned)std::max<int64_t>(1, (nrec + 255) / 256)), blockr(256);
@@ -986,6 +997,7 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
     if (!D.rec_list) PSM_HIP(h, dalloc_d(&D.rec_list, h->rec_cap * sizeof(int32_t)));
     if (!D.rpos) PSM_HIP(h, dalloc_d(&D.rpos, h->g.nc * sizeof(int32_t)));
     if (!D.bin_off) PSM_HIP(h, dalloc_d(&D.bin_off, h->nbins * sizeof(int64_t)));
+    if (!D.rown) PSM_HIP(h, dalloc_d(&D.rown, h->rec_cap * sizeof(int32_t)));
     PSM_HIP(h, hipMemsetAsync(D.rpos, 0xff, h->g.nc * sizeof(int32_t), h->stream));   /* -1: no record */
     {
         hipLaunchKernelGGL(k_count_records, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, h->sflag,
@@ -1004,7 +1016,8 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
                                   h->stream));
         hipLaunchKernelGGL(k_list_records, dim3((unsigned)h->nbins), dim3(256), 0, h->stream, h->sflag,
                            h->bin_any[h->cur_any], h->g.n[0], h->g.n[1], h->g.n[2], h->g.nb[1], h->g.nb[2],
-                           (const int64_t*)D.bin_off, D.rec_list, D.rpos);
+                           (const int64_t*)D.bin_off, h->rec.id, h->id_map, h->id_map_size, D.rec_list,
+                           D.rpos, D.rown);
         PSM_HIP(h, hipGetLastError());
         /* the host copy of the offsets is released only after the upload */
         PSM_HIP(h, hipStreamSynchronize(h->stream));

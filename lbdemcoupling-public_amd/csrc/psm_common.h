@@ -473,6 +473,7 @@ struct lbdem_psm {
         void* pc = nullptr;          /* 5 per record (list position): coefficients of the BGK adjoint */
         int32_t* rec_list = nullptr; /* record slots of the frozen geometry */
         int32_t* rpos = nullptr;     /* per cell: position in rec_list, -1 without a record */
+        int32_t* rown = nullptr;     /* per list position: the record's particle index, -1 for none */
         int64_t pcap = 0;            /* entries of gacc, pb, pc (the record count) */
         size_t ptsz = 0;             /* arithmetic-type bytes of pb, pc */
         bool pp = false;             /* the window ran with the periodic-pressure rescale */
