@@ -1031,12 +1031,16 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
         rc = launch_collide_stream(h);
         if (rc) return rc;
         if (np > 0) {
-            rc = run_force_sum(h, h->psum_x);
+            /* straight into the window's force history */
+            rc = run_force_sum(h, h->psum_x, D.d_ft + (size_t)t * np * 6);
             if (rc) return rc;
-            PSM_HIP(h, hipMemcpyAsync(D.d_ft + (size_t)t * np * 6, h->pft, np * 6 * sizeof(double),
-                                      hipMemcpyDeviceToDevice, h->stream));
         }
     }
+    /* the handle's force buffer holds the last step's sums, as after a
+     * coupled step */
+    if (np > 0 && nsteps > 0)
+        PSM_HIP(h, hipMemcpyAsync(h->pft, D.d_ft + (size_t)(nsteps - 1) * np * 6, np * 6 * sizeof(double),
+                                  hipMemcpyDeviceToDevice, h->stream));
     PSM_HIP(h, hipStreamSynchronize(h->stream));
     if (ft && np > 0) {
         std::vector<double> tmp((size_t)nsteps * np * 6);

@@ -522,7 +522,9 @@ int run_voxelize(lbdem_psm* h);
 int mirror_records(lbdem_psm* h);   /* float view of the live records (FP32/FP16 storage) */
 int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_r, double* d_sf,
                    int32_t* d_cnt, int32_t* d_br);
-int run_force_sum(lbdem_psm* h, const double* xs);   /* xs: positions for the torque arms */
+/* xs: positions for the torque arms; out: 6 per particle (F, T), the
+ * handle's own buffer when null */
+int run_force_sum(lbdem_psm* h, const double* xs, double* out = nullptr);
 int count_records(lbdem_psm* h);
 int launch_advance(lbdem_psm* h);
 int launch_define_velocity(lbdem_psm* h);

@@ -236,7 +236,8 @@ __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_
     }
 }
 
-int run_force_sum(lbdem_psm* h, const double* xs) {
+int run_force_sum(lbdem_psm* h, const double* xs, double* out) {
+    double* pft = out ? out : h->pft;
     hipEvent_t ev = nullptr;
     prof_begin(h, 2, &ev);
     ForceArgs a;
@@ -258,9 +259,9 @@ int run_force_sum(lbdem_psm* h, const double* xs) {
     else hipLaunchKernelGGL(k_force_bin<double>, grid, dim3(256), 0, h->stream, a);
     if (h->np > 0) {
         hipLaunchKernelGGL(k_force_particle, dim3((unsigned)((h->np + 3) / 4)), dim3(256), 0, h->stream,
-                           h->np, h->p_ent_start, h->p_nent, h->owner_ent, h->partial, h->pft, h->ent_cap);
+                           h->np, h->p_ent_start, h->p_nent, h->owner_ent, h->partial, pft, h->ent_cap);
         hipLaunchKernelGGL(k_force_orphans, dim3(64), dim3(256), 0, h->stream, a, h->orphan, h->counters,
-                           h->orphan_cap, h->id_map, h->id_map_size, h->pft);
+                           h->orphan_cap, h->id_map, h->id_map_size, pft);
     }
     PSM_HIP(h, hipGetLastError());
     prof_end(h, 2, ev);
