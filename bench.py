@@ -141,14 +141,21 @@ def run_adjoint(precision):
                     body_accel=c["body"], voxel_mode="overlap")
     try:
         lat.set_spheres(x, v, w, r, ids)
-        lat.diff_forward(4, 2)            # warm-up of both passes (allocations, code load)
-        lat.diff_backward(np.zeros((4, len(r), 6)))
+        # The first window of a shape also allocates the handle's state pool
+        # (10 checkpoints and 49 recompute states); it is timed on its own.
+        # The timed window is the next one of the same shape, as in a loop of
+        # windows, where the pool is reused.
         lat.synchronize()
+        c0 = time.perf_counter()
+        ft = lat.diff_forward(c["steps"], c["checkpoint_every"])
+        c1 = time.perf_counter()
+        G = np.zeros_like(ft)
+        G[:, :, 2] = 1.0
+        lat.diff_backward(G)
+        c2 = time.perf_counter()
         t0 = time.perf_counter()
         ft = lat.diff_forward(c["steps"], c["checkpoint_every"])
         t1 = time.perf_counter()
-        G = np.zeros_like(ft)
-        G[:, :, 2] = 1.0
         g = lat.diff_backward(G)
         t2 = time.perf_counter()
         # the plain coupled step of the same geometry (no checkpoints), for the cost ratio
@@ -167,6 +174,7 @@ def run_adjoint(precision):
     return {"forward_s": round(fwd, 4), "backward_s": round(bwd, 4), "plain_forward_s": round(plain, 4),
             "backward_over_forward": round(bwd / fwd, 3),
             "total_over_plain_forward": round((fwd + bwd) / plain, 3),
+            "first_window_forward_s": round(c1 - c0, 4), "first_window_backward_s": round(c2 - c1, 4),
             "drag_z_total": float(ft[:, :, 2].sum()),
             "grad_tau": g["tau"], "grad_v_norm": float(np.linalg.norm(g["v"])),
             "grad_x_norm": float(np.linalg.norm(g["x"])), "grad_w_norm": float(np.linalg.norm(g["w"])),
@@ -390,6 +398,8 @@ TRACE_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace -o run -- "
              "python3 bench.py --steps 10 --warmup 2 --repeats 1 --no-extra --no-adjoint --cpu-steps 1")
 TRACE512_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace512 -o run -- "
                 "python3 bench.py --workload channel512 --steps 4 --warmup 1 --repeats 1 --no-adjoint --cpu-steps 0")
+TRACEADJ_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace_adj -o run -- "
+                "python3 bench.py --steps 2 --warmup 1 --repeats 1 --no-extra --cpu-steps 0")
 
 
 def _short(name):
@@ -468,7 +478,15 @@ def summarize_profiles(d):
              "The run does 1 + 2 + 10 coupled steps per precision (FP64, FP32-dev, FP16-store)."),
             ("trace512", "kernels_channel512_trace.md",
              "channel 512x512x768 (100000 spheres r~U[3,6], walls x/y, tau 0.55, 8^3 samples)", TRACE512_CMD,
-             "The run does 1 + 1 + 4 coupled steps per precision (FP64, FP32-dev, FP16-store).")):
+             "The run does 1 + 1 + 4 coupled steps per precision (FP64, FP32-dev, FP16-store)."),
+            ("trace_adj", "kernels_adjoint128_trace.md",
+             "differentiable channel 128x128x256 (200 spheres r=5, walls x/y, body force along z, "
+             "analytic overlap)", TRACEADJ_CMD,
+             "The run is the adjoint line of the default bench: per precision (FP64, FP32-dev) a 4-step "
+             "warm-up window, then a 500-step window forward (checkpoint every 50) and its backward "
+             "pass, then 502 plain coupled steps. Three coupled steps per precision of the 256^3 "
+             "suspension come first; their kernels are the k_fold and k_collide_stream rows without "
+             "body force.")):
         dbs = glob.glob(os.path.join(d, sub, "*results.db"))
         if not dbs:
             continue
