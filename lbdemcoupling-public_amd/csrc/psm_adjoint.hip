@@ -1265,12 +1265,9 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
         /* omega = 1/tau, B depends on tau - 1/2 */
         *g_tau = -hred[0] * (h->omega * h->omega) + hred[1];
     }
-    if (g_sf || g_B) {
-        std::vector<double> co(2 * (size_t)nc);
-        PSM_HIP(h, hipMemcpy(co.data(), D.cell_out, co.size() * sizeof(double), hipMemcpyDeviceToHost));
-        if (g_sf) std::copy(co.begin(), co.begin() + nc, g_sf);
-        if (g_B) std::copy(co.begin() + nc, co.end(), g_B);
-    }
+    /* the per-cell gradients go straight into the caller's arrays */
+    if (g_sf) PSM_HIP(h, hipMemcpy(g_sf, D.cell_out, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost));
+    if (g_B) PSM_HIP(h, hipMemcpy(g_B, D.cell_out + nc, (size_t)nc * sizeof(double), hipMemcpyDeviceToHost));
     if (g_x3 || g_v3 || g_w3) {
         std::vector<double> gp(9 * (size_t)np);
         if (np > 0) PSM_HIP(h, hipMemcpy(gp.data(), D.gpart, gp.size() * sizeof(double), hipMemcpyDeviceToHost));
