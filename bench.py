@@ -638,9 +638,13 @@ def main():
         log(f"channel512 fp32_dev: {r32['mlups']:.0f} MLUPS, kernel {r32['kernel_gbs']:.0f} GB/s")
         r64 = run_gpu(wl5, "fp64", steps5, max(2, args.warmup // 2), sph5, want_field=True)
         log(f"channel512 fp64: {r64['mlups']:.0f} MLUPS")
+        r16 = run_gpu(wl5, "fp16_store", steps5, max(2, args.warmup // 2), sph5, want_field=True)
+        log(f"channel512 fp16_store: {r16['mlups']:.0f} MLUPS, kernel {r16['kernel_gbs']:.0f} GB/s")
         e32 = summary(r32, r64)
+        e16 = summary(r16, r64)
+        del r16
         extra["channel512"] = {"workload": wl5["desc"], "steps": steps5,
-                               "fp32_dev": e32,
+                               "fp32_dev": e32, "fp16_store": e16,
                                "fp64": {"mlups": round(r64["mlups"], 1),
                                         "kernel_gbs": round(r64["kernel_gbs"], 1),
                                         "max_u": r64["max_u"], "health_rc": r64["health_rc"]},
