@@ -1,17 +1,28 @@
 """Benchmark of the MI355X PSM hot path (D3Q19 Noble-Torczynski collide+stream,
 sphere voxelization, per-particle force/torque sum) on one GPU.
 
-Default workload: the periodic dense suspension -- 256^3 D3Q19, tau 0.7,
-2000 non-overlapping spheres with radii U[4,8] (seeded rejection sampling),
-velocities N(0, 0.01) and angular velocities N(0, 1e-3) per component,
-8^3 sub-cell sampling, prescribed kinematics (x += v each step), FP64.
-One step = voxelize + fused collide/stream + force/torque sum, all
-resident in HBM.  `value` is whole-step MLUPS of that FP64 run; the FP32
-deviation and FP16-storage variants of the same run are reported beside it
-with their drag, torque and velocity-field error against FP64, and so are
-the 512x512x768 channel (FP32 deviation and FP64), the 384^3 polydisperse
-suspension in all three precisions, and the differentiable 128x128x256
-channel (forward window and hand-written adjoint, FP64 and FP32).
+Headline workload: the production-scale rectangular channel -- 512x512x768
+D3Q19, walls in x/y, pressure-driven along z (Zhang-Kwok periodic pressure
+from a Poiseuille initial field), tau 0.55, 100000 non-overlapping spheres
+with radii U[3,6] filling ~30 % of the volume, 8^3 sub-cell sampling,
+prescribed kinematics (x += v each step), FP64.  One step = voxelize + fused
+collide/stream + force/torque sum, all resident in HBM.  `value` is the
+whole-step MLUPS of the FP64 run, the median over fresh lattices (each
+lattice's memory placement sets its speed for its lifetime, +-8 % between
+placements).  The FP32-deviation and FP16-storage runs of the same workload
+are reported beside it, each with its own roofline, and with drag, torque
+and velocity-field errors against the FP64 run of the same seed after the
+configuration's 200 steps (norm-wise and per particle).  Also reported: the
+512^3 periodic box at the same sphere density, the 256^3 dense suspension,
+the 384^3 polydisperse suspension over its 1000 steps, the differentiable
+128x128x256 channel and the adjoint of the last 50 steps of the channel,
+the host-driven exchange loop (set_spheres -> collide_stream ->
+sum_force_torque through the C ABI) beside the resident step, and the
+device copy ceiling of the population store.
+
+cpu_baseline: the CPU restatement of the reference algorithm on a slab of
+the headline channel (512x512x96 with its sphere density), all host threads
+(x-slabs, as the reference's MPI ranks), plus one step on one core.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -34,27 +45,45 @@ import lbdem_psm as L  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 METRIC = "MLUPS of PSM step (FP64 vs FP32-dev) + %HBM roofline; drag rel-err vs FP64"
 BYTES_PER_POP = {"fp64": 8, "fp32_dev": 4, "fp16_store": 2}
+PRECISIONS = ("fp64", "fp32_dev", "fp16_store")
+KERNEL = {"fp64": "k_collide_stream<FP64>", "fp32_dev": "k_collide_stream<FP32_DEV>",
+          "fp16_store": "k_collide_stream<FP16_STORE>"}
 
 WORKLOADS = {
-    "dense256": dict(n=(256, 256, 256), periodic=(1, 1, 1), tau=0.7, count=2000,
-                     law=L.RADIUS_UNIFORM, a=4.0, b=8.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
-                     desc="periodic dense suspension 256^3, tau 0.7, 2000 spheres r~U[4,8]"),
-    "poly384": dict(n=(384, 384, 384), periodic=(1, 1, 1), tau=0.6, count=20000,
-                    law=L.RADIUS_LOGNORMAL, a=5.0, b=0.25, lo=3.0, hi=10.0, body=(0.0, 0.0, 0.0),
-                    desc="periodic polydisperse 384^3, tau 0.6, 20000 spheres lognormal(5,0.25) in [3,10]"),
     "channel512": dict(n=(512, 512, 768), periodic=(0, 0, 1), tau=0.55, count=100000,
                        law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
                        pressure=dict(dim=2, rho_hi=1.0, rho_lo=1.0 - 1e-4, u_max=0.05),
                        # slow prescribed kinematics (0.4 lu in 200 steps) keep the spheres apart
                        # and off the walls: with v ~ N(0, 0.01) they pass through each other and
                        # the walls within 200 steps and the flow leaves the stable range
-                       sigma_v=0.002, sigma_w=2e-4, wall_gap=2.0,
+                       sigma_v=0.002, sigma_w=2e-4, wall_gap=2.0, full_steps=200,
                        desc="channel 512x512x768, walls x/y, pressure-driven along z (Zhang-Kwok "
                             "periodic pressure rho 1 -> 1-1e-4, Poiseuille initial field u_max 0.05), "
                             "tau 0.55, 100000 spheres r~U[3,6], v~N(0,0.002), w~N(0,2e-4)"),
+    "channel512_slab": dict(n=(512, 512, 96), periodic=(0, 0, 1), tau=0.55, count=12500,
+                            law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
+                            pressure=dict(dim=2, rho_hi=1.0, rho_lo=1.0 - 1.25e-5, u_max=0.05),
+                            sigma_v=0.002, sigma_w=2e-4, wall_gap=2.0,
+                            desc="slab 512x512x96 of the channel (1/8 of its length, same sphere "
+                                 "density and pressure gradient): 12500 spheres r~U[3,6]"),
     "cube512": dict(n=(512, 512, 512), periodic=(1, 1, 1), tau=0.55, count=66667,
                     law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
-                    desc="periodic 512^3, tau 0.55, 66667 spheres r~U[3,6] (channel512 density)"),
+                    sigma_v=0.002, sigma_w=2e-4, full_steps=200,
+                    desc="periodic 512^3, tau 0.55, 66667 spheres r~U[3,6] (the channel's sphere "
+                         "density), v~N(0,0.002), w~N(0,2e-4)"),
+    "dense256": dict(n=(256, 256, 256), periodic=(1, 1, 1), tau=0.7, count=2000,
+                     law=L.RADIUS_UNIFORM, a=4.0, b=8.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
+                     desc="periodic dense suspension 256^3, tau 0.7, 2000 spheres r~U[4,8], "
+                          "v~N(0,0.01), w~N(0,1e-3)"),
+    "poly384": dict(n=(384, 384, 384), periodic=(1, 1, 1), tau=0.6, count=20000,
+                    law=L.RADIUS_LOGNORMAL, a=5.0, b=0.25, lo=3.0, hi=10.0, body=(0.0, 0.0, 0.0),
+                    # over its 1000 steps the configuration's v ~ N(0, 0.01) carries the
+                    # spheres through each other and the flow past |u| = 0.3 after ~200
+                    # steps (FP64 as well); slower kinematics (1 lu in 1000 steps) keep it
+                    # stable (the packing leaves no room for a gap between the spheres)
+                    sigma_v=0.001, sigma_w=1e-4, full_steps=1000,
+                    desc="periodic polydisperse 384^3, tau 0.6, 20000 spheres lognormal(5,0.25) in "
+                         "[3,10], v~N(0,0.001), w~N(0,1e-4), 1000 steps"),
 }
 
 
@@ -65,22 +94,30 @@ def log(msg):
 def make_spheres(wl, seed=2024):
     return L.generate_spheres(wl["n"], wl["count"], wl["law"], a=wl["a"], b=wl["b"], lo=wl["lo"],
                               hi=wl["hi"] if wl["hi"] > 0 else 1e30, periodic=wl["periodic"],
-                              gap=0.0, wall_gap=wl.get("wall_gap", 1.0), sigma_v=wl.get("sigma_v", 0.01),
-                              sigma_w=wl.get("sigma_w", 1e-3), seed=seed,
-                              max_attempts=10 ** 8)
+                              gap=wl.get("gap", 0.0), wall_gap=wl.get("wall_gap", 1.0),
+                              sigma_v=wl.get("sigma_v", 0.01), sigma_w=wl.get("sigma_w", 1e-3),
+                              seed=seed, max_attempts=10 ** 8)
 
 
-def run_gpu(wl, precision, steps, warmup, spheres, slices=8, want_field=False):
-    n = wl["n"]
-    nc = int(np.prod(n))
+def new_lattice(wl, precision, voxel_mode="subcell"):
+    lat = L.Lattice(*wl["n"], periodic=wl["periodic"], tau=wl["tau"], precision=precision,
+                    slices_per_dim=8, body_accel=wl["body"], voxel_mode=voxel_mode)
+    pr = wl.get("pressure")
+    if pr:
+        lat.init_poiseuille(pr["dim"], pr["rho_hi"], pr["rho_lo"], pr["u_max"])
+        lat.set_periodic_pressure(pr["dim"], pr["rho_hi"], pr["rho_lo"])
+    return lat
+
+
+def run_gpu(wl, precision, steps, warmup, spheres, full_steps=0, copy_reps=0):
+    """warmup + `steps` timed coupled steps on a fresh lattice.  full_steps >
+    0: continue to that many steps in all, then read the particle forces and
+    the velocity field (for the error against FP64).  copy_reps > 0: the
+    copy ceiling of this lattice's population store afterwards."""
+    nc = int(np.prod(wl["n"]))
     x, v, w, r, ids = spheres
-    lat = L.Lattice(*n, periodic=wl["periodic"], tau=wl["tau"], precision=precision,
-                    slices_per_dim=slices, body_accel=wl["body"])
+    lat = new_lattice(wl, precision)
     try:
-        pr = wl.get("pressure")
-        if pr:
-            lat.init_poiseuille(pr["dim"], pr["rho_hi"], pr["rho_lo"], pr["u_max"])
-            lat.set_periodic_pressure(pr["dim"], pr["rho_hi"], pr["rho_lo"])
         lat.set_spheres(x, v, w, r, ids)
         lat.step(warmup, advance=True)
         lat.synchronize()
@@ -92,36 +129,149 @@ def run_gpu(wl, precision, steps, warmup, spheres, slices=8, want_field=False):
         t1 = time.perf_counter()
         prof = lat.profile_read(reset=True)
         lat.profile(False)
-        F, T = lat.particle_forces()
         st = lat.stats()
-        rc, umax, nonfin = lat.check_health()
-        field = lat.macroscopic()[1] if want_field else None
+        out = {}
+        if full_steps > 0:
+            rest = full_steps - warmup - steps
+            if rest > 0:
+                lat.step(rest, advance=True)
+            F, T = lat.particle_forces()
+            rc, umax, _ = lat.check_health()
+            out.update(F=F, T=T, u=lat.macroscopic()[1], max_u=umax, health_rc=rc, steps_total=max(full_steps, warmup + steps))
+        if copy_reps > 0:
+            ms, nbytes = lat.copy_bandwidth(copy_reps)
+            out["copy_gbs"] = nbytes / (ms * 1e-3) / 1e9
     finally:
         lat.close()
     wall = t1 - t0
     cs_ms, cs_n = prof["collide_stream"]
     cs_avg_s = cs_ms / max(cs_n, 1) * 1e-3
     b_lu = 2 * 19 * BYTES_PER_POP[precision]
-    return {
+    out.update({
         "precision": precision,
         "mlups": nc * steps / wall / 1e6,
         "ms_per_step": wall / steps * 1e3,
-        "kernel_mlups": nc / cs_avg_s / 1e6,
         "kernel_ms": cs_avg_s * 1e3,
         "kernel_gbs": b_lu * nc / cs_avg_s / 1e9,
         "voxelize_ms": prof["voxelize"][0] / max(prof["voxelize"][1], 1),
         "force_ms": prof["force_sum"][0] / max(prof["force_sum"][1], 1),
         "advance_ms": prof["advance"][0] / max(prof["advance"][1], 1),
         "bytes_per_lu": b_lu,
-        "records": st["records"],
-        "images": st["images"],
-        "orphans": st["orphans"],
-        "max_u": umax,
-        "health_rc": rc,
-        "F": F,
-        "T": T,
-        "u": field,
-    }
+        "nc": nc,
+        "records": st["records"], "images": st["images"], "orphans": st["orphans"],
+    })
+    return out
+
+
+def run_line(wl, precision, steps, warmup, spheres, repeats, full_steps=0, copy_reps=0):
+    """`repeats` fresh lattices; the median repetition by step rate, with the
+    forces and fields of the first (identical in every repetition)."""
+    runs = []
+    for k in range(max(1, repeats)):
+        runs.append(run_gpu(wl, precision, steps, warmup, spheres, full_steps if k == 0 else 0,
+                            copy_reps if k == 0 else 0))
+        log(f"  {precision} rep {k + 1}: {runs[-1]['mlups']:.0f} MLUPS step, collide "
+            f"{runs[-1]['kernel_ms']:.3f} ms ({runs[-1]['kernel_gbs']:.0f} GB/s), voxelize "
+            f"{runs[-1]['voxelize_ms']:.3f} ms, force {runs[-1]['force_ms']:.3f} ms")
+    med = dict(sorted(runs, key=lambda rr: rr["mlups"])[len(runs) // 2])
+    for key in ("F", "T", "u", "max_u", "health_rc", "copy_gbs", "steps_total"):
+        if key in runs[0]:
+            med[key] = runs[0][key]
+    med["runs_mlups"] = [round(rr["mlups"], 1) for rr in runs]
+    med["runs_kernel_ms"] = [round(rr["kernel_ms"], 4) for rr in runs]
+    return med
+
+
+def rel(a, b):
+    den = np.abs(b).max()
+    return float(np.abs(a - b).max() / den) if den > 0 else 0.0
+
+
+def particle_err(a, ref):
+    """Per-particle relative error |a_p - ref_p| / |ref_p| (vector norms), with
+    |ref_p| floored at 1e-3 of the median |ref|, so that particles whose load
+    is nearly zero do not divide by round-off."""
+    mag = np.linalg.norm(ref, axis=1)
+    floor = 1e-3 * float(np.median(mag))
+    e = np.linalg.norm(a - ref, axis=1) / np.maximum(mag, max(floor, 1e-300))
+    return {"median": float(np.median(e)), "p99": float(np.percentile(e, 99)), "max": float(e.max()),
+            "floor": floor}
+
+
+def field_err(u, ref):
+    """Relative L2 and Linf error of a velocity field against the FP64 one."""
+    if u is None or ref is None:
+        return None
+    d = u - ref
+    return {"l2": float(np.linalg.norm(d) / np.linalg.norm(ref)), "linf": float(np.abs(d).max() / np.abs(ref).max())}
+
+
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "collide_stream_traffic.json")
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of a kernel from the calibrated FETCH_SIZE /
+    WRITE_SIZE counters of that workload, or None."""
+    try:
+        t = json.load(open(TRAFFIC_JSON))
+    except (OSError, ValueError):
+        return None
+    v = t.get("workloads", {}).get(workload, {}).get("bytes_per_launch", {}).get(kernel)
+    return None if v is None else int(v)
+
+
+def roofline(rr, workload):
+    k = KERNEL[rr["precision"]]
+    return {"bound": "hbm", "achieved": round(rr["kernel_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(rr["kernel_gbs"] / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(workload, k),
+            "kernel": k,
+            "algorithmic_bytes_per_launch": rr["bytes_per_lu"] * rr["nc"],
+            "kernel_ms": round(rr["kernel_ms"], 4)}
+
+
+def summary(rr, ref, workload):
+    out = {"mlups": round(rr["mlups"], 1), "ms_per_step": round(rr["ms_per_step"], 4),
+           "runs_mlups": rr.get("runs_mlups"), "runs_kernel_ms": rr.get("runs_kernel_ms"),
+           "kernel_ms": round(rr["kernel_ms"], 4), "kernel_gbs": round(rr["kernel_gbs"], 1),
+           "roofline": roofline(rr, workload),
+           "step_roofline_frac": round(rr["mlups"] * 1e6 * rr["bytes_per_lu"] / 1e9 / HBM_PEAK_GBS, 4),
+           "voxelize_ms": round(rr["voxelize_ms"], 4), "force_ms": round(rr["force_ms"], 4),
+           "advance_ms": round(rr["advance_ms"], 4),
+           "records": rr["records"], "images": rr["images"], "orphans": rr["orphans"]}
+    if "copy_gbs" in rr:
+        out["device_copy_gbs"] = round(rr["copy_gbs"], 1)
+    if "F" in rr:
+        out["steps_compared"] = rr["steps_total"]
+        out["max_u"] = rr["max_u"]
+        out["health_rc"] = rr["health_rc"]
+        if ref is not None and rr is not ref:
+            out["drag_rel_err_vs_fp64"] = rel(rr["F"], ref["F"])
+            out["torque_rel_err_vs_fp64"] = rel(rr["T"], ref["T"])
+            out["drag_particle_rel_err_vs_fp64"] = particle_err(rr["F"], ref["F"])
+            out["torque_particle_rel_err_vs_fp64"] = particle_err(rr["T"], ref["T"])
+            out["velocity_err_vs_fp64"] = field_err(rr.get("u"), ref.get("u"))
+    return out
+
+
+def run_workload(key, steps, warmup, repeats, copy_reps=0, spheres=None):
+    """All three precisions of one workload; errors against its FP64 run."""
+    wl = WORKLOADS[key]
+    if spheres is None:
+        log(f"{key}: generating {wl['count']} spheres")
+        spheres = make_spheres(wl)
+    full = wl.get("full_steps", warmup + steps)
+    res = {}
+    for prec in PRECISIONS:
+        log(f"{key} {prec}: {repeats} x ({warmup} warmup + {steps} timed steps), errors after {full} steps")
+        res[prec] = run_line(wl, prec, steps, warmup, spheres, repeats, full_steps=full,
+                             copy_reps=copy_reps if prec == "fp64" else 0)
+    out = {"workload": wl["desc"], "cells": int(np.prod(wl["n"])), "spheres": wl["count"]}
+    for prec in PRECISIONS:
+        out[prec] = summary(res[prec], res["fp64"], key)
+    out["fp32_over_fp64_mlups"] = round(res["fp32_dev"]["mlups"] / res["fp64"]["mlups"], 3)
+    out["fp16_over_fp64_mlups"] = round(res["fp16_store"]["mlups"] / res["fp64"]["mlups"], 3)
+    return out, res, spheres
 
 
 ADJOINT_CASE = dict(n=(128, 128, 256), periodic=(0, 0, 1), tau=0.8, body=(0.0, 0.0, 1e-6), count=200,
@@ -158,13 +308,12 @@ def run_adjoint(precision):
         t1 = time.perf_counter()
         g = lat.diff_backward(G)
         t2 = time.perf_counter()
-        # the plain coupled step of the same geometry (no checkpoints), for the cost ratio
-        lat.collide_stream(2)
+        # the plain forward of the same frozen geometry: collide+stream and the
+        # device force sum per step, no checkpoints, no host copies
+        lat.step(2, frozen=True)
         lat.synchronize()
         t3 = time.perf_counter()
-        for _ in range(c["steps"]):
-            lat.collide_stream(1)
-            lat.sum_force_torque()
+        lat.step(c["steps"], frozen=True)
         lat.synchronize()
         t4 = time.perf_counter()
         lat.diff_release()
@@ -173,6 +322,7 @@ def run_adjoint(precision):
     fwd, bwd, plain = t1 - t0, t2 - t1, t4 - t3
     return {"forward_s": round(fwd, 4), "backward_s": round(bwd, 4), "plain_forward_s": round(plain, 4),
             "backward_over_forward": round(bwd / fwd, 3),
+            "forward_over_plain_forward": round(fwd / plain, 3),
             "total_over_plain_forward": round((fwd + bwd) / plain, 3),
             "first_window_forward_s": round(c1 - c0, 4), "first_window_backward_s": round(c2 - c1, 4),
             "drag_z_total": float(ft[:, :, 2].sum()),
@@ -182,7 +332,7 @@ def run_adjoint(precision):
 
 
 ADJOINT_C5 = dict(pre_steps=150, window=50, budget=10,
-                  desc="the 512x512x768 pressure-driven channel above in FP32-dev with the analytic "
+                  desc="the 512x512x768 pressure-driven channel in FP32-dev with the analytic "
                        "overlap: 150 coupled steps, then the adjoint of the last 50 with a budget of 10 "
                        "population states (segments of 10, 9, ... steps, each finished checkpoint reused "
                        "for recompute), d(total z-drag)/d(tau, v, w, x, sf); central differences of the "
@@ -209,20 +359,13 @@ def run_adjoint_c5(spheres):
     for _ in range(c["pre_steps"]):
         xw = L.advance_positions(xw, v, wl["n"], wl["periodic"])
 
-    def lattice(tau):
-        lat = L.Lattice(*wl["n"], periodic=wl["periodic"], tau=wl["tau"], precision="fp32_dev",
-                        slices_per_dim=8, voxel_mode="overlap")
-        lat.init_poiseuille(pr["dim"], pr["rho_hi"], pr["rho_lo"], pr["u_max"])
-        lat.set_periodic_pressure(pr["dim"], pr["rho_hi"], pr["rho_lo"])
-        lat.set_spheres(x, v, w, r, ids)
-        lat.step(c["pre_steps"], advance=True)
-        if tau != wl["tau"]:
-            lat.set_tau(tau)
-        return lat
-
     def objective(dtau=0.0, dvz=0.0, dwz=0.0):
-        lat = lattice(wl["tau"] + dtau)
+        lat = new_lattice(wl, "fp32_dev", voxel_mode="overlap")
         try:
+            lat.set_spheres(x, v, w, r, ids)
+            lat.step(c["pre_steps"], advance=True)
+            if dtau:
+                lat.set_tau(wl["tau"] + dtau)
             vv, ww = np.array(v, dtype=np.float64), np.array(w, dtype=np.float64)
             vv[:, 2] += dvz
             ww[:, 2] += dwz
@@ -232,14 +375,12 @@ def run_adjoint_c5(spheres):
             lat.close()
         return float(ft[:, :, 2].sum())
 
-    lat = L.Lattice(*wl["n"], periodic=wl["periodic"], tau=wl["tau"], precision="fp32_dev",
-                    slices_per_dim=8, voxel_mode="overlap")
+    lat = new_lattice(wl, "fp32_dev", voxel_mode="overlap")
     try:
-        lat.set_periodic_pressure(pr["dim"], pr["rho_hi"], pr["rho_lo"])
         lat.set_spheres(x, v, w, r, ids)
         # warm-up of both passes at the same budget (one segment: all M states):
         # allocates the state pool and the adjoint buffers, which the timed
-        # window reuses; then the same start as lattice() from a fresh history
+        # window reuses; then the same start as objective() from a fresh history
         lat.diff_forward(c["budget"], -c["budget"])
         lat.diff_backward(np.zeros((c["budget"], len(r), 6)), want_cells=False)
         lat.clear_records()
@@ -259,9 +400,7 @@ def run_adjoint_c5(spheres):
         lat.diff_release()
         lat.synchronize()
         t3 = time.perf_counter()
-        for _ in range(c["window"]):
-            lat.collide_stream(1)
-            lat.sum_force_torque()
+        lat.step(c["window"], frozen=True)
         lat.synchronize()
         t4 = time.perf_counter()
         state_gb = lat.nc * 19 * 4 / 1e9
@@ -279,6 +418,7 @@ def run_adjoint_c5(spheres):
     fwd, bwd, plain = t1 - t0, t2 - t1, t4 - t3
     return {"workload": c["desc"], "forward_s": round(fwd, 4), "backward_s": round(bwd, 4),
             "plain_forward_s": round(plain, 4), "backward_over_forward": round(bwd / fwd, 3),
+            "forward_over_plain_forward": round(fwd / plain, 3),
             "total_over_plain_forward": round((fwd + bwd) / plain, 3),
             "population_state_gb": round(state_gb, 2), "state_budget": c["budget"],
             "device_memory_used_after_backward_gb": mem[0] if mem else None,
@@ -292,114 +432,100 @@ def run_adjoint_c5(spheres):
                     "solid-fraction thresholds (0.001, 0.999), where the objective jumps"}
 
 
-def measured_traffic(workload, kernel):
+def run_exchange(key, precision, steps, warmup, spheres):
+    """The reference's coupling loop through the C ABI with host particle
+    arrays (setSpheresOnLattice -> collideAndStream -> getForcesFromLattice,
+    sedimentingSphere.cpp:212-225), against the resident step of the same
+    workload."""
+    wl = WORKLOADS[key]
+    nc = int(np.prod(wl["n"]))
+    x, v, w, r, ids = spheres
+    lat = new_lattice(wl, precision)
     try:
-        t = json.load(open(TRAFFIC_JSON))
-    except (OSError, ValueError):
-        return None
-    if t.get("workload") != workload:
-        return None
-    v = t.get("bytes_per_launch", {}).get(kernel)
-    return None if v is None else int(v)
+        xs = np.array(x, dtype=np.float64)
+
+        def one():
+            nonlocal xs
+            lat.set_spheres(xs, v, w, r, ids)
+            lat.collide_stream(1)
+            F, T = lat.sum_force_torque(xs)
+            xs = L.advance_positions(xs, v, wl["n"], wl["periodic"])
+            return F
+
+        for _ in range(warmup):
+            one()
+        lat.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            one()
+        lat.synchronize()
+        t1 = time.perf_counter()
+    finally:
+        lat.close()
+    return {"workload": wl["desc"], "precision": precision, "steps": steps,
+            "mlups": round(nc * steps / (t1 - t0) / 1e6, 1), "ms_per_step": round((t1 - t0) / steps * 1e3, 4),
+            "loop": "host arrays each step: set_spheres (upload, voxelize) -> collide_stream -> "
+                    "sum_force_torque (force sum, download, map(tag) fold on the host)"}
 
 
-def rel(a, b):
-    den = np.abs(b).max()
-    return float(np.abs(a - b).max() / den) if den > 0 else 0.0
-
-
-def field_err(u, ref):
-    """Relative L2 and Linf error of a velocity field against the FP64 one."""
-    if u is None or ref is None:
-        return None
-    d = u - ref
-    return {"l2": float(np.linalg.norm(d) / np.linalg.norm(ref)), "linf": float(np.abs(d).max() / np.abs(ref).max())}
-
-
-def cpu_baseline(wl, spheres, nsteps):
+def cpu_baseline(nsteps):
+    """The CPU restatement of the reference algorithm on a slab of the headline
+    channel: every step voxelize + collide/stream + force sum, on all host
+    threads (x-slabs, as the reference's MPI ranks), then one step on one core."""
     from oracle import oracle as O
+    wl = WORKLOADS["channel512_slab"]
     n = wl["n"]
     nc = int(np.prod(n))
-    x, v, w, r, ids = spheres
-    o = O.OracleLattice(*n, periodic=wl["periodic"], tau=wl["tau"], slices_per_dim=8,
-                        body_accel=wl["body"])
-    o.init_equilibrium()
+    x, v, w, r, ids = make_spheres(wl)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    for s in range(nsteps):
-        if s:
-            x = L.advance_positions(x, v, n, wl["periodic"])
+    o = O.OracleLattice(*n, periodic=wl["periodic"], tau=wl["tau"], slices_per_dim=8, body_accel=wl["body"])
+    pr = wl["pressure"]
+    o.init_poiseuille(pr["dim"], pr["rho_hi"], pr["rho_lo"], pr["u_max"])
+    o.set_periodic_pressure(pr["dim"], pr["rho_hi"], pr["rho_lo"])
+
+    def step(nthreads):
+        nonlocal x
+        o.set_threads(nthreads)
         o.set_spheres(x, v, w, r, ids)
-        o.collide_stream(1, threads)
+        o.collide_stream(1, nthreads)
         o.sum_force_torque(x)
+        x = L.advance_positions(x, v, n, wl["periodic"])
+
+    step(threads)   # first step: builds the records
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        step(threads)
     dt = time.perf_counter() - t0
-    # one more step on a single core, for the per-core figure
     t1 = time.perf_counter()
-    x = L.advance_positions(x, v, n, wl["periodic"])
-    o.set_spheres(x, v, w, r, ids)
-    o.collide_stream(1, 1)
-    o.sum_force_torque(x)
+    step(1)
     dt1 = time.perf_counter() - t1
     o.close()
-    return {"value": nc * nsteps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
-            "single_core_value": round(nc / dt1 / 1e6, 2),
-            "sample": f"{nsteps} full coupled FP64 steps (voxelize + collide/stream + force sum) of the "
-                      f"same {n[0]}x{n[1]}x{n[2]} workload by the CPU restatement of the reference "
-                      f"algorithm; collide/stream on {threads} OpenMP threads, voxelization and force "
-                      f"sum single-threaded as in the reference; {dt:.1f} s (single_core_value: one "
-                      f"more step on one core, {dt1:.1f} s)"}
-
-
-def stream_copy_gbs(nbytes=2 << 30, reps=10):
-    """Device-to-device hipMemcpy of nbytes, timed with HIP events: a copy
-    ceiling (read + write bytes per second) for context."""
+    cpu_model = None
     try:
-        hip = ctypes.CDLL("libamdhip64.so")
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
     except OSError:
-        return None
-    vp, sz = ctypes.c_void_p, ctypes.c_size_t
-    hip.hipMalloc.argtypes = [ctypes.POINTER(vp), sz]
-    hip.hipFree.argtypes = [vp]
-    hip.hipMemset.argtypes = [vp, ctypes.c_int, sz]
-    hip.hipMemcpyAsync.argtypes = [vp, vp, sz, ctypes.c_int, vp]
-    hip.hipEventCreate.argtypes = [ctypes.POINTER(vp)]
-    hip.hipEventRecord.argtypes = [vp, vp]
-    hip.hipEventSynchronize.argtypes = [vp]
-    hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), vp, vp]
-    hip.hipEventDestroy.argtypes = [vp]
-    a, b, e0, e1 = vp(), vp(), vp(), vp()
-    if hip.hipMalloc(ctypes.byref(a), nbytes) or hip.hipMalloc(ctypes.byref(b), nbytes):
-        return None
-    try:
-        hip.hipMemset(a, 0, nbytes)
-        hip.hipEventCreate(ctypes.byref(e0))
-        hip.hipEventCreate(ctypes.byref(e1))
-        for _ in range(2):
-            hip.hipMemcpyAsync(b, a, nbytes, 3, None)
-        hip.hipEventRecord(e0, None)
-        for _ in range(reps):
-            hip.hipMemcpyAsync(b, a, nbytes, 3, None)
-        hip.hipEventRecord(e1, None)
-        hip.hipEventSynchronize(e1)
-        ms = ctypes.c_float()
-        hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1)
-        hip.hipEventDestroy(e0)
-        hip.hipEventDestroy(e1)
-        return round(2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9, 1)
-    finally:
-        hip.hipFree(a)
-        hip.hipFree(b)
+        pass
+    return {"value": round(nc * nsteps / dt / 1e6, 2), "unit": "MLUPS", "cores": threads, "kind": "port",
+            "single_core_value": round(nc / dt1 / 1e6, 2), "cpu_model": cpu_model,
+            "sample": f"{nsteps} full coupled FP64 steps (voxelize + collide/stream + force sum, after one "
+                      f"untimed step) of {wl['desc']}, by the CPU restatement of the reference algorithm on "
+                      f"{threads} threads (x-slabs, as the reference's MPI ranks), {dt:.1f} s; "
+                      f"single_core_value: one more step on one core, {dt1:.1f} s. The slab has the "
+                      f"channel's per-cell work; MLUPS scale by cell count"}
 
 
-TRAFFIC_JSON = os.path.join(ROOT, "profiles", "collide_stream_traffic.json")
+# ---------------------------------------------------------------- profiles
+
 PROBE_CMD = ("rocprofv3 --pmc {ctr} --output-format csv -d DIR/{tag} -o run -- "
-             "python3 bench.py --kernel-probe")
-TRACE_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace -o run -- "
-             "python3 bench.py --steps 10 --warmup 2 --repeats 1 --no-extra --no-adjoint --cpu-steps 1")
-TRACE512_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace512 -o run -- "
-                "python3 bench.py --workload channel512 --steps 4 --warmup 1 --repeats 1 --no-adjoint --cpu-steps 0")
-TRACEADJ_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace_adj -o run -- "
-                "python3 bench.py --steps 2 --warmup 1 --repeats 1 --no-extra --cpu-steps 0")
+             "python3 bench.py --kernel-probe --workload {wl}")
+TRACE_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace_{wl} -o run -- "
+             "python3 bench.py --workload {wl} --only-headline --steps 10 --warmup 2 --repeats 1")
+ISSUE_CTRS = "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE"
+ISSUE_CMD = ("rocprofv3 --pmc " + ISSUE_CTRS + " --output-format csv -d DIR/issue_{wl} -o run -- "
+             "python3 bench.py --kernel-probe --workload {wl}")
 
 
 def _short(name):
@@ -415,95 +541,69 @@ def _short(name):
         if len(args) > 2 and args[2] == "false":
             extra.append("z walls")
         return f"k_collide_stream<{tag}>" + (f" ({', '.join(extra)})" if extra else "")
-    if base.startswith("k_pop_convert<0"):
-        return "k_pop_convert<FP64> (copy)"
     return base[:48]
 
 
-ISSUE_CTRS = "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_SALU SQ_INSTS_SMEM SQ_INSTS_LDS SQ_WAVE_CYCLES SQ_BUSY_CYCLES GRBM_GUI_ACTIVE"
-ISSUE_CMD = ("rocprofv3 --pmc " + ISSUE_CTRS + " --output-format csv -d DIR/{tag} -o run -- "
-             "python3 bench.py {args}--kernel-probe")
+def _kernel_key(sk):
+    """trace/counter name -> the bench's kernel key (drops the z-walls note)."""
+    return sk.split(" (")[0]
 
 
-def summarize_issue(d):
-    """Instruction counters of the step's kernels -> profiles/kernel_issue_counters.md."""
+def summarize_issue(d, wl):
     import csv
-    out = ["# Instructions per wavefront and issue share, the coupled step's kernels\n",
-           "Counters: `" + ISSUE_CMD.format(tag="issue", args="") + "` (256^3 dense suspension) and `"
-           + ISSUE_CMD.format(tag="issue512", args="--workload channel512 ") + "` (512x512x768 channel);",
-           "one MI355X, ROCm 7.2.  Counter values are summed over the device.  The issue share is",
-           "instructions x 4 cycles (a wave64 instruction on a 16-lane SIMD) over 1024 SIMDs x the",
-           "kernel's cycles (GRBM_GUI_ACTIVE / 8, one count per XCD); VALU and SALU issue from",
-           "different wavefronts in the same cycle.\n"]
-    found = False
-    for tag, title in (("issue", "256^3 dense suspension"), ("issue512", "512x512x768 channel")):
-        path = os.path.join(d, tag, "run_counter_collection.csv")
-        if not os.path.exists(path):
+    path = os.path.join(d, f"issue_{wl}", "run_counter_collection.csv")
+    if not os.path.exists(path):
+        return
+    out = [f"# Instructions per wavefront and issue share, the coupled step's kernels ({wl})\n",
+           "Counters: `" + ISSUE_CMD.format(wl=wl) + "`; one MI355X, ROCm 7.2. Counter values are",
+           "summed over the device. The issue share is instructions x 4 cycles (a wave64 instruction on",
+           "a 16-lane SIMD) over 1024 SIMDs x the kernel's cycles (GRBM_GUI_ACTIVE / 8, one count per XCD);",
+           "VALU and SALU issue from different wavefronts in the same cycle.\n",
+           "| kernel | waves | VALU / wave | SALU / wave | SMEM / wave | LDS / wave | VALU issue share | "
+           "SALU issue share |", "|---|---|---|---|---|---|---|---|"]
+    acc = {}
+    for r in csv.DictReader(open(path)):
+        acc.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    rows = []
+    for k, v in acc.items():
+        sk = _short(k)
+        if not (sk.startswith("k_collide_stream") or sk.startswith("k_fold") or sk.startswith("k_force_bin")):
             continue
-        found = True
-        acc = {}
-        for r in csv.DictReader(open(path)):
-            acc.setdefault(r["Kernel_Name"], {}).setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-        out += [f"## {title}\n", "| kernel | waves | VALU / wave | SALU / wave | SMEM / wave | LDS / wave | "
-                "VALU issue share | SALU issue share |", "|---|---|---|---|---|---|---|---|"]
-        rows = []
-        for k, v in acc.items():
-            sk = _short(k)
-            if not (sk.startswith("k_collide_stream") or sk.startswith("k_fold") or sk.startswith("k_force_bin")):
-                continue
-            m = {c: sum(x) / len(x) for c, x in v.items()}
-            waves = max(m.get("SQ_WAVES", 0.0), 1.0)
-            cyc = m.get("GRBM_GUI_ACTIVE", 0.0) / 8.0
-            share = (lambda n: n * 4.0 / (1024.0 * cyc)) if cyc > 0 else (lambda n: 0.0)
-            rows.append((sk, waves, m.get("SQ_INSTS_VALU", 0) / waves, m.get("SQ_INSTS_SALU", 0) / waves,
-                         m.get("SQ_INSTS_SMEM", 0) / waves, m.get("SQ_INSTS_LDS", 0) / waves,
-                         share(m.get("SQ_INSTS_VALU", 0)), share(m.get("SQ_INSTS_SALU", 0))))
-        for sk, waves, va, sa, sm, ld, vs, ss in sorted(rows):
-            out.append(f"| {sk} | {waves:.0f} | {va:.0f} | {sa:.0f} | {sm:.1f} | {ld:.1f} | {vs:.2f} | {ss:.2f} |")
-        out.append("")
-    if found:
-        with open(os.path.join(ROOT, "profiles", "kernel_issue_counters.md"), "w") as f:
-            f.write("\n".join(out) + "\n")
+        m = {c: sum(x) / len(x) for c, x in v.items()}
+        waves = max(m.get("SQ_WAVES", 0.0), 1.0)
+        cyc = m.get("GRBM_GUI_ACTIVE", 0.0) / 8.0
+        share = (lambda n: n * 4.0 / (1024.0 * cyc)) if cyc > 0 else (lambda n: 0.0)
+        rows.append((sk, waves, m.get("SQ_INSTS_VALU", 0) / waves, m.get("SQ_INSTS_SALU", 0) / waves,
+                     m.get("SQ_INSTS_SMEM", 0) / waves, m.get("SQ_INSTS_LDS", 0) / waves,
+                     share(m.get("SQ_INSTS_VALU", 0)), share(m.get("SQ_INSTS_SALU", 0))))
+    for sk, waves, va, sa, sm, ld, vs, ss in sorted(rows):
+        out.append(f"| {sk} | {waves:.0f} | {va:.0f} | {sa:.0f} | {sm:.1f} | {ld:.1f} | {vs:.2f} | {ss:.2f} |")
+    with open(os.path.join(ROOT, "profiles", f"kernel_issue_counters_{wl}.md"), "w") as f:
+        f.write("\n".join(out) + "\n")
 
 
-def summarize_profiles(d):
-    """Kernel-trace stats and FETCH/WRITE counters -> profiles/*.md + traffic json."""
+def summarize_profiles(d, wl):
+    """Kernel-trace stats and FETCH/WRITE counters of one workload ->
+    profiles/*.md and the traffic entry of profiles/collide_stream_traffic.json."""
     import csv
     import glob
     import sqlite3
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
-    for sub, fname, title, cmd, what in (
-            ("trace", "kernels_dense256_trace.md",
-             "dense suspension 256^3 (2000 spheres r~U[4,8], tau 0.7, 8^3 samples)", TRACE_CMD,
-             "The run does 1 + 2 + 10 coupled steps per precision (FP64, FP32-dev, FP16-store)."),
-            ("trace512", "kernels_channel512_trace.md",
-             "channel 512x512x768 (100000 spheres r~U[3,6], walls x/y, tau 0.55, 8^3 samples)", TRACE512_CMD,
-             "The run does 1 + 1 + 4 coupled steps per precision (FP64, FP32-dev, FP16-store)."),
-            ("trace_adj", "kernels_adjoint128_trace.md",
-             "differentiable channel 128x128x256 (200 spheres r=5, walls x/y, body force along z, "
-             "analytic overlap)", TRACEADJ_CMD,
-             "The run is the adjoint line of the default bench: per precision (FP64, FP32-dev) a 4-step "
-             "warm-up window, then a 500-step window forward (checkpoint every 50) and its backward "
-             "pass, then 502 plain coupled steps. Three coupled steps per precision of the 256^3 "
-             "suspension come first; their kernels are the k_fold and k_collide_stream rows without "
-             "body force.")):
-        dbs = glob.glob(os.path.join(d, sub, "*results.db"))
-        if not dbs:
-            continue
+    dbs = glob.glob(os.path.join(d, f"trace_{wl}", "*results.db"))
+    if dbs:
         con = sqlite3.connect(dbs[0])
         rows = list(con.execute("select name,total_calls,total_duration,average,percentage from top_kernels"))
-        with open(os.path.join(ROOT, "profiles", fname), "w") as f:
-            f.write(f"# Kernel durations, {title}\n\n")
-            f.write("Command: `" + cmd + "` (rocprofv3 kernel trace, ROCm 7.2, one MI355X).\n")
-            f.write(what + "\n\n")
+        with open(os.path.join(ROOT, "profiles", f"kernels_{wl}_trace.md"), "w") as f:
+            f.write(f"# Kernel durations, {WORKLOADS[wl]['desc']}\n\n")
+            f.write("Command: `" + TRACE_CMD.format(wl=wl) + "` (rocprofv3 kernel trace, ROCm 7.2, one MI355X).\n")
+            f.write("The run does 2 + 10 coupled steps per precision (FP64, FP32-dev, FP16-store), then "
+                    "the copy ceiling (k_copy16: 1 + 10 copies of the FP64 population store).\n\n")
             f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
-            for name, n, tot, avg, pct in rows[:24]:
+            for name, n, tot, avg, pct in rows[:28]:
                 f.write(f"| {_short(name)} | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |\n")
-            f.write("\n`__amd_rocclr_copyBuffer` is the run's device-to-device copy ceiling "
-                    "(hipMemcpy of 2 GiB, the `device_copy_gbs` figure), not part of a step.\n")
-    summarize_issue(d)
+    summarize_issue(d, wl)
     res = {}
-    for tag, ctr in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
+    for tag, ctr in ((f"fetch_{wl}", "FETCH_SIZE"), (f"write_{wl}", "WRITE_SIZE")):
         path = os.path.join(d, tag, "run_counter_collection.csv")
         if not os.path.exists(path):
             continue
@@ -516,51 +616,72 @@ def summarize_profiles(d):
             res.setdefault(k, {})[ctr] = sum(v) / len(v)
     if not res:
         return
-    copy = [k for k in res if "k_pop_convert<0>" in k]
-    nc = 256 ** 3
-    copy_bytes = 19 * nc * 8
+    wlx = WORKLOADS[wl]
+    nc = int(np.prod(wlx["n"]))
+    # the calibration copy: k_copy16 over the FP64 population store (row-blocked, padded)
+    zc_bytes = 8 * 19 * nc   # lower bound; the exact store size is printed by the probe
+    probe_log = os.path.join(d, f"probe_{wl}.txt")
+    try:
+        zc_bytes = int(open(probe_log).read().split()[0])
+    except (OSError, ValueError, IndexError):
+        pass
+    copy = [k for k in res if "k_copy16" in k]
     fetch_f = write_f = None
-    if copy:
-        fetch_f = copy_bytes / (res[copy[0]]["FETCH_SIZE"] * 1024.0)
-        write_f = copy_bytes / (res[copy[0]]["WRITE_SIZE"] * 1024.0)
-    lines = ["# HBM traffic per dispatch, dense suspension 256^3\n",
-             "Counters: `" + PROBE_CMD.format(ctr="FETCH_SIZE", tag="fetch") + "`, then the same with WRITE_SIZE.",
-             f"Calibration: the FP64 population copy k_pop_convert<FP64> reads and writes {copy_bytes} B;",
-             f"FETCH_SIZE scale = {fetch_f:.4f} B per counted KiB/1024, WRITE_SIZE scale = {write_f:.4f}.\n" if fetch_f else "",
-             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read GB (scaled) | written GB (scaled) | algorithmic GB |",
-             "|---|---|---|---|---|---|"]
+    if copy and res[copy[0]].get("FETCH_SIZE") and res[copy[0]].get("WRITE_SIZE"):
+        fetch_f = zc_bytes / (res[copy[0]]["FETCH_SIZE"] * 1024.0)
+        write_f = zc_bytes / (res[copy[0]]["WRITE_SIZE"] * 1024.0)
+    lines = [f"# HBM traffic per dispatch, {wlx['desc']}\n",
+             "Counters: `" + PROBE_CMD.format(ctr="FETCH_SIZE", tag=f"fetch_{wl}", wl=wl) +
+             "`, then the same with WRITE_SIZE (tag write_" + wl + ").",
+             f"Calibration: the copy k_copy16 reads and writes the FP64 population store, {zc_bytes} B each way;",
+             (f"FETCH_SIZE scale = {fetch_f:.4f} B per counted KiB/1024, WRITE_SIZE scale = {write_f:.4f}.\n"
+              if fetch_f else "no calibration copy found: unscaled.\n"),
+             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | read GB (scaled) | written GB (scaled) | algorithmic GB | measured / algorithmic |",
+             "|---|---|---|---|---|---|---|"]
     traffic = {}
-    algo = {"k_collide_stream<FP64>": 304 * nc, "k_collide_stream<FP32_DEV>": 152 * nc,
-            "k_collide_stream<FP16_STORE>": 76 * nc, "k_pop_convert<FP64> (copy)": 2 * copy_bytes}
+    algo = {KERNEL[p]: 2 * 19 * BYTES_PER_POP[p] * nc for p in PRECISIONS}
+    algo["k_copy16"] = 2 * zc_bytes
     for k, v in sorted(res.items(), key=lambda kv: -kv[1].get("FETCH_SIZE", 0)):
         fk, wk = v.get("FETCH_SIZE", 0.0), v.get("WRITE_SIZE", 0.0)
         rd = fk * 1024 * (fetch_f or 1) / 1e9
         wr = wk * 1024 * (write_f or 1) / 1e9
         sk = _short(k)
-        al = algo.get(sk)
-        lines.append(f"| {sk} | {fk:.0f} | {wk:.0f} | {rd:.3f} | {wr:.3f} | {al / 1e9:.3f} |" if al else
-                     f"| {sk} | {fk:.0f} | {wk:.0f} | {rd:.3f} | {wr:.3f} | |")
-        if sk.startswith("k_collide_stream"):
-            traffic[sk] = (rd + wr) * 1e9
-    with open(os.path.join(ROOT, "profiles", "collide_stream_dense256_hbm_traffic.md"), "w") as f:
-        f.write("\n".join(lines[:14] + lines[14:40]) + "\n")
+        al = algo.get(_kernel_key(sk))
+        if al:
+            lines.append(f"| {sk} | {fk:.0f} | {wk:.0f} | {rd:.3f} | {wr:.3f} | {al / 1e9:.3f} | {(rd + wr) * 1e9 / al:.3f} |")
+        else:
+            lines.append(f"| {sk} | {fk:.0f} | {wk:.0f} | {rd:.3f} | {wr:.3f} | | |")
+        if sk.startswith("k_collide_stream") or sk.startswith("k_copy16"):
+            traffic[_kernel_key(sk)] = (rd + wr) * 1e9
+    with open(os.path.join(ROOT, "profiles", f"collide_stream_{wl}_hbm_traffic.md"), "w") as f:
+        f.write("\n".join(lines[:40]) + "\n")
+    try:
+        t = json.load(open(TRAFFIC_JSON))
+    except (OSError, ValueError):
+        t = {}
+    t.setdefault("workloads", {})[wl] = {"bytes_per_launch": traffic, "fetch_scale": fetch_f,
+                                         "write_scale": write_f, "source": f"collide_stream_{wl}_hbm_traffic.md"}
+    t.pop("workload", None)
+    t.pop("bytes_per_launch", None)
     with open(TRAFFIC_JSON, "w") as f:
-        json.dump({"workload": "dense256", "bytes_per_launch": traffic,
-                   "fetch_scale": fetch_f, "write_scale": write_f}, f, indent=1)
+        json.dump(t, f, indent=1)
 
 
-def kernel_probe(wl, sph):
-    """Counter-calibration copy of known size, then three coupled steps per
-    precision.  The copy is the FP64 population upload: k_pop_convert<FP64>
-    reads 19*Nc doubles and writes 19*Nc doubles."""
-    n = wl["n"]
-    nc = int(np.prod(n))
-    lat = L.Lattice(*n, periodic=wl["periodic"], tau=wl["tau"], precision="fp64", slices_per_dim=8)
-    lat.set_populations(np.zeros((19, nc)))
-    lat.synchronize()
+def kernel_probe(key, out_dir=None):
+    """Short run for hardware-counter profiles: the copy k_copy16 of the FP64
+    population store (known size, for calibration), then 3 coupled steps per
+    precision."""
+    wl = WORKLOADS[key]
+    sph = make_spheres(wl)
+    lat = new_lattice(wl, "fp64")
+    ms, nbytes = lat.copy_bandwidth(1)
     lat.close()
-    log(f"probe copy: {2 * 19 * nc * 8} bytes")
-    for prec in ("fp64", "fp32_dev", "fp16_store"):
+    log(f"probe copy: {int(nbytes // 2)} bytes read and written per copy, {ms:.3f} ms")
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        with open(os.path.join(out_dir, f"probe_{key}.txt"), "w") as f:
+            f.write(f"{int(nbytes // 2)}\n")
+    for prec in PRECISIONS:
         run_gpu(wl, prec, 3, 1, sph)
         log(f"probe {prec} done")
 
@@ -568,120 +689,73 @@ def kernel_probe(wl, sph):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--repeats", type=int, default=5,
-                    help="repetitions of the FP64 line (each: warmup + steps); value is their median")
-    ap.add_argument("--workload", default="dense256", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-extra", action="store_true", help="skip the 512^3-class FP32 line")
-    ap.add_argument("--cpu-steps", type=int, default=4)
-    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-window line")
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="fresh lattices per precision of the headline workload; value is the median")
+    ap.add_argument("--workload", default="channel512", choices=sorted(k for k in WORKLOADS if k != "channel512_slab"))
+    ap.add_argument("--only-headline", action="store_true",
+                    help="the headline workload only (no other workloads, adjoint, exchange loop or CPU baseline)")
+    ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-window lines")
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--summarize-profiles", metavar="DIR",
-                    help="turn rocprofv3 outputs under DIR (kernel trace db, fetch/ and write/ "
-                         "counter csv) into profiles/*.md and profiles/collide_stream_traffic.json")
+                    help="turn rocprofv3 outputs under DIR (trace_<wl>/, fetch_<wl>/, write_<wl>/, "
+                         "issue_<wl>/) into profiles/*.md and profiles/collide_stream_traffic.json")
     ap.add_argument("--kernel-probe", action="store_true",
-                    help="short run for hardware-counter profiles: the FP64 population copy "
-                         "(known size, for calibration) then 3 coupled steps per precision")
+                    help="short run for hardware-counter profiles: the calibration copy of known size, "
+                         "then 3 coupled steps per precision")
+    ap.add_argument("--probe-out", default=os.path.join(ROOT, "gpurun_out"),
+                    help="where --kernel-probe writes the calibration copy's size")
     args = ap.parse_args()
     if args.summarize_profiles:
-        summarize_profiles(args.summarize_profiles)
+        summarize_profiles(args.summarize_profiles, args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU slab decomposition is a design, not a measured path)", file=sys.stderr)
         sys.exit(2)
-
-    wl = WORKLOADS[args.workload]
-    n = wl["n"]
-    nc = int(np.prod(n))
-    log(f"workload {args.workload}: generating {wl['count']} spheres")
-    sph = make_spheres(wl)
     if args.kernel_probe:
-        kernel_probe(wl, sph)
+        kernel_probe(args.workload, args.probe_out)
         return
-    res = {}
-    runs64 = []
-    for k in range(max(1, args.repeats)):
-        log(f"fp64 run {k + 1}/{max(1, args.repeats)}: {args.warmup} warmup + {args.steps} timed steps")
-        runs64.append(run_gpu(wl, "fp64", args.steps, args.warmup, sph, want_field=(k == 0)))
-        log(f"fp64: {runs64[-1]['mlups']:.0f} MLUPS full step, kernel {runs64[-1]['kernel_gbs']:.0f} GB/s")
-    # the median repetition; fields and forces (identical in every repetition) from the first
-    med = sorted(runs64, key=lambda rr: rr["mlups"])[len(runs64) // 2]
-    res["fp64"] = dict(med, u=runs64[0].get("u"), F=runs64[0]["F"], T=runs64[0]["T"])
-    for prec in ("fp32_dev", "fp16_store"):
-        log(f"{prec}: {args.warmup} warmup + {args.steps} timed steps")
-        res[prec] = run_gpu(wl, prec, args.steps, args.warmup, sph, want_field=True)
-        log(f"{prec}: {res[prec]['mlups']:.0f} MLUPS full step, kernel {res[prec]['kernel_gbs']:.0f} GB/s")
+
+    key = args.workload
+    head, res, sph = run_workload(key, args.steps, args.warmup, args.repeats, copy_reps=10)
     f64 = res["fp64"]
-    copy_gbs = stream_copy_gbs()
-    log(f"device copy: {copy_gbs} GB/s")
-
-    def summary(rr, ref=f64):
-        return {"mlups": round(rr["mlups"], 1), "ms_per_step": round(rr["ms_per_step"], 4),
-                "kernel_mlups": round(rr["kernel_mlups"], 1), "kernel_ms": round(rr["kernel_ms"], 4),
-                "kernel_gbs": round(rr["kernel_gbs"], 1),
-                "kernel_roofline_frac": round(rr["kernel_gbs"] / HBM_PEAK_GBS, 4),
-                "step_roofline_frac": round(rr["mlups"] * 1e6 * rr["bytes_per_lu"] / 1e9 / HBM_PEAK_GBS, 4),
-                "voxelize_ms": round(rr["voxelize_ms"], 4), "force_ms": round(rr["force_ms"], 4),
-                "drag_rel_err_vs_fp64": rel(rr["F"], ref["F"]),
-                "torque_rel_err_vs_fp64": rel(rr["T"], ref["T"]),
-                "velocity_err_vs_fp64": field_err(rr.get("u"), ref.get("u")),
-                "max_u": rr["max_u"], "health_rc": rr["health_rc"]}
-
+    extra_on = not (args.only_headline or args.no_extra)
+    adj_on = not (args.only_headline or args.no_adjoint)
     extra = {}
-    if not args.no_extra and args.workload == "dense256":
-        wl5 = WORKLOADS["channel512"]
-        log("channel512: generating 100000 spheres")
-        sph5 = make_spheres(wl5)
-        steps5 = max(10, min(200, 2 * args.steps))
-        r32 = run_gpu(wl5, "fp32_dev", steps5, max(2, args.warmup // 2), sph5, want_field=True)
-        log(f"channel512 fp32_dev: {r32['mlups']:.0f} MLUPS, kernel {r32['kernel_gbs']:.0f} GB/s")
-        r64 = run_gpu(wl5, "fp64", steps5, max(2, args.warmup // 2), sph5, want_field=True)
-        log(f"channel512 fp64: {r64['mlups']:.0f} MLUPS")
-        r16 = run_gpu(wl5, "fp16_store", steps5, max(2, args.warmup // 2), sph5, want_field=True)
-        log(f"channel512 fp16_store: {r16['mlups']:.0f} MLUPS, kernel {r16['kernel_gbs']:.0f} GB/s")
-        e32 = summary(r32, r64)
-        e16 = summary(r16, r64)
-        del r16
-        extra["channel512"] = {"workload": wl5["desc"], "steps": steps5,
-                               "fp32_dev": e32, "fp16_store": e16,
-                               "fp64": {"mlups": round(r64["mlups"], 1),
-                                        "kernel_gbs": round(r64["kernel_gbs"], 1),
-                                        "max_u": r64["max_u"], "health_rc": r64["health_rc"]},
-                               "fp32_over_fp64": round(r32["mlups"] / r64["mlups"], 3)}
-        del r32, r64
-        if not args.no_adjoint:
-            log("channel512: adjoint of the last 50 of 200 steps (FP32, state budget 10)")
-            try:
-                extra["channel512"]["adjoint"] = run_adjoint_c5(sph5)
-                a5 = extra["channel512"]["adjoint"]
-                log(f"channel512 adjoint: forward {a5['forward_s']:.3f} s, backward {a5['backward_s']:.3f} s, "
-                    f"device memory {a5['device_memory_used_after_backward_gb']} GB")
-            except L.PSMError as e:
-                extra["channel512"]["adjoint"] = {"error": str(e)}
-                log(f"channel512 adjoint failed: {e}")
-
-    if not args.no_extra and args.workload == "dense256":
-        wl3 = WORKLOADS["poly384"]
-        log("poly384: generating 20000 spheres")
-        sph3 = make_spheres(wl3)
-        # 100 steps: with prescribed random kinematics this workload leaves
-        # the stable range (max |u| > 0.3) after ~200 steps in FP64 as well
-        steps3 = 100
-        r3 = {}
-        for prec in ("fp64", "fp32_dev", "fp16_store"):
-            r3[prec] = run_gpu(wl3, prec, steps3, 0, sph3, want_field=True)
-            log(f"poly384 {prec}: {r3[prec]['mlups']:.0f} MLUPS, kernel {r3[prec]['kernel_gbs']:.0f} GB/s")
-        extra["poly384"] = {"workload": wl3["desc"], "steps": steps3,
-                            "fp64": {"mlups": round(r3["fp64"]["mlups"], 1),
-                                     "kernel_gbs": round(r3["fp64"]["kernel_gbs"], 1),
-                                     "max_u": r3["fp64"]["max_u"], "health_rc": r3["fp64"]["health_rc"]},
-                            "fp32_dev": summary(r3["fp32_dev"], r3["fp64"]),
-                            "fp16_store": summary(r3["fp16_store"], r3["fp64"]),
-                            "fp32_over_fp64": round(r3["fp32_dev"]["mlups"] / r3["fp64"]["mlups"], 3)}
-        del r3
-
-    if not args.no_adjoint:
+    if adj_on and key == "channel512":
+        log("channel512: adjoint of the last 50 of 200 steps (FP32, state budget 10)")
+        try:
+            head["adjoint"] = run_adjoint_c5(sph)
+            a5 = head["adjoint"]
+            log(f"channel512 adjoint: forward {a5['forward_s']:.3f} s, backward {a5['backward_s']:.3f} s, "
+                f"device memory {a5['device_memory_used_after_backward_gb']} GB")
+        except L.PSMError as e:
+            head["adjoint"] = {"error": str(e)}
+            log(f"channel512 adjoint failed: {e}")
+    if not args.only_headline:
+        ex = {}
+        for prec in ("fp64", "fp32_dev"):
+            log(f"{key} exchange loop {prec}")
+            ex[prec] = run_exchange(key, prec, max(5, args.steps // 5), 2, sph)
+            ex[prec]["resident_step_mlups"] = head[prec]["mlups"]
+            log(f"  {ex[prec]['mlups']:.0f} MLUPS")
+        head["exchange_loop"] = ex
+    del res
+    if extra_on:
+        for k2 in ("cube512", "dense256", "poly384"):
+            if k2 == key:
+                continue
+            steps2 = args.steps
+            w2 = WORKLOADS[k2]
+            if k2 == "poly384":
+                # its configuration's 1000 steps: 2 warm-up, the rest timed
+                extra[k2], _, _ = run_workload(k2, w2["full_steps"] - 2, 2, 1)
+            else:
+                extra[k2], _, _ = run_workload(k2, steps2, args.warmup, 1)
+    if adj_on:
         adj = {}
         for prec in ("fp64", "fp32_dev"):
             log(f"adjoint window {prec}: {ADJOINT_CASE['steps']} steps forward + backward")
@@ -692,22 +766,25 @@ def main():
                             "fp32_grad_rel_err_vs_fp64": {k: rel(g32[k], g64[k]) for k in ("v", "w", "x", "sf")}}
 
     cpu = None
-    if args.cpu_steps > 0:
-        log(f"cpu baseline: {args.cpu_steps} oracle steps")
-        cpu = cpu_baseline(wl, sph, args.cpu_steps)
-        log(f"cpu baseline: {cpu['value']:.1f} MLUPS")
+    if args.cpu_steps > 0 and not args.only_headline:
+        log(f"cpu baseline: {args.cpu_steps} oracle steps on the channel slab")
+        cpu = cpu_baseline(args.cpu_steps)
+        log(f"cpu baseline: {cpu['value']:.1f} MLUPS on {cpu['cores']} threads")
 
+    wl = WORKLOADS[key]
+    nc = int(np.prod(wl["n"]))
+    h64 = head["fp64"]
     out = {
         "metric": METRIC,
-        "value": round(f64["mlups"], 1),
+        "value": h64["mlups"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
         "repeats": max(1, args.repeats),
-        "value_is": "median over the repetitions of the FP64 line (each times exactly `steps` steps)",
-        "fp64_runs_mlups": [round(rr["mlups"], 1) for rr in runs64],
-        "ms_per_step": round(f64["ms_per_step"], 4),
+        "value_is": "median over fresh lattices of the FP64 full-step rate (each: warmup, then exactly "
+                    "`steps` timed steps)",
+        "ms_per_step": h64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -716,22 +793,21 @@ def main():
         "config": {"workload": wl["desc"], "cells": nc, "spheres": wl["count"], "sub_cell_samples": 8,
                    "kinematics": "prescribed x += v", "precision": "fp64",
                    "step": "voxelize + fused collide/stream + force/torque sum"},
-        "roofline": {"bound": "hbm", "achieved": round(f64["kernel_gbs"], 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(f64["kernel_gbs"] / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(args.workload, "k_collide_stream<FP64>"),
-                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, scaled by a known-size copy; "
-                                       "profiles/collide_stream_dense256_hbm_traffic.md",
-                     "kernel": "k_collide_stream<FP64>",
-                     "algorithmic_bytes_per_launch": 304 * nc,
-                     "device_copy_gbs": copy_gbs,
-                     "device_copy": "hipMemcpy device to device of 2 GiB, read + write bytes per second"},
+        "roofline": dict(h64["roofline"], device_copy_gbs=h64.get("device_copy_gbs"),
+                         device_copy="k_copy16 over the FP64 population store of the same lattice, "
+                                     "read + write bytes per second (HIP events)",
+                         traffic_source=f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by the k_copy16 "
+                                        f"copy of known size; profiles/collide_stream_{key}_hbm_traffic.md"),
         "cpu_baseline": cpu,
-        "fp64": summary(f64),
-        "fp32_dev": summary(res["fp32_dev"]),
-        "fp16_store": summary(res["fp16_store"]),
-        "fp32_over_fp64_mlups": round(res["fp32_dev"]["mlups"] / f64["mlups"], 3),
-        "records": f64["records"], "images": f64["images"], "orphans": f64["orphans"],
+        "fp64": h64,
+        "fp32_dev": head["fp32_dev"],
+        "fp16_store": head["fp16_store"],
+        "fp32_over_fp64_mlups": head["fp32_over_fp64_mlups"],
+        "fp16_over_fp64_mlups": head["fp16_over_fp64_mlups"],
     }
+    for k in ("adjoint", "exchange_loop"):
+        if k in head:
+            out[f"{key}_{k}"] = head[k]
     out.update(extra)
     print(json.dumps(out), flush=True)
 

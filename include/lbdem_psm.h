@@ -53,6 +53,8 @@ extern "C" {
 
 /* lbdem_psm_step flags */
 #define LBDEM_PSM_STEP_ADVANCE 1 /* prescribed kinematics: x += v (periodic wrap) first */
+#define LBDEM_PSM_STEP_FROZEN 2  /* no voxelization: the records of the last one stay (particles at rest
+                                  * in the lattice frame); excludes STEP_ADVANCE */
 
 typedef struct lbdem_psm lbdem_psm;
 
@@ -116,7 +118,8 @@ int lbdem_psm_sum_force_torque(lbdem_psm* h, int64_t n, const double* x3, double
 /* One full coupled step on resident data, nsteps times: [x += v] ->
  * voxelize -> collide+stream -> force/torque sum (kept on the device; read
  * with lbdem_psm_get_particle_forces).  The loop of
- * sedimentingSphere.cpp:212-225 without the DEM substeps. */
+ * sedimentingSphere.cpp:212-225 without the DEM substeps.  With
+ * LBDEM_PSM_STEP_FROZEN the voxelization is skipped. */
 int lbdem_psm_step(lbdem_psm* h, int32_t nsteps, int32_t flags);
 int lbdem_psm_get_particle_forces(lbdem_psm* h, int64_t n, double* F3, double* T3);
 int lbdem_psm_get_particles(lbdem_psm* h, int64_t n, double* x3);
@@ -189,6 +192,15 @@ int lbdem_psm_synchronize(lbdem_psm* h);
  * 3 advance.  read: total ms and launch count per class; reset clears. */
 int lbdem_psm_profile(lbdem_psm* h, int32_t enable);
 int lbdem_psm_profile_read(lbdem_psm* h, double* ms4, int64_t* count4, int32_t reset);
+
+/* Measurement utility (no counterpart in the reference): the device copy
+ * ceiling of the handle's population store.  Copies the current population
+ * buffer into the second one (which holds no state between steps) reps
+ * times with a plain 16-byte-per-lane copy kernel on the handle's stream,
+ * after one untimed copy; ms_per_copy is the HIP-event time per copy and
+ * bytes_per_copy the bytes read plus written by one copy.  Populations are
+ * unchanged. */
+int lbdem_psm_copy_bandwidth(lbdem_psm* h, int32_t reps, double* ms_per_copy, double* bytes_per_copy);
 
 /* Zhang-Kwok periodic pressure difference across the periodic axis
  * `dimension` (ZhangPeriodicPressureFunctional3D,

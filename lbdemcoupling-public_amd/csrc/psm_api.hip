@@ -514,7 +514,10 @@ int lbdem_psm_sum_force_torque(lbdem_psm* h, int64_t n, const double* x3, double
 int lbdem_psm_step(lbdem_psm* h, int32_t nsteps, int32_t flags) {
     if (!h || nsteps < 0) return LBDEM_PSM_ERR_INVALID_ARG;
     if (!h->have_particles) return set_err(h, LBDEM_PSM_ERR_STATE, "step before set_spheres");
-    h->geom_gen += 1;
+    const bool frozen = (flags & LBDEM_PSM_STEP_FROZEN) != 0;
+    if (frozen && (flags & LBDEM_PSM_STEP_ADVANCE))
+        return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "STEP_FROZEN excludes STEP_ADVANCE");
+    if (!frozen) h->geom_gen += 1;
     PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
     for (int s = 0; s < nsteps; ++s) {
         int rc;
@@ -522,8 +525,10 @@ int lbdem_psm_step(lbdem_psm* h, int32_t nsteps, int32_t flags) {
             rc = launch_advance(h);
             if (rc) return rc;
         }
-        rc = run_voxelize(h);
-        if (rc) return rc;
+        if (!frozen) {
+            rc = run_voxelize(h);
+            if (rc) return rc;
+        }
         rc = launch_collide_stream(h);
         if (rc) return rc;
         rc = run_force_sum(h, h->px);

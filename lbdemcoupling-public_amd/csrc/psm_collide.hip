@@ -341,6 +341,19 @@ int launch_macroscopic(lbdem_psm* h, double* d_rho, double* d_u) {
     return LBDEM_PSM_OK;
 }
 
+/* Plain device copy of n16 16-byte words, one word per lane per pass: the
+ * bandwidth ceiling the collide is measured against (every byte of a
+ * population store read once and written once, like the collide's
+ * algorithmic traffic), and a transfer of known size for counter
+ * calibration. */
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_copy16(const u32x4* __restrict__ src, u32x4* __restrict__ dst, int64_t n16) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += stride)
+        dst[k] = __builtin_nontemporal_load(src + k);
+}
+
 int launch_define_velocity(lbdem_psm* h) {
     if (!h->dv_cell) return LBDEM_PSM_OK;
     dim3 grid(1024), block(256);
@@ -360,3 +373,39 @@ int launch_define_velocity(lbdem_psm* h) {
 }
 
 }  // namespace psm
+
+using namespace psm;
+
+extern "C" {
+
+int lbdem_psm_copy_bandwidth(lbdem_psm* h, int32_t reps, double* ms_per_copy, double* bytes_per_copy) {
+    if (!h || reps < 1) return LBDEM_PSM_ERR_INVALID_ARG;
+    /* the current population store into the other one (which holds no state
+     * between steps); the store size is a multiple of 16 bytes */
+    const int64_t n16 = (int64_t)(h->pop_bytes / 16);
+    const u32x4* src = (const u32x4*)h->f[h->cur_f];
+    u32x4* dst = (u32x4*)h->f[1 - h->cur_f];
+    const unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(1, (n16 + 255) / 256), 1 << 20);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    PSM_HIP(h, hipEventCreate(&e0));
+    hipError_t e = hipEventCreate(&e1);
+    if (e != hipSuccess) { (void)hipEventDestroy(e0); return hip_check(h, e, "hipEventCreate"); }
+    hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, h->stream, src, dst, n16);   /* warm-up */
+    e = hipEventRecord(e0, h->stream);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+        hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, h->stream, src, dst, n16);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_check(h, e, "copy_bandwidth");
+    if (ms_per_copy) *ms_per_copy = (double)ms / reps;
+    if (bytes_per_copy) *bytes_per_copy = 2.0 * (double)(n16 * 16);
+    return LBDEM_PSM_OK;
+}
+
+}  // extern "C"

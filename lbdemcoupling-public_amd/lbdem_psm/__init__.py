@@ -49,6 +49,7 @@ VOXEL_MODES = {"subcell": VOXEL_SUBCELL, "overlap": VOXEL_OVERLAP}
 INIT_VEL = 1
 AUTO_IMAGES = 2
 STEP_ADVANCE = 1
+STEP_FROZEN = 2
 
 RADIUS_CONST = 0
 RADIUS_UNIFORM = 1
@@ -153,6 +154,7 @@ _SIGS = {
     "lbdem_psm_checkpoint_read_desc": ([ctypes.c_char_p, _P], ctypes.c_int),
     "lbdem_psm_checkpoint_read_arrays": ([ctypes.c_char_p, _P, _P, _P, _P, _P], ctypes.c_int),
     "lbdem_psm_save_checkpoint": ([_P, ctypes.c_char_p], ctypes.c_int),
+    "lbdem_psm_copy_bandwidth": ([_P, ctypes.c_int32, _P, _P], ctypes.c_int),
     "lbdem_psm_load_checkpoint": ([_P, ctypes.c_char_p], ctypes.c_int),
 }
 
@@ -384,8 +386,11 @@ class Lattice:
         self._c(self.lib.lbdem_psm_sum_force_torque(self.h, n, _ptr(xx), _ptr(F), _ptr(T)))
         return F, T
 
-    def step(self, nsteps=1, advance=True):
-        self._c(self.lib.lbdem_psm_step(self.h, int(nsteps), STEP_ADVANCE if advance else 0))
+    def step(self, nsteps=1, advance=True, frozen=False):
+        """nsteps coupled steps on resident data; frozen: no voxelization
+        (and no advance), the records of the last one stay."""
+        flags = STEP_FROZEN if frozen else (STEP_ADVANCE if advance else 0)
+        self._c(self.lib.lbdem_psm_step(self.h, int(nsteps), flags))
 
     def particle_forces(self):
         n = self.np_
@@ -520,6 +525,14 @@ class Lattice:
 
     def synchronize(self):
         self._c(self.lib.lbdem_psm_synchronize(self.h))
+
+    def copy_bandwidth(self, reps=10):
+        """Device copy ceiling of this lattice's population store: (ms per
+        copy, bytes read + written per copy), HIP events on the handle's
+        stream."""
+        ms, nb = ctypes.c_double(), ctypes.c_double()
+        self._c(self.lib.lbdem_psm_copy_bandwidth(self.h, int(reps), ctypes.byref(ms), ctypes.byref(nb)))
+        return ms.value, nb.value
 
     def profile(self, enable=True):
         self._c(self.lib.lbdem_psm_profile(self.h, int(bool(enable))))

@@ -44,6 +44,7 @@ def load():
         "orc_last_n_voxelized": ([_P], ctypes.c_int64),
         "orc_overlap_fraction": ([ctypes.c_double, ctypes.c_double, _P, _P], ctypes.c_double),
         "orc_set_voxel_mode": ([_P, ctypes.c_int], None),
+        "orc_set_threads": ([_P, ctypes.c_int], None),
         "orc_set_periodic_pressure": ([_P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _P], None),
         "orc_init_poiseuille": ([_P, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P], None),
     }
@@ -145,6 +146,10 @@ class OracleLattice:
         if rc != 0:
             raise ValueError(f"orc_set_spheres failed: {rc}")
         self.np_ = len(r)
+
+    def set_threads(self, nthreads):
+        """x-slab threads of set_spheres and sum_force_torque (1 = sequential)."""
+        self.lib.orc_set_threads(self.h, int(nthreads))
 
     def collide_stream(self, nsteps=1, nthreads=0):
         self.lib.orc_collide_stream(self.h, int(nsteps), int(nthreads))

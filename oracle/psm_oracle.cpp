@@ -178,6 +178,7 @@ struct orc_lattice {
     std::map<int64_t, int64_t> idmap;
     int64_t last_n_vox = 0;
     int voxel_mode = 0;
+    int threads = 1;                /* x-slabs of set_spheres / sum_force_torque */
     /* periodic pressure */
     bool pp = false;
     int pp_dim = 0;
@@ -396,30 +397,51 @@ int orc_set_spheres(orc_lattice* L, int64_t n, const double* x3, const double* v
     L->last_n_vox = 0;
 
     const int N = L->slices;
-    /* src/ibDataExchangeWrappers3D.hh:62-140 */
+    /* src/ibDataExchangeWrappers3D.hh:62-140: the voxelized images in order,
+     * each with its box intersected with the lattice */
+    struct Vox { int64_t q; double x[3]; int64_t lo[3], hi[3]; };
+    std::vector<Vox> vox;
+    vox.reserve(list.size());
     for (const Img& im : list) {
         const int64_t q = im.q;
         bool excl = false;
         for (int t = 0; t < n_exclude; ++t)
             if (type && type[q] == exclude[t]) { excl = true; break; }
         if (excl) continue;
-        double x[3];
-        for (int d = 0; d < 3; ++d) x[d] = x3[3 * q + d] + (double)im.s[d] * (double)L->n[d];
+        Vox vx;
+        vx.q = q;
+        for (int d = 0; d < 3; ++d) vx.x[d] = x3[3 * q + d] + (double)im.s[d] * (double)L->n[d];
+        const double rr = r[q];
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) {   /* getBoundingBox + intersect */
+            vx.lo[d] = std::max<int64_t>(bbox_lo(vx.x[d], rr), 0);
+            vx.hi[d] = std::min<int64_t>(bbox_hi(vx.x[d], rr), L->n[d] - 1);
+            if (vx.lo[d] > vx.hi[d]) ok = false;
+        }
+        if (!ok) continue;
+        vox.push_back(vx);
+    }
+    L->last_n_vox = (int64_t)vox.size();
+    /* With threads > 1 each thread owns an x-slab of the lattice and runs the
+     * whole particle loop on its part of every box, as each MPI rank of the
+     * reference does on its own block; a cell is visited by one thread in
+     * the same particle order, so the records are the same. */
+    const int nthr = std::max(1, std::min(L->threads, L->n[0]));
+#pragma omp parallel for num_threads(nthr) schedule(static, 1)
+    for (int slab = 0; slab < nthr; ++slab) {
+      const int64_t sx0 = (int64_t)L->n[0] * slab / nthr, sx1 = (int64_t)L->n[0] * (slab + 1) / nthr - 1;
+      for (const Vox& vx : vox) {
+        const int64_t q = vx.q;
+        const double* x = vx.x;
         const double* v = &v3[3 * q];
         const double* w = &w3[3 * q];
         const double rr = r[q];
         const int pid = (int)id[q];
-        int64_t lo[3], hi[3];
-        bool ok = true;
-        for (int d = 0; d < 3; ++d) {   /* getBoundingBox + intersect */
-            lo[d] = std::max<int64_t>(bbox_lo(x[d], rr), 0);
-            hi[d] = std::min<int64_t>(bbox_hi(x[d], rr), L->n[d] - 1);
-            if (lo[d] > hi[d]) ok = false;
-        }
-        if (!ok) continue;
-        ++L->last_n_vox;
+        const int64_t* lo = vx.lo;
+        const int64_t* hi = vx.hi;
+        const int64_t xa = std::max(lo[0], sx0), xb = std::min(hi[0], sx1);
         /* SetSingleSphere3D::process, src/ibProcessors3D.hh:52-114 */
-        for (int64_t iX = lo[0]; iX <= hi[0]; ++iX)
+        for (int64_t iX = xa; iX <= xb; ++iX)
             for (int64_t iY = lo[1]; iY <= hi[1]; ++iY)
                 for (int64_t iZ = lo[2]; iZ <= hi[2]; ++iZ) {
                     const int64_t c = (iX * L->n[1] + iY) * L->n[2] + iZ;
@@ -458,9 +480,12 @@ int orc_set_spheres(orc_lattice* L, int64_t n, const double* x3, const double* v
                         equilibria(0.0, 1.0 / (0.0 + 1.0), p.us, uSqr, cell);
                     }
                 }
+      }
     }
     return 0;
 }
+
+void orc_set_threads(orc_lattice* L, int nthreads) { L->threads = nthreads > 0 ? nthreads : 1; }
 
 void orc_collide_stream(orc_lattice* L, int nsteps, int nthreads) {
 #ifdef _OPENMP
@@ -561,6 +586,45 @@ int orc_sum_force_torque(const orc_lattice* L, int64_t n, const double* x3, doub
                          double* T3) {
     for (int64_t k = 0; k < 3 * n; ++k) { F3[k] = 0.0; T3[k] = 0.0; }
     const int64_t nx = L->n[0], ny = L->n[1], nz = L->n[2];
+    const int nthr = std::max(1, std::min<int>(L->threads, (int)nx));
+    if (nthr > 1) {
+        /* one partial sum per x-slab (an MPI rank's SumForceTorque3D over its
+         * block), then the slabs in order: the sum's grouping differs from
+         * the sequential scan, so this form serves timing, not parity */
+        std::vector<std::vector<double>> part(nthr, std::vector<double>(6 * (size_t)n, 0.0));
+#pragma omp parallel for num_threads(nthr) schedule(static, 1)
+        for (int slab = 0; slab < nthr; ++slab) {
+            const int64_t x0 = nx * slab / nthr, x1 = nx * (slab + 1) / nthr;
+            double* Fp = part[slab].data();
+            double* Tp = Fp + 3 * n;
+            for (int64_t iX = x0; iX < x1; ++iX)
+                for (int64_t iY = 0; iY < ny; ++iY)
+                    for (int64_t iZ = 0; iZ < nz; ++iZ) {
+                        const Record& p = L->rec[(iX * ny + iY) * nz + iZ];
+                        if (p.id < 1) continue;
+                        auto it = L->idmap.find(p.id);
+                        if (it == L->idmap.end() || it->second >= n) continue;
+                        const int64_t ind = it->second;
+                        double d[3] = {(double)iX - x3[3 * ind], (double)iY - x3[3 * ind + 1],
+                                       (double)iZ - x3[3 * ind + 2]};
+                        const int64_t nn[3] = {nx, ny, nz};
+                        for (int k = 0; k < 3; ++k) {
+                            if (d[k] > (double)(nn[k] / 2)) d[k] -= (double)nn[k];
+                            else if (d[k] < (double)(-nn[k] / 2)) d[k] += (double)nn[k];
+                        }
+                        Fp[3 * ind] += p.F[0]; Fp[3 * ind + 1] += p.F[1]; Fp[3 * ind + 2] += p.F[2];
+                        Tp[3 * ind] += d[1] * p.F[2] - d[2] * p.F[1];
+                        Tp[3 * ind + 1] += -d[0] * p.F[2] + d[2] * p.F[0];
+                        Tp[3 * ind + 2] += d[0] * p.F[1] - d[1] * p.F[0];
+                    }
+        }
+        for (int slab = 0; slab < nthr; ++slab)
+            for (int64_t k = 0; k < 3 * n; ++k) {
+                F3[k] += part[slab][k];
+                T3[k] += part[slab][3 * n + k];
+            }
+        return 0;
+    }
     for (int64_t iX = 0; iX < nx; ++iX)
         for (int64_t iY = 0; iY < ny; ++iY)
             for (int64_t iZ = 0; iZ < nz; ++iZ) {

@@ -60,6 +60,13 @@ int orc_set_spheres(orc_lattice* L, int64_t n, const double* x3, const double* v
                     const int32_t* type, const int32_t* exclude, int n_exclude,
                     int init_vel, int auto_images);
 
+/* Threads of orc_set_spheres and orc_sum_force_torque (default 1): each
+ * thread owns an x-slab, as each MPI rank of the reference owns a block.
+ * set_spheres gives the same records for any count; sum_force_torque adds
+ * the slabs' partial sums in slab order (a different grouping than the
+ * sequential scan). */
+void orc_set_threads(orc_lattice* L, int nthreads);
+
 /* nthreads <= 0: all OpenMP threads. */
 void orc_collide_stream(orc_lattice* L, int nsteps, int nthreads);
 
