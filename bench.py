@@ -596,8 +596,12 @@ def summarize_profiles(d, wl):
         with open(os.path.join(ROOT, "profiles", f"kernels_{wl}_trace.md"), "w") as f:
             f.write(f"# Kernel durations, {WORKLOADS[wl]['desc']}\n\n")
             f.write("Command: `" + TRACE_CMD.format(wl=wl) + "` (rocprofv3 kernel trace, ROCm 7.2, one MI355X).\n")
-            f.write("The run does 2 + 10 coupled steps per precision (FP64, FP32-dev, FP16-store), then "
-                    "the copy ceiling (k_copy16: 1 + 10 copies of the FP64 population store).\n\n")
+            f.write("Per precision (FP64, FP32-dev, FP16-store) the run does 2 + 10 timed coupled steps, "
+                    "then continues to the configuration's step count (200 for the channel) for the error "
+                    "against FP64; after the FP64 run, the copy ceiling (k_copy16: 1 + 10 copies of the FP64 "
+                    "population store). One lattice per precision: the collide's time moves by up to +-8 % "
+                    "between lattices (memory placement), so the bench reports the median over fresh "
+                    "lattices.\n\n")
             f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
             for name, n, tot, avg, pct in rows[:28]:
                 f.write(f"| {_short(name)} | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |\n")
@@ -661,8 +665,8 @@ def summarize_profiles(d, wl):
         t = {}
     t.setdefault("workloads", {})[wl] = {"bytes_per_launch": traffic, "fetch_scale": fetch_f,
                                          "write_scale": write_f, "source": f"collide_stream_{wl}_hbm_traffic.md"}
-    t.pop("workload", None)
-    t.pop("bytes_per_launch", None)
+    for stale in ("workload", "bytes_per_launch", "fetch_scale", "write_scale"):
+        t.pop(stale, None)
     with open(TRAFFIC_JSON, "w") as f:
         json.dump(t, f, indent=1)
 
