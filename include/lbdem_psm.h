@@ -139,6 +139,18 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
 int lbdem_psm_subcell_counts(lbdem_psm* h, int64_t n, const double* d3, const double* r,
                              double* sf, int32_t* count, int32_t* branch);
 
+/* Allocation-free form of lbdem_psm_subcell_counts.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results; it calls no hipMalloc / hipFree
+ * and stages the queries in a fixed device array of
+ * LBDEM_PSM_STATIC_MAX_QUERIES x 48 bytes (3 MiB) that each GPU holds until
+ * the process ends.  n above that limit returns LBDEM_PSM_ERR_CAPACITY.
+ * All calls on one GPU share the array: calls that overlap on one GPU race. */
+#define LBDEM_PSM_STATIC_MAX_QUERIES 65536
+int lbdem_psm_subcell_counts_static(lbdem_psm* h, int64_t n, const double* d3, const double* r,
+                                    double* sf, int32_t* count, int32_t* branch);
+
 /* Density and velocity per cell, computed on the device (Palabos
  * computeDensity / computeVelocity of the BGK cell; with Guo forcing
  * u = j/rho + a/2): rho (Nc), u3 (3*Nc, component-major); either may be

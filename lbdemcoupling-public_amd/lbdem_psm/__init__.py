@@ -127,6 +127,7 @@ _SIGS = {
     "lbdem_psm_get_cell_records": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "lbdem_psm_set_cell_records": ([_P, _P, _P, _P], ctypes.c_int),
     "lbdem_psm_subcell_counts": ([_P, ctypes.c_int64, _P, _P, _P, _P, _P], ctypes.c_int),
+    "lbdem_psm_subcell_counts_static": ([_P, ctypes.c_int64, _P, _P, _P, _P, _P], ctypes.c_int),
     "lbdem_psm_conservation": ([_P, _P, _P], ctypes.c_int),
     "lbdem_psm_check_health": ([_P, _P, _P], ctypes.c_int),
     "lbdem_psm_stats": ([_P, _P], ctypes.c_int),
