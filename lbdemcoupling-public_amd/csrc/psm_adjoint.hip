@@ -715,25 +715,41 @@ __global__ __launch_bounds__(256) void k_grad_particle(GradArgs a) {
 }
 
 /* Records no candidate set in the last voxelization (kept from earlier
- * steps), added with FP64 atomics onto map(id). */
-__global__ __launch_bounds__(256) void k_grad_orphans(GradArgs a, const int32_t* __restrict__ orphan,
-                                                      const int64_t* __restrict__ counters, int64_t ocap,
-                                                      const int32_t* __restrict__ id_map,
-                                                      int64_t id_map_size) {
-    const int64_t n = counters[2] < ocap ? counters[2] : ocap;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = orphan[k];
-        const int id = a.rid[r];
-        if (id < 1 || id >= id_map_size) continue;
-        const int o = id_map[id];
-        if (o < 0) continue;
+ * steps), grouped by owner in cell order (order_orphans): one wavefront per
+ * particle adds its records' contributions lane-strided in that order, then
+ * a fixed-order tree, once onto the particle's sum of its entries.  Clears
+ * the particle's count and fill cursor for the next grouping. */
+__global__ __launch_bounds__(256) void k_grad_orphans(GradArgs a, int64_t np, int32_t* __restrict__ cnt,
+                                                      int32_t* __restrict__ fill,
+                                                      const int32_t* __restrict__ start,
+                                                      const int32_t* __restrict__ sorted) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= np) return;
+    const int m = cnt[q];
+    if (m == 0) return;
+    const int lane = (int)__lane_id();
+    const int32_t* seg = sorted + start[q];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = lane; i < m; i += 64) {
+        const int64_t r = seg[i];
+        if (!a.sflag[r]) continue;
         int X, Y, Z;
         rec_cell(r, a.n[1], a.n[2], X, Y, Z);
-        if (!a.sflag[r]) continue;
-        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        record_grad(a, r, X, Y, Z, o, acc);
-        for (int m = 0; m < 9; ++m) atomicAdd(&a.gpart[9 * (int64_t)o + m], acc[m]);
+        record_grad(a, r, X, Y, Z, (int)q, acc);
+    }
+    double first8[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) first8[k] = acc[k];
+    const double s8 = wave_sum8_scatter(first8, lane);
+    double s9 = acc[8];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s9 += __shfl_xor(s9, off, 64);
+    const int mm = lane >> 3;
+    if ((lane & 7) == 0) a.gpart[9 * q + mm] += s8;
+    if (lane == 0) {
+        a.gpart[9 * q + 8] += s9;
+        cnt[q] = 0;
+        fill[q] = 0;
     }
 }
 
@@ -1246,8 +1262,10 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
         ga.overlap = h->desc.voxel_mode == LBDEM_PSM_VOXEL_OVERLAP ? 1 : 0;
         ga.gpart = D.gpart;
         hipLaunchKernelGGL(k_grad_particle, dim3((unsigned)np), dim3(256), 0, h->stream, ga);
-        hipLaunchKernelGGL(k_grad_orphans, dim3(64), dim3(256), 0, h->stream, ga, h->orphan, h->counters,
-                           h->orphan_cap, h->id_map, h->id_map_size);
+        rc = order_orphans(h);
+        if (rc) { dfree_d(tmp64); return rc; }
+        hipLaunchKernelGGL(k_grad_orphans, dim3((unsigned)((np + 3) / 4)), dim3(256), 0, h->stream, ga, (int64_t)np,
+                           h->orph_cnt, h->orph_fill, h->orph_start, h->orph_sorted);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) { dfree_d(tmp64); return hip_check(h, e, "adjoint reductions"); }

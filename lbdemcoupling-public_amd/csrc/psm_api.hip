@@ -108,6 +108,14 @@ static int ensure_particles(lbdem_psm* h, int64_t n) {
     const int64_t cap = std::max<int64_t>(n, std::max<int64_t>(2 * h->p_cap, 256));
     dfree(h->px); dfree(h->pv); dfree(h->pw); dfree(h->pr); dfree(h->pid); dfree(h->pexcl);
     dfree(h->powner); dfree(h->p_nent); dfree(h->p_ent_start); dfree(h->pft); dfree(h->psum_x);
+    dfree(h->orph_cnt); dfree(h->orph_fill); dfree(h->orph_start);
+    h->orph_cnt = h->orph_fill = h->orph_start = nullptr;
+    PSM_HIP(h, dalloc(&h->orph_cnt, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->orph_fill, cap * sizeof(int32_t)));
+    PSM_HIP(h, dalloc(&h->orph_start, cap * sizeof(int32_t)));
+    /* the grouping kernels keep these zero between uses */
+    PSM_HIP(h, hipMemset(h->orph_cnt, 0, cap * sizeof(int32_t)));
+    PSM_HIP(h, hipMemset(h->orph_fill, 0, cap * sizeof(int32_t)));
     PSM_HIP(h, dalloc(&h->px, 3 * cap * sizeof(double)));
     PSM_HIP(h, dalloc(&h->pv, 3 * cap * sizeof(double)));
     PSM_HIP(h, dalloc(&h->pw, 3 * cap * sizeof(double)));
@@ -325,6 +333,8 @@ int lbdem_psm_create(const lbdem_psm_desc* desc, lbdem_psm** out) {
         CREATE_CHECK(hipMemset(h->bin_any[k], 0, h->nbins * sizeof(uint32_t)));
     }
     CREATE_CHECK(dalloc(&h->orphan, h->orphan_cap * sizeof(int32_t)));
+    CREATE_CHECK(dalloc(&h->orph_slot, h->orphan_cap * sizeof(int32_t)));
+    CREATE_CHECK(dalloc(&h->orph_sorted, h->orphan_cap * sizeof(int32_t)));
     CREATE_CHECK(dalloc(&h->bin_cand_count, h->nbins * sizeof(int32_t)));
     CREATE_CHECK(dalloc(&h->bin_cand_start, h->nbins * sizeof(int32_t)));
     CREATE_CHECK(dalloc(&h->bin_fill, h->nbins * sizeof(int32_t)));
@@ -360,6 +370,7 @@ int lbdem_psm_destroy(lbdem_psm* h) {
     dfree(h->bin_fill); dfree(h->cand_key); dfree(h->cand_bin); dfree(h->p_nent);
     dfree(h->p_ent_start); dfree(h->owner_ent); dfree(h->partial); dfree(h->pft); dfree(h->psum_x);
     dfree(h->scan_tmp); dfree(h->counters); dfree(h->red_tmp); dfree(h->orphan);
+    dfree(h->orph_cnt); dfree(h->orph_fill); dfree(h->orph_start); dfree(h->orph_slot); dfree(h->orph_sorted);
     dfree(h->dv_cell); dfree(h->dv_us); dfree(h->pp.buf);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     for (auto& s : h->spans) { (void)hipEventDestroy(s.a); (void)hipEventDestroy(s.b); }

@@ -335,6 +335,37 @@ __device__ __forceinline__ double readlane_d(double v, int L) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+/* Sum of v[0..7] over the 64 lanes of a wavefront, reduce-scatter style:
+ * three halving exchanges (lane bits 5, 4, 3) leave each lane one value,
+ * index m = 4 b5 + 2 b4 + b3 of the lane, summed over the eight lanes that
+ * share those bits; three butterflies over bits 2, 1, 0 finish it.  Ten
+ * shuffles of a double in place of 36 for six separate trees, and the order
+ * of every addition is fixed (a + b and b + a round alike), so the sum is
+ * deterministic. */
+__device__ __forceinline__ double wave_sum8_scatter(const double (&v)[8], int lane) {
+    double w4[4], w2[2];
+    const bool u5 = (lane & 32) != 0, u4 = (lane & 16) != 0, u3 = (lane & 8) != 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        const double keep = u5 ? v[4 + m] : v[m];
+        const double send = u5 ? v[m] : v[4 + m];
+        w4[m] = keep + __shfl_xor(send, 32, 64);
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const double keep = u4 ? w4[2 + m] : w4[m];
+        const double send = u4 ? w4[m] : w4[2 + m];
+        w2[m] = keep + __shfl_xor(send, 16, 64);
+    }
+    const double keep = u3 ? w2[1] : w2[0];
+    const double send = u3 ? w2[0] : w2[1];
+    double w = keep + __shfl_xor(send, 8, 64);
+    w += __shfl_xor(w, 4, 64);
+    w += __shfl_xor(w, 2, 64);
+    w += __shfl_xor(w, 1, 64);
+    return w;
+}
+
 struct PopLayout {
     int zc;
     int zsh;
@@ -427,6 +458,10 @@ struct lbdem_psm {
     int32_t *p_nent = nullptr, *p_ent_start = nullptr, *owner_ent = nullptr;
     int32_t* orphan = nullptr;       /* record indices without a candidate this step */
     int64_t orphan_cap = 0;
+    /* stale records grouped by owner in cell order (order_orphans): per
+     * particle count / fill cursor / start, and the placed and sorted cells */
+    int32_t *orph_cnt = nullptr, *orph_fill = nullptr, *orph_start = nullptr;
+    int32_t *orph_slot = nullptr, *orph_sorted = nullptr;
     double* partial = nullptr;       /* 6 per candidate entry */
     double* pft = nullptr;           /* 6 per particle: F[3], T[3] */
     double* psum_x = nullptr;        /* positions for torque arms */
@@ -525,6 +560,9 @@ int launch_subcell(lbdem_psm* h, int64_t n, const double* d_d3, const double* d_
 /* xs: positions for the torque arms; out: 6 per particle (F, T), the
  * handle's own buffer when null */
 int run_force_sum(lbdem_psm* h, const double* xs, double* out = nullptr);
+/* Groups the stale-record list by owner particle, each particle's cells in
+ * increasing cell order: orph_sorted[orph_start[q] .. + orph_cnt[q]) */
+int order_orphans(lbdem_psm* h);
 int count_records(lbdem_psm* h);
 int launch_advance(lbdem_psm* h);
 int launch_define_velocity(lbdem_psm* h);

@@ -15,9 +15,12 @@
  *  2. per particle: its candidate entries are added in the fixed
  *     (image rank, bin) order.
  * Records whose owner set no candidate of their bin this step (stale
- * records the reference also keeps) are added afterwards with FP64 atomics.
+ * records the reference also keeps) are grouped by owner and added
+ * afterwards, each particle's in cell order (k_force_orphans).
  */
 #include "psm_common.h"
+
+#include <hipcub/hipcub.hpp>
 
 namespace psm {
 
@@ -37,37 +40,6 @@ struct ForceArgs {
     int64_t ent_cap;
     int64_t nbins;
 };
-
-/* Sum of v[0..7] over the 64 lanes of a wavefront, reduce-scatter style:
- * three halving exchanges (lane bits 5, 4, 3) leave each lane one value,
- * index m = 4 b5 + 2 b4 + b3 of the lane, summed over the eight lanes that
- * share those bits; three butterflies over bits 2, 1, 0 finish it.  Ten
- * shuffles of a double in place of 36 for six separate trees, and the order
- * of every addition is fixed (a + b and b + a round alike), so the sum is
- * deterministic. */
-__device__ __forceinline__ double wave_sum8_scatter(const double (&v)[8], int lane) {
-    double w4[4], w2[2];
-    const bool u5 = (lane & 32) != 0, u4 = (lane & 16) != 0, u3 = (lane & 8) != 0;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-        const double keep = u5 ? v[4 + m] : v[m];
-        const double send = u5 ? v[m] : v[4 + m];
-        w4[m] = keep + __shfl_xor(send, 32, 64);
-    }
-#pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const double keep = u4 ? w4[2 + m] : w4[m];
-        const double send = u4 ? w4[m] : w4[2 + m];
-        w2[m] = keep + __shfl_xor(send, 16, 64);
-    }
-    const double keep = u3 ? w2[1] : w2[0];
-    const double send = u3 ? w2[0] : w2[1];
-    double w = keep + __shfl_xor(send, 8, 64);
-    w += __shfl_xor(w, 4, 64);
-    w += __shfl_xor(w, 2, 64);
-    w += __shfl_xor(w, 1, 64);
-    return w;
-}
 
 /* One wavefront per bin, four bins per workgroup.  Lane (y, z) of the bin
  * holds its eight cells (x, y, z), x = 0..7, in registers: the liveness
@@ -203,23 +175,114 @@ __global__ __launch_bounds__(256) void k_force_particle(int64_t np, const int32_
     if (q < np && (lane & 7) == 0 && m < 6) pft[6 * q + m] = v;
 }
 
-__global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_t* __restrict__ orphan,
-                                                       const int64_t* __restrict__ counters, int64_t ocap,
-                                                       const int32_t* __restrict__ id_map,
-                                                       int64_t id_map_size, double* __restrict__ pft) {
+/* ---- stale records, summed per owner in cell order ----
+ * Records whose owner set no candidate of their bin this step (kept from
+ * earlier steps, or all records after set_cell_records / load_checkpoint)
+ * come as an unordered list of cells.  They are grouped by owner particle
+ * (count, scan, place) and each particle's cells are put in increasing cell
+ * order (each cell's rank among its particle's cells), the order of the
+ * reference's lexicographic scan (src/ibProcessors3D.hh:210-260); the sum
+ * over them then has a fixed order whatever order the list was built in. */
+
+__device__ __forceinline__ int orphan_owner(int64_t r, const int32_t* __restrict__ rid,
+                                            const int32_t* __restrict__ id_map, int64_t id_map_size) {
+    const int id = rid[r];
+    if (id < 1 || id >= id_map_size) return -1;
+    return id_map[id];
+}
+
+__global__ __launch_bounds__(256) void k_orph_count(const int32_t* __restrict__ orphan,
+                                                    const int64_t* __restrict__ counters, int64_t ocap,
+                                                    const int32_t* __restrict__ rid,
+                                                    const int32_t* __restrict__ id_map, int64_t id_map_size,
+                                                    int32_t* __restrict__ cnt) {
     const int64_t n = counters[2] < ocap ? counters[2] : ocap;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t r = orphan[k];
-        const int id = a.rec.id[r];
-        if (id < 1 || id >= id_map_size) continue;
-        const int o = id_map[id];
-        if (o < 0) continue;
+        const int o = orphan_owner(orphan[k], rid, id_map, id_map_size);
+        if (o >= 0) atomicAdd(&cnt[o], 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_orph_place(const int32_t* __restrict__ orphan,
+                                                    const int64_t* __restrict__ counters, int64_t ocap,
+                                                    const int32_t* __restrict__ rid,
+                                                    const int32_t* __restrict__ id_map, int64_t id_map_size,
+                                                    const int32_t* __restrict__ start, int32_t* __restrict__ fill,
+                                                    int32_t* __restrict__ slot) {
+    const int64_t n = counters[2] < ocap ? counters[2] : ocap;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t r = orphan[k];
+        const int o = orphan_owner(r, rid, id_map, id_map_size);
+        if (o >= 0) slot[start[o] + atomicAdd(&fill[o], 1)] = r;
+    }
+}
+
+/* One wavefront per particle: each of its cells goes to its rank among
+ * them (the cells are distinct), so orph_sorted holds them in cell order. */
+__global__ __launch_bounds__(256) void k_orph_rank(int64_t np, const int32_t* __restrict__ cnt,
+                                                   const int32_t* __restrict__ start,
+                                                   const int32_t* __restrict__ slot,
+                                                   int32_t* __restrict__ sorted) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= np) return;
+    const int m = cnt[q];
+    if (m == 0) return;
+    const int lane = (int)__lane_id();
+    const int32_t* seg = slot + start[q];
+    for (int i0 = 0; i0 < m; i0 += 64) {
+        const bool mine = i0 + lane < m;
+        const int32_t ka = mine ? seg[i0 + lane] : 0;
+        int rank = 0;
+        for (int c0 = 0; c0 < m; c0 += 64) {
+            const int32_t kb = c0 + lane < m ? seg[c0 + lane] : INT32_MAX;
+            const int top = m - c0 < 64 ? m - c0 : 64;
+            for (int s2 = 0; s2 < top; ++s2) rank += __shfl(kb, s2, 64) < ka ? 1 : 0;
+        }
+        if (mine) sorted[start[q] + rank] = ka;
+    }
+}
+
+int order_orphans(lbdem_psm* h) {
+    if (h->np == 0) return LBDEM_PSM_OK;
+    const unsigned og = 256;
+    hipLaunchKernelGGL(k_orph_count, dim3(og), dim3(256), 0, h->stream, h->orphan, h->counters, h->orphan_cap,
+                       h->rec.id, h->id_map, h->id_map_size, h->orph_cnt);
+    size_t tb = h->scan_tmp_bytes;
+    PSM_HIP(h, hipcub::DeviceScan::ExclusiveSum(h->scan_tmp, tb, h->orph_cnt, h->orph_start, (int)h->np, h->stream));
+    hipLaunchKernelGGL(k_orph_place, dim3(og), dim3(256), 0, h->stream, h->orphan, h->counters, h->orphan_cap,
+                       h->rec.id, h->id_map, h->id_map_size, h->orph_start, h->orph_fill, h->orph_slot);
+    hipLaunchKernelGGL(k_orph_rank, dim3((unsigned)((h->np + 3) / 4)), dim3(256), 0, h->stream, h->np, h->orph_cnt,
+                       h->orph_start, h->orph_slot, h->orph_sorted);
+    PSM_HIP(h, hipGetLastError());
+    return LBDEM_PSM_OK;
+}
+
+/* One wavefront per particle: its stale records in cell order, lane-strided,
+ * then the fixed reduce-scatter tree; added once to the particle's sum of
+ * its candidate entries.  Clears the particle's count and fill cursor for
+ * the next grouping. */
+__global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, int64_t np, int32_t* __restrict__ cnt,
+                                                       int32_t* __restrict__ fill,
+                                                       const int32_t* __restrict__ start,
+                                                       const int32_t* __restrict__ sorted,
+                                                       double* __restrict__ pft) {
+    const int64_t q = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (q >= np) return;
+    const int m = cnt[q];
+    if (m == 0) return;
+    const int lane = (int)__lane_id();
+    const int32_t* seg = sorted + start[q];
+    const double xo = a.xs[3 * q], yo = a.xs[3 * q + 1], zo = a.xs[3 * q + 2];
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = lane; i < m; i += 64) {
+        const int64_t r = seg[i];
         int X, Y, Z;
         rec_cell(r, a.n[1], a.n[2], X, Y, Z);
-        const double dx = min_image((double)X - a.xs[3 * o], a.n[0]);
-        const double dy = min_image((double)Y - a.xs[3 * o + 1], a.n[1]);
-        const double dz = min_image((double)Z - a.xs[3 * o + 2], a.n[2]);
+        const double dx = min_image((double)X - xo, a.n[0]);
+        const double dy = min_image((double)Y - yo, a.n[1]);
+        const double dz = min_image((double)Z - zo, a.n[2]);
         double fx, fy, fz;
         if (a.f32) {
             const float* F32 = (const float*)a.rec.F;
@@ -227,12 +290,19 @@ __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, const int32_
         } else {
             fx = a.rec.F[r]; fy = a.rec.F[a.rcap + r]; fz = a.rec.F[2 * a.rcap + r];
         }
-        atomicAdd(&pft[6 * o + 0], fx);
-        atomicAdd(&pft[6 * o + 1], fy);
-        atomicAdd(&pft[6 * o + 2], fz);
-        atomicAdd(&pft[6 * o + 3], dy * fz - dz * fy);
-        atomicAdd(&pft[6 * o + 4], -dx * fz + dz * fx);
-        atomicAdd(&pft[6 * o + 5], dx * fy - dy * fx);
+        acc[0] += fx;
+        acc[1] += fy;
+        acc[2] += fz;
+        acc[3] += dy * fz - dz * fy;
+        acc[4] += -dx * fz + dz * fx;
+        acc[5] += dx * fy - dy * fx;
+    }
+    const double s = wave_sum8_scatter(acc, lane);
+    const int mm = lane >> 3;
+    if ((lane & 7) == 0 && mm < 6) pft[6 * q + mm] += s;
+    if (lane == 0) {
+        cnt[q] = 0;
+        fill[q] = 0;
     }
 }
 
@@ -260,8 +330,10 @@ int run_force_sum(lbdem_psm* h, const double* xs, double* out) {
     if (h->np > 0) {
         hipLaunchKernelGGL(k_force_particle, dim3((unsigned)((h->np + 3) / 4)), dim3(256), 0, h->stream,
                            h->np, h->p_ent_start, h->p_nent, h->owner_ent, h->partial, pft, h->ent_cap);
-        hipLaunchKernelGGL(k_force_orphans, dim3(64), dim3(256), 0, h->stream, a, h->orphan, h->counters,
-                           h->orphan_cap, h->id_map, h->id_map_size, pft);
+        int rc = order_orphans(h);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_force_orphans, dim3((unsigned)((h->np + 3) / 4)), dim3(256), 0, h->stream, a, h->np,
+                           h->orph_cnt, h->orph_fill, h->orph_start, h->orph_sorted, pft);
     }
     PSM_HIP(h, hipGetLastError());
     prof_end(h, 2, ev);

@@ -154,8 +154,18 @@ public:
         torqueFactor = std::pow(dx, 5) / (dt * dt) * rho_p * std::pow(l_p, 5) / (t_p * t_p);
     }
     T getRe() const { return param.getRe(); }
+    /* offsets of getLbPositionX/Y/Z only; the exchange's getLbPosition keeps 1/2 */
+    void setLbOffset(T ox, T oy, T oz) { off[0] = ox; off[1] = oy; off[2] = oz; }
     T getLbLength(T x) const { return x / l_p / param.getDeltaX(); }
     T getLbPosition(T x) const { return x / l_p / param.getDeltaX() - 0.5; }
+    T getLbPositionX(T x) const { return x / l_p / param.getDeltaX() - off[0]; }
+    T getLbPositionY(T x) const { return x / l_p / param.getDeltaX() - off[1]; }
+    T getLbPositionZ(T x) const { return x / l_p / param.getDeltaX() - off[2]; }
+    T getLbDensity(T rho) const { return rho / rho_p; }
+    /* lattice density deviation rho - 1 of a physical pressure (p = (rho - 1)
+     * c_s^2 in the force / area unit); as in the reference it is not the
+     * inverse of getPhysPress, which takes rho itself */
+    T getLbRho(T p, T pOffset = 0) const { return (p - pOffset) * 3. / getPhysForce(1.) * std::pow(getPhysLength(1.), 2); }
     T getLbVel(T u) const { return u / u_p * param.getLatticeU(); }
     T getLbFreq(T f) const { return f * t_p * param.getDeltaT(); }
     T getLbAccel(T a) const { return a * (t_p * t_p / l_p) * (param.getDeltaT() * param.getDeltaT() / param.getDeltaX()); }
@@ -163,6 +173,12 @@ public:
     plint getLbSteps(T t) const { return (plint)(t / param.getDeltaT() / t_p + 0.5); }
     T getPhysLength(T l) const { return l * param.getDeltaX() * l_p; }
     T getPhysPosition(T l) const { return (l + 0.5) * param.getDeltaX() * l_p; }
+    T getPhysPositionX(T l) const { return (l + off[0]) * param.getDeltaX() * l_p; }
+    T getPhysPositionY(T l) const { return (l + off[1]) * param.getDeltaX() * l_p; }
+    T getPhysPositionZ(T l) const { return (l + off[2]) * param.getDeltaX() * l_p; }
+    T getPhysDensity(T rho) const { return rho * rho_p; }
+    T getPhysAccel(T a) const { return a / getLbAccel(1.); }
+    T getPhysPress(T rho, T pOffset = 0) const { return (rho - 1.) / 3. * getPhysForce(1.) / std::pow(getPhysLength(1.), 2) + pOffset; }
     T getPhysVel(T u) const { return u / param.getLatticeU() * u_p; }
     T getPhysTime(T t) const { return t * param.getDeltaT() * t_p; }
     T getPhysForce(T f) const { return f * forceFactor; }
@@ -172,6 +188,7 @@ private:
     T l_p, u_p, t_p, nu_p, rho_p;
     IncomprFlowParam<T> param;
     T forceFactor = 0, torqueFactor = 0;
+    T off[3] = {0.5, 0.5, 0.5};
 };
 
 /* ---- the lattice ---- */

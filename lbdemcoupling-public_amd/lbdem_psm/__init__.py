@@ -474,19 +474,45 @@ class Lattice:
         assert dsf.shape[0] == self.nc
         self._c(self.lib.lbdem_psm_perturb_solid_fraction(self.h, _ptr(dsf)))
 
+    # the last differentiable window: (token, nsteps, particles); every
+    # diff_forward gets a new token, so a backward pass can tell whether the
+    # lattice still holds its window
+    _window = None
+    _window_token = 0
+
     def diff_forward(self, nsteps, checkpoint_every):
         """Runs the window; returns (nsteps, n, 6) forces and torques."""
+        self._window = None
         ft = np.zeros((int(nsteps), self.np_, 6))
         self._c(self.lib.lbdem_psm_diff_forward(self.h, int(nsteps), int(checkpoint_every),
                                                 _ptr(ft)))
+        self._window_token += 1
+        self._window = (self._window_token, int(nsteps), self.np_)
         return ft
 
-    def diff_backward(self, g_ft=None, g_fT=None, want_f0=False, want_cells=True):
+    def window_token(self):
+        """Token of the window the lattice holds (None without one)."""
+        return None if self._window is None else self._window[0]
+
+    def diff_backward(self, g_ft=None, g_fT=None, want_f0=False, want_cells=True, token=None):
         """Adjoint of the last diff_forward: dict of gradients.  want_cells=False
-        skips the dense per-cell sf and B gradients (2 Nc doubles)."""
-        n = self.np_
+        skips the dense per-cell sf and B gradients (2 Nc doubles).  token:
+        the window_token() taken after that forward; raises if the lattice
+        has run another window since.  g_ft must be (nsteps, n, 6) and g_fT
+        19 x Nc."""
+        if self._window is None:
+            raise PSMError(ERR_STATE, "diff_backward without a diff_forward window")
+        tok, nsteps, n = self._window
+        if token is not None and token != tok:
+            raise PSMError(ERR_STATE, "the lattice ran another window after this one's forward pass")
         gft = None if g_ft is None else _f64(g_ft)
-        gfT = None if g_fT is None else _f64(g_fT).reshape(Q, -1)
+        gfT = None if g_fT is None else _f64(g_fT)
+        if gft is not None and gft.shape != (nsteps, n, 6):
+            raise PSMError(ERR_INVALID_ARG, f"g_ft has shape {gft.shape}, the window needs {(nsteps, n, 6)}")
+        if gfT is not None:
+            if gfT.size != Q * self.nc:
+                raise PSMError(ERR_INVALID_ARG, f"g_fT has {gfT.size} values, the window needs {Q * self.nc}")
+            gfT = gfT.reshape(Q, -1)
         gx = np.zeros((n, 3))
         gv = np.zeros((n, 3))
         gw = np.zeros((n, 3))
@@ -503,6 +529,7 @@ class Lattice:
         return out
 
     def diff_release(self):
+        self._window = None
         self._c(self.lib.lbdem_psm_diff_release(self.h))
 
     # diagnostics -------------------------------------------------------

@@ -41,12 +41,15 @@ class PSMWindow(torch.autograd.Function):
             lattice.perturb_solid_fraction(d)
         ft = lattice.diff_forward(int(nsteps), int(checkpoint_every))
         ctx.lattice = lattice
+        ctx.token = lattice.window_token()
         ctx.dtype = x.dtype
         return torch.from_numpy(ft).to(x.dtype)
 
     @staticmethod
     def backward(ctx, g_ft):
-        g = ctx.lattice.diff_backward(g_ft=_np(g_ft.contiguous()))
+        # raises if the lattice ran another window after this forward pass
+        # (its checkpoints, tau and records would be that window's)
+        g = ctx.lattice.diff_backward(g_ft=_np(g_ft.contiguous()), token=ctx.token)
         dt = ctx.dtype
         return (torch.from_numpy(g["x"]).to(dt), torch.from_numpy(g["v"]).to(dt),
                 torch.from_numpy(g["w"]).to(dt), torch.tensor(g["tau"], dtype=dt),

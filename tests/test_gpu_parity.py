@@ -209,6 +209,48 @@ def test_stale_records_are_kept_and_summed():
     g.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_dev"])
+def test_stale_record_sums_are_deterministic(precision):
+    """Stale records of several particles -- every record after
+    set_cell_records, and the records a particle leaves behind when it jumps
+    beyond its bounding box -- are summed per owner in cell order: two
+    handles give bit-identical forces and torques, and FP64 matches the
+    oracle within 1e-12."""
+    n = (40, 36, 32)
+    x, v, w, r, ids = spheres(n, 12, 91, 3.0, 5.0)
+    ref = O.OracleLattice(*n, tau=0.7, slices_per_dim=8)
+    ref.set_spheres(x + 0.4, v, w, r, ids)
+    prior = ref.cell_records()
+    ref.close()
+    jump = x.copy()
+    jump[::2, 2] = (jump[::2, 2] + 9.0) % n[2]        # half of them leave their records behind
+    runs = []
+    for _ in range(2):
+        g = L.Lattice(*n, tau=0.7, precision=precision, slices_per_dim=8)
+        o = O.OracleLattice(*n, tau=0.7, slices_per_dim=8) if precision == "fp64" else None
+        out = []
+        for lat in ([g, o] if o else [g]):
+            lat.set_spheres(x, v, w, r, ids)
+            lat.set_cell_records(prior["sf"], prior["id"], prior["us"])   # all stale
+            lat.collide_stream(1)
+            F1, T1 = lat.sum_force_torque(x)
+            lat.set_spheres(jump, v, w, r, ids)                          # records left behind
+            lat.collide_stream(1)
+            F2, T2 = lat.sum_force_torque(jump)
+            out.append((F1, T1, F2, T2))
+        assert g.stats()["orphans"] > 0
+        if o:
+            for a, b in ((0, 1), (2, 3)):
+                assert rel_err(out[0][a], out[1][a]) < FORCE_RTOL
+                assert torque_err(out[0][b], out[1][b], out[1][a]) < FORCE_RTOL
+            o.close()
+        runs.append(out[0])
+        g.close()
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+    assert np.abs(runs[0][0]).max() > 1e-6 and np.abs(runs[0][2]).max() > 1e-6
+
+
 def test_many_candidates_in_one_bin():
     """More than 64 small spheres whose boxes meet one 8^3 bin: the force sum
     takes the positions of entries past the 64th from memory, and the fold
