@@ -17,7 +17,11 @@ implementation.
   fraction; Hasimoto 1959, J. Fluid Mech. 5, 317; coefficients as tabulated
   by Sangani and Acrivos 1982, Int. J. Multiphase Flow 8, 343).  The
   balance is exact at steady state; Stokes' law holds to the method's
-  effective radius, so its tolerance is 10 %.
+  effective radius, which approaches the geometric one with resolution.
+- Sedimenting sphere (the reference's validation case,
+  examples/sedimentingSphere/README.md:46-61): after 2000 steps the sphere
+  settles at a steady velocity and the hydrodynamic force balances the
+  buoyancy-reduced weight m g (rho_s - rho_f) / rho_s.
 """
 import numpy as np
 import pytest
@@ -58,27 +62,30 @@ def hasimoto_K(c):
     return 1.0 / (1.0 - 1.7601 * c ** (1.0 / 3.0) + c - 1.5593 * c * c)
 
 
-def test_fixed_sphere_in_periodic_box_stokes_drag():
-    n, a, tau, g = 32, 5.0, 1.0, 1e-6
+def fixed_sphere_drag(n, a, tau=1.0, g=1e-6, steps=8000, check=2000):
+    """Steady drag on a fixed sphere in a periodic box driven by a body force:
+    (drag, mean velocity, drag change over the last `check` steps, momentum
+    balance ratio, transverse drag ratio)."""
     nu = (tau - 0.5) / 3.0
     c = 4.0 / 3.0 * np.pi * a ** 3 / n ** 3
     lat = L.Lattice(n, n, n, periodic=(1, 1, 1), tau=tau, precision="fp64", slices_per_dim=8,
                     body_accel=(g, 0.0, 0.0))
     try:
-        # The mean flow relaxes with the drag time rho V / (6 pi mu a K), about
-        # 1200 steps here: start it at the Stokes-Hasimoto estimate, so that
-        # 8000 steps leave a residual far below the tolerances.
+        # The mean flow relaxes with the drag time rho V / (6 pi mu a K)
+        # (~1200 steps at n = 32, a = 5, growing as a^2 at fixed c): start it
+        # at the Stokes-Hasimoto estimate, so that ~6.7 relaxation times leave
+        # a residual far below the tolerances.
         nc = n ** 3
         u_guess = g * nc / (6.0 * np.pi * nu * a * hasimoto_K(c))
         u0 = np.zeros((3, nc))
         u0[0] = u_guess
         lat.init_equilibrium(np.ones(nc), u0)
-        x = np.array([[15.8, 15.6, 15.3]])
+        x = np.array([[n / 2 - 0.2, n / 2 - 0.4, n / 2 - 0.7]])
         z = np.zeros((1, 3))
         lat.set_spheres(x, z, z, np.array([a]), np.array([1]))
-        lat.step(8000, advance=False)
+        lat.step(steps, frozen=True)
         F1, _ = lat.particle_forces()
-        lat.step(2000, advance=False)
+        lat.step(check, frozen=True)
         F2, _ = lat.particle_forces()
         rho, u = lat.macroscopic()
         sf = lat.cell_records()["sf"]
@@ -87,22 +94,69 @@ def test_fixed_sphere_in_periodic_box_stokes_drag():
     fx = F2[0, 0]
     U = u[0].mean()
     ratio = fx / (6.0 * np.pi * nu * a * U * hasimoto_K(c))
+    ot = tau - 0.5
+    B = np.where(sf > 0.999, 1.0, sf * ot / ((1.0 - sf) + ot))
+    balance = fx / (g * (rho * (1.0 - B)).sum())
+    return fx, U, abs(F2[0, 0] - F1[0, 0]) / abs(fx), ratio, balance, np.abs(F2[0, 1:]).max() / fx
+
+
+def test_fixed_sphere_in_periodic_box_stokes_drag():
+    n, a, tau = 32, 5.0, 1.0
+    fx, U, change, ratio, balance, cross = fixed_sphere_drag(n, a, tau)
+    nu = (tau - 0.5) / 3.0
     print(f"Stokes-Hasimoto ratio {ratio:.4f}, Re = {U * 2 * a / nu:.3f}")
     # steady: the last 2000 steps changed the drag by < 1e-3
-    assert abs(F2[0, 0] - F1[0, 0]) < 1e-3 * abs(fx), (F1, F2)
+    assert change < 1e-3
     # the drag is along the body force; the cross components are round-off
     # and lattice anisotropy
-    assert fx > 0 and np.abs(F2[0, 1:]).max() < 1e-3 * fx
+    assert fx > 0 and cross < 1e-3
     # momentum balance: the drag equals the body force on the fluid.  The
     # Guo term enters a cell's BGK part, which the blend weights by 1 - B
     # (B = 1 in full cells, src/ibCompositeDynamics3D.hh:159-199)
-    ot = tau - 0.5
-    B = np.where(sf > 0.999, 1.0, sf * ot / ((1.0 - sf) + ot))
-    total = g * (rho * (1.0 - B)).sum()
-    assert abs(fx / total - 1.0) < 2e-3, (fx, total)
-    # Stokes drag with Hasimoto's periodic-array correction
-    # measured 0.926: the drag of a sphere of hydrodynamic radius 0.955 a, the
-    # smaller effective radius partially-saturated cells are known to give at
-    # this resolution (a = 5 cells); a dropped or sign-flipped term moves the
-    # ratio by far more than the tolerance
-    assert abs(ratio - 1.0) < 0.10, ratio
+    assert abs(balance - 1.0) < 2e-3, balance
+    # Stokes drag with Hasimoto's periodic-array correction: measured 0.926 at
+    # a = 5 cells (the resolution sweep below shows it approaching 1 as the
+    # sphere is resolved); pinned as a regression value
+    assert abs(ratio - 0.926) < 0.01, ratio
+
+
+def test_stokes_drag_converges_with_resolution():
+    """The same volume fraction (c = 0.0639) at a = 5, 7.5 and 10 cells: the
+    drag against Hasimoto's periodic-array Stokes law moves toward 1 as the
+    sphere is resolved (partially-saturated cells give a hydrodynamic radius
+    below the geometric one, by a margin that shrinks with resolution), and
+    the momentum balance holds at every size."""
+    ratios = []
+    for n, a, steps in ((32, 5.0, 8000), (48, 7.5, 18000), (64, 10.0, 32000)):
+        fx, U, change, ratio, balance, cross = fixed_sphere_drag(n, a, 1.0, steps=steps, check=2000)
+        print(f"a = {a}: ratio {ratio:.4f}, effective radius {a * ratio:.3f}, change {change:.1e}, balance {balance:.5f}")
+        assert change < 2e-3 and abs(balance - 1.0) < 2e-3
+        ratios.append(ratio)
+    assert ratios[0] < ratios[1] < ratios[2] <= 1.01, ratios
+    assert abs(ratios[2] - 1.0) < abs(ratios[0] - 1.0) / 2, ratios
+
+
+def test_sedimenting_sphere_reaches_force_balance():
+    """The sedimenting-sphere configuration (40 x 40 x 120, walls x/y,
+    periodic z, tau 0.8, r = 6, density ratio 1.5, g = 1e-6 on the particle
+    reduced by (rho_s - rho_f) / rho_s, 8^3 sub-cell sampling, 10 DEM
+    substeps) through the C++ facade for its 2000 steps: the settling
+    velocity levels off and the hydrodynamic force equals the
+    buoyancy-reduced weight."""
+    import os
+    import re
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "lbdemcoupling-public_amd",
+                       "examples", "sedimenting_sphere")
+    out = subprocess.run([exe, "2000", "100"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    rows = re.findall(r"step (\d+) z ([-\d.e+]+) vz ([-\d.e+]+) fz ([-\d.e+]+)", out.stdout)
+    assert len(rows) == 20
+    vz = np.array([float(r[2]) for r in rows])
+    fz = np.array([float(r[3]) for r in rows])
+    r, rho_s, g = 6.0, 1.5, 1e-6
+    weight = rho_s * 4.0 / 3.0 * np.pi * r ** 3 * g * (rho_s - 1.0) / rho_s
+    print("vz", vz[-4:], "fz / weight", fz[-4:] / weight)
+    assert vz[-1] < 0
+    assert abs(vz[-1] - vz[-3]) < 0.01 * abs(vz[-1])          # levelled off
+    assert abs(fz[-1] / weight - 1.0) < 0.01
