@@ -106,6 +106,9 @@ static int64_t entry_bound(const lbdem_psm* h, int64_t n, const double* r, bool 
 static int ensure_particles(lbdem_psm* h, int64_t n) {
     if (n <= h->p_cap && h->px) return LBDEM_PSM_OK;
     const int64_t cap = std::max<int64_t>(n, std::max<int64_t>(2 * h->p_cap, 256));
+    /* queued kernels may still use the buffers being replaced */
+    if (h->stream) PSM_HIP(h, hipStreamSynchronize(h->stream));
+    h->ids_valid = false;
     dfree(h->px); dfree(h->pv); dfree(h->pw); dfree(h->pr); dfree(h->pid); dfree(h->pexcl);
     dfree(h->powner); dfree(h->p_nent); dfree(h->p_ent_start); dfree(h->pft); dfree(h->psum_x);
     dfree(h->orph_cnt); dfree(h->orph_fill); dfree(h->orph_start);
@@ -141,6 +144,7 @@ static int ensure_entries(lbdem_psm* h, int64_t need) {
     if (need > (int64_t)INT32_MAX)
         return set_err(h, LBDEM_PSM_ERR_CAPACITY, "candidate entries exceed 2^31");
     if (need <= h->ent_cap && h->cand_key) return LBDEM_PSM_OK;
+    if (h->stream) PSM_HIP(h, hipStreamSynchronize(h->stream));
     dfree(h->cand_key); dfree(h->cand_bin); dfree(h->owner_ent); dfree(h->partial); dfree(h->cand_data);
     PSM_HIP(h, dalloc(&h->cand_data, need * sizeof(Cand)));
     PSM_HIP(h, dalloc(&h->cand_key, need * sizeof(uint32_t)));
@@ -157,13 +161,9 @@ static int read_counters(lbdem_psm* h) {
     return LBDEM_PSM_OK;
 }
 
-static int check_overflow(lbdem_psm* h) {
-    if (h->max_records > 0) {
-        int rc0 = count_records(h);
-        if (rc0) return rc0;
-    }
-    int rc = read_counters(h);
-    if (rc) return rc;
+/* The error flags of a fresh copy of the device counters in h_counters. */
+static int counters_status(lbdem_psm* h) {
+    h->pending_counters = false;
     if (h->h_counters[1] & 2)
         return set_err(h, LBDEM_PSM_ERR_CAPACITY, "candidate-entry capacity exceeded; lattice state is invalid");
     if (h->h_counters[1] & 4)
@@ -175,12 +175,68 @@ static int check_overflow(lbdem_psm* h) {
     return LBDEM_PSM_OK;
 }
 
-static int fold_forces_to_host(lbdem_psm* h, int64_t n, double* F3, double* T3) {
-    std::vector<double> pft(6 * (size_t)n);
-    if (n > 0) {
-        PSM_HIP(h, hipMemcpyAsync(pft.data(), h->pft, 6 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        PSM_HIP(h, hipStreamSynchronize(h->stream));
+static int check_overflow(lbdem_psm* h) {
+    if (h->max_records > 0) {
+        int rc0 = count_records(h);
+        if (rc0) return rc0;
     }
+    int rc = read_counters(h);
+    if (rc) return rc;
+    return counters_status(h);
+}
+
+/* After a stream synchronization: the flags of the counters a set_spheres
+ * call copied without waiting. */
+static int pending_status(lbdem_psm* h) {
+    return h->pending_counters ? counters_status(h) : LBDEM_PSM_OK;
+}
+
+/* A pinned staging buffer of at least `bytes`, the older of the two; waits
+ * for the copies still reading it. */
+static int stage_acquire(lbdem_psm* h, size_t bytes, int* which) {
+    const int k = h->stage_next;
+    h->stage_next ^= 1;
+    lbdem_psm::Stage& s = h->stage[k];
+    if (s.inflight) {
+        PSM_HIP(h, hipEventSynchronize(s.ev));
+        s.inflight = false;
+    }
+    if (!s.ev) PSM_HIP(h, hipEventCreateWithFlags(&s.ev, hipEventDisableTiming));
+    if (bytes > s.bytes) {
+        if (s.buf) (void)hipHostFree(s.buf);
+        s.buf = nullptr;
+        s.bytes = 0;
+        PSM_HIP(h, hipHostMalloc(&s.buf, bytes, hipHostMallocDefault));
+        s.bytes = bytes;
+    }
+    *which = k;
+    return LBDEM_PSM_OK;
+}
+
+static int stage_release(lbdem_psm* h, int which) {
+    PSM_HIP(h, hipEventRecord(h->stage[which].ev, h->stream));
+    h->stage[which].inflight = true;
+    return LBDEM_PSM_OK;
+}
+
+static int io_buffer(lbdem_psm* h, size_t bytes) {
+    if (bytes <= h->h_io_bytes) return LBDEM_PSM_OK;
+    if (h->h_io) (void)hipHostFree(h->h_io);
+    h->h_io = nullptr;
+    h->h_io_bytes = 0;
+    PSM_HIP(h, hipHostMalloc((void**)&h->h_io, bytes, hipHostMallocDefault));
+    h->h_io_bytes = bytes;
+    return LBDEM_PSM_OK;
+}
+
+static int fold_forces_to_host(lbdem_psm* h, int64_t n, double* F3, double* T3) {
+    int rc = io_buffer(h, 6 * (size_t)std::max<int64_t>(n, 1) * sizeof(double));
+    if (rc) return rc;
+    const double* pft = h->h_io;
+    if (n > 0) PSM_HIP(h, hipMemcpyAsync(h->h_io, h->pft, 6 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    rc = pending_status(h);
+    if (rc) return rc;
     /* getForcesFromLattice: f[map(tag[i])] += force[i]; the ghost slots end 0. */
     for (int64_t q = 0; q < n; ++q)
         for (int d = 0; d < 3; ++d) {
@@ -373,6 +429,11 @@ int lbdem_psm_destroy(lbdem_psm* h) {
     dfree(h->orph_cnt); dfree(h->orph_fill); dfree(h->orph_start); dfree(h->orph_slot); dfree(h->orph_sorted);
     dfree(h->dv_cell); dfree(h->dv_us); dfree(h->pp.buf);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
+    for (auto& st : h->stage) {
+        if (st.buf) (void)hipHostFree(st.buf);
+        if (st.ev) (void)hipEventDestroy(st.ev);
+    }
+    if (h->h_io) (void)hipHostFree(h->h_io);
     for (auto& s : h->spans) { (void)hipEventDestroy(s.a); (void)hipEventDestroy(s.b); }
     for (auto e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -443,45 +504,84 @@ int lbdem_psm_set_spheres(lbdem_psm* h, int64_t n, const double* x3, const doubl
     if (rc) return set_err(h, rc, g_create_err);
     if (n_exclude < 0 || (n_exclude > 0 && !exclude))
         return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "bad exclude list");
-    /* host copies below go through the null stream: drain the handle's stream first */
-    PSM_HIP(h, hipStreamSynchronize(h->stream));
+    /* Stream-ordered: the particle arrays go up through pinned staging with
+     * asynchronous copies on the handle's stream, the voxelization is queued
+     * behind them, and nothing waits for the device (unless a buffer must
+     * grow, or max_solid_cells asks for the record count now). */
     rc = ensure_particles(h, n);
     if (rc) return rc;
     const bool auto_images = (flags & LBDEM_PSM_AUTO_IMAGES) != 0;
     rc = ensure_entries(h, entry_bound(h, n, r, auto_images));
     if (rc) return rc;
 
-    /* map(tag): the first index holding each id (owned atoms precede ghosts) */
+    /* the map(tag) inputs: rebuilt and uploaded only when they changed */
+    const bool same_ids = h->ids_valid && (int64_t)h->h_ids.size() == n &&
+                          (n == 0 || std::memcmp(h->h_ids.data(), id, (size_t)n * sizeof(int64_t)) == 0) &&
+                          h->h_has_types == (type != nullptr) &&
+                          (!type || std::memcmp(h->h_types.data(), type, (size_t)n * sizeof(int32_t)) == 0) &&
+                          (int64_t)h->h_exclude.size() == n_exclude &&
+                          (n_exclude == 0 || std::memcmp(h->h_exclude.data(), exclude, (size_t)n_exclude * sizeof(int32_t)) == 0);
     int64_t max_id = 0;
-    for (int64_t q = 0; q < n; ++q) max_id = std::max<int64_t>(max_id, id[q]);
-    std::vector<int32_t> idmap(max_id + 1, -1);
-    h->h_owner.assign(n, 0);
-    h->h_ids.assign(id, id + n);
-    std::vector<int32_t> pid(n), excl(n, 0);
-    for (int64_t q = 0; q < n; ++q) {
-        if (idmap[id[q]] < 0) idmap[id[q]] = (int32_t)q;
-        h->h_owner[q] = idmap[id[q]];
-        pid[q] = (int32_t)id[q];
-        if (type)
-            for (int t = 0; t < n_exclude; ++t)
-                if (type[q] == exclude[t]) { excl[q] = 1; break; }
-    }
-    if (max_id + 1 > h->id_map_size) {
-        dfree(h->id_map);
-        PSM_HIP(h, dalloc(&h->id_map, (max_id + 1) * sizeof(int32_t)));
-        h->id_map_size = max_id + 1;
-    }
+    if (!same_ids)
+        for (int64_t q = 0; q < n; ++q) max_id = std::max<int64_t>(max_id, id[q]);
+    const size_t nd = 10 * (size_t)n;                      /* x, v, w (3 n each), r (n) */
+    const size_t ni = same_ids ? 0 : 3 * (size_t)n + (size_t)(max_id + 1);   /* pid, excl, owner, id map */
+    int sk = 0;
+    rc = stage_acquire(h, std::max<size_t>(8, nd * sizeof(double) + ni * sizeof(int32_t)), &sk);
+    if (rc) return rc;
+    double* sd = (double*)h->stage[sk].buf;
+    std::memcpy(sd, x3, 3 * (size_t)n * sizeof(double));
+    std::memcpy(sd + 3 * n, v3, 3 * (size_t)n * sizeof(double));
+    std::memcpy(sd + 6 * n, w3, 3 * (size_t)n * sizeof(double));
+    std::memcpy(sd + 9 * n, r, (size_t)n * sizeof(double));
     if (n > 0) {
-        PSM_HIP(h, hipMemcpy(h->px, x3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(h->pv, v3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(h->pw, w3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(h->pr, r, n * sizeof(double), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(h->pid, pid.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(h->pexcl, excl.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
-        PSM_HIP(h, hipMemcpy(h->powner, h->h_owner.data(), n * sizeof(int32_t), hipMemcpyHostToDevice));
+        PSM_HIP(h, hipMemcpyAsync(h->px, sd, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        PSM_HIP(h, hipMemcpyAsync(h->pv, sd + 3 * n, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        PSM_HIP(h, hipMemcpyAsync(h->pw, sd + 6 * n, 3 * n * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        PSM_HIP(h, hipMemcpyAsync(h->pr, sd + 9 * n, n * sizeof(double), hipMemcpyHostToDevice, h->stream));
     }
-    PSM_HIP(h, hipMemset(h->id_map, 0xff, h->id_map_size * sizeof(int32_t)));
-    PSM_HIP(h, hipMemcpy(h->id_map, idmap.data(), (max_id + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!same_ids) {
+        if (max_id + 1 > h->id_map_size) {
+            PSM_HIP(h, hipStreamSynchronize(h->stream));
+            dfree(h->id_map);
+            h->id_map = nullptr;
+            h->id_map_size = 0;
+            PSM_HIP(h, dalloc(&h->id_map, (max_id + 1) * sizeof(int32_t)));
+            h->id_map_size = max_id + 1;
+        }
+        int32_t* pid = (int32_t*)(sd + nd);
+        int32_t* excl = pid + n;
+        int32_t* owner = excl + n;
+        int32_t* idmap = owner + n;
+        std::fill(idmap, idmap + (max_id + 1), -1);
+        /* map(tag): the first index holding each id (owned atoms precede ghosts) */
+        h->h_owner.assign(n, 0);
+        for (int64_t q = 0; q < n; ++q) {
+            if (idmap[id[q]] < 0) idmap[id[q]] = (int32_t)q;
+            owner[q] = idmap[id[q]];
+            h->h_owner[q] = owner[q];
+            pid[q] = (int32_t)id[q];
+            excl[q] = 0;
+            if (type)
+                for (int t = 0; t < n_exclude; ++t)
+                    if (type[q] == exclude[t]) { excl[q] = 1; break; }
+        }
+        if (n > 0) {
+            PSM_HIP(h, hipMemcpyAsync(h->pid, pid, n * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+            PSM_HIP(h, hipMemcpyAsync(h->pexcl, excl, n * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+            PSM_HIP(h, hipMemcpyAsync(h->powner, owner, n * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+        }
+        PSM_HIP(h, hipMemsetAsync(h->id_map, 0xff, h->id_map_size * sizeof(int32_t), h->stream));
+        PSM_HIP(h, hipMemcpyAsync(h->id_map, idmap, (max_id + 1) * sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+        h->h_ids.assign(id, id + n);
+        h->h_has_types = type != nullptr;
+        if (type) h->h_types.assign(type, type + n);
+        else h->h_types.clear();
+        h->h_exclude.assign(exclude, exclude + n_exclude);
+        h->ids_valid = true;
+    }
+    rc = stage_release(h, sk);
+    if (rc) return rc;
     h->np = n;
     h->flags = flags;
     h->have_particles = true;
@@ -493,9 +593,12 @@ int lbdem_psm_set_spheres(lbdem_psm* h, int64_t n, const double* x3, const doubl
     PSM_HIP(h, hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream));
     rc = run_voxelize(h);
     if (rc) return rc;
-    rc = check_overflow(h);
-    if (rc) return rc;
     h->voxelized = true;
+    if (h->max_records > 0) return check_overflow(h);
+    /* the capacity flags are checked by the next call that waits for the
+     * device (sum_force_torque, get_particle_forces, step, synchronize) */
+    PSM_HIP(h, hipMemcpyAsync(h->h_counters, h->counters, 8 * sizeof(int64_t), hipMemcpyDeviceToHost, h->stream));
+    h->pending_counters = true;
     return LBDEM_PSM_OK;
 }
 
@@ -512,10 +615,20 @@ int lbdem_psm_sum_force_torque(lbdem_psm* h, int64_t n, const double* x3, double
     if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
     if (!h->have_particles) return set_err(h, LBDEM_PSM_ERR_STATE, "sum_force_torque before set_spheres");
     if (n != h->np) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "particle count differs from set_spheres");
-    PSM_HIP(h, hipStreamSynchronize(h->stream));
     if (n > 0) {
-        if (x3) PSM_HIP(h, hipMemcpy(h->psum_x, x3, 3 * n * sizeof(double), hipMemcpyHostToDevice));
-        else PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->px, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        if (x3) {
+            /* through pinned staging, queued behind the collision */
+            int sk = 0;
+            int rc0 = stage_acquire(h, 3 * (size_t)n * sizeof(double), &sk);
+            if (rc0) return rc0;
+            std::memcpy(h->stage[sk].buf, x3, 3 * (size_t)n * sizeof(double));
+            PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->stage[sk].buf, 3 * n * sizeof(double), hipMemcpyHostToDevice,
+                                      h->stream));
+            rc0 = stage_release(h, sk);
+            if (rc0) return rc0;
+        } else {
+            PSM_HIP(h, hipMemcpyAsync(h->psum_x, h->px, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        }
     }
     int rc = run_force_sum(h, h->psum_x);
     if (rc) return rc;
@@ -545,6 +658,7 @@ int lbdem_psm_step(lbdem_psm* h, int32_t nsteps, int32_t flags) {
         rc = run_force_sum(h, h->px);
         if (rc) return rc;
     }
+    h->pending_counters = false;
     return check_overflow(h);
 }
 
@@ -735,7 +849,7 @@ int lbdem_psm_stats(lbdem_psm* h, int64_t* out4) {
 int lbdem_psm_synchronize(lbdem_psm* h) {
     if (!h) return LBDEM_PSM_ERR_INVALID_ARG;
     PSM_HIP(h, hipStreamSynchronize(h->stream));
-    return LBDEM_PSM_OK;
+    return pending_status(h);
 }
 
 int lbdem_psm_profile(lbdem_psm* h, int32_t enable) {

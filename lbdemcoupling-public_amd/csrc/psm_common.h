@@ -444,6 +444,18 @@ struct lbdem_psm {
     int64_t id_map_size = 0;
     std::vector<int32_t> h_owner;    /* host copy of powner */
     std::vector<int64_t> h_ids;
+    /* the mapping inputs the device's pid / pexcl / powner / id_map were built
+     * from (reused while a set_spheres call passes the same ones) */
+    std::vector<int32_t> h_types, h_exclude;
+    bool h_has_types = false;
+    bool ids_valid = false;
+    /* pinned host staging of the exchange calls: two buffers used in turn,
+     * each free again once the event recorded after its copies completes */
+    struct Stage { void* buf = nullptr; size_t bytes = 0; hipEvent_t ev = nullptr; bool inflight = false; } stage[2];
+    int stage_next = 0;
+    double* h_io = nullptr;          /* pinned: sum_force_torque positions in, forces out */
+    size_t h_io_bytes = 0;
+    bool pending_counters = false;   /* h_counters holds an unchecked copy (set_spheres) */
     int32_t flags = 0;
     bool have_particles = false;
     bool voxelized = false;
