@@ -121,7 +121,7 @@ def test_fixed_sphere_in_periodic_box_stokes_drag():
 
 
 def test_stokes_drag_converges_with_resolution():
-    """The same volume fraction (c = 0.0639) at a = 5, 7.5 and 10 cells: the
+    """The same volume fraction (c = 0.016) at a = 5, 7.5 and 10 cells: the
     drag against Hasimoto's periodic-array Stokes law moves toward 1 as the
     sphere is resolved (partially-saturated cells give a hydrodynamic radius
     below the geometric one, by a margin that shrinks with resolution), and
