@@ -164,6 +164,10 @@ static CollideArgs make_args(lbdem_psm* h) {
     for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
     a.sflag = h->sflag;
     a.rsf = h->rec.sf; a.rus = h->rec.us; a.rF = h->rec.F; a.rcap = h->rec_cap;
+    a.div_nzc = make_fastdiv((uint32_t)h->g.L.nzc);
+    a.sub_shift = 0;
+    while ((collide_wg(h->g.L.zc) << a.sub_shift) < h->g.L.zc) ++a.sub_shift;
+    a.div_ny = make_fastdiv((uint32_t)h->g.n[1]);
     a.rm4 = h->rec.m4;
     return a;
 }
@@ -173,7 +177,8 @@ void launch_cs_prec<LBDEM_PSM_FP64>(lbdem_psm* h, const CollideArgs& a, const vo
     typedef typename Store<LBDEM_PSM_FP64>::S S;
     const S* fin = (const S*)vin;
     S* fout = (S*)vout;
-    dim3 grid((unsigned)((int64_t)a.nx * a.ny * a.L.nzc)), block((unsigned)a.L.zc);
+    const int wgz = collide_wg(a.L.zc);
+    dim3 grid((unsigned)((int64_t)a.nx * a.ny * a.L.nzc * (a.L.zc / wgz))), block((unsigned)wgz);
     if (h->has_force) {
         if (a.pz) hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, true>), grid, block, 0, h->stream, fin, fout, a);
         else hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, false>), grid, block, 0, h->stream, fin, fout, a);

@@ -29,7 +29,16 @@ struct CollideArgs {
     void* rF;        /* per-cell force: double for FP64 storage, float otherwise */
     int64_t rcap;
     const float* rm4;  /* float view of the records (FP32/FP16 storage) */
+    FastDiv div_nzc, div_ny;   /* block -> (row, chunk), row -> (x, y) */
+    int sub_shift;             /* workgroups per (row, chunk) block: 1 << sub_shift */
 };
+
+#ifndef COLLIDE_WG
+#define COLLIDE_WG 256
+#endif
+/* Lanes per collide workgroup: a (row, chunk) block of zc cells is
+ * zc / COLLIDE_WG workgroups (one when zc is smaller). */
+inline int collide_wg(int zc) { return zc < COLLIDE_WG ? zc : COLLIDE_WG; }
 
 /* Equilibria of a population pair (i, i+9): c_{i+9} = -c_i, so the pair
  * shares c.j, (3 c.j)(c.j) - |j|^2 and its product with 1.5/rho; each value
@@ -295,15 +304,16 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     typedef typename Store<PREC>::T T;
     typedef typename Store<PREC>::S S;
     const int nzc = a.L.nzc;
-    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
-    const int row = blk / nzc;
+    const int wg = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int blk = wg >> a.sub_shift;
+    const int row = (int)fdiv((uint32_t)blk, a.div_nzc);
     const int chunk = blk - row * nzc;
-    const int zl = threadIdx.x;
+    const int zl = (wg - (blk << a.sub_shift)) * (int)blockDim.x + (int)threadIdx.x;
     const int z = chunk * a.L.zc + zl;
     if (z >= a.nz) return;
     /* x, y are wave-uniform: keep them (and the row-block addresses derived
      * from them) in scalar registers */
-    const int x = __builtin_amdgcn_readfirstlane(row / a.ny);
+    const int x = __builtin_amdgcn_readfirstlane((int)fdiv((uint32_t)row, a.div_ny));
     const int y = __builtin_amdgcn_readfirstlane(row - x * a.ny);
     const T omega = T(a.omega);
     const T acc[3] = {T(a.acc[0]), T(a.acc[1]), T(a.acc[2])};

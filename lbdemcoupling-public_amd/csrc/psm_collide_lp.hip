@@ -13,7 +13,8 @@ static void launch_lp(lbdem_psm* h, const CollideArgs& a, const void* vin, void*
     typedef typename Store<PREC>::S S;
     const S* fin = (const S*)vin;
     S* fout = (S*)vout;
-    dim3 grid((unsigned)((int64_t)a.nx * a.ny * a.L.nzc)), block((unsigned)a.L.zc);
+    const int wgz = collide_wg(a.L.zc);
+    dim3 grid((unsigned)((int64_t)a.nx * a.ny * a.L.nzc * (a.L.zc / wgz))), block((unsigned)wgz);
     if (h->has_force) {
         if (a.pz) hipLaunchKernelGGL((k_collide_stream<PREC, true, true>), grid, block, 0, h->stream, fin, fout, a);
         else hipLaunchKernelGGL((k_collide_stream<PREC, true, false>), grid, block, 0, h->stream, fin, fout, a);

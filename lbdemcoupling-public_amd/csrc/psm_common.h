@@ -327,6 +327,32 @@ __device__ __forceinline__ int xcd_block(int i, int G) {
     return xcd * per + (xcd < rem ? xcd : rem) + local;
 }
 
+/* Division by a run-time constant d > 0 for dividends n < 2^31, as a
+ * multiply-high, an add and a shift (Granlund and Montgomery, "Division by
+ * invariant integers using multiplication", PLDI 1994): with
+ * l = ceil(log2 d) and m = floor(2^32 (2^l - d) / d) + 1,
+ * n / d = (mulhi(m, n) + n) >> l.  mulhi(m, n) < n, so the sum fits 32 bits
+ * for n < 2^31.  The kernels' block and bin index splits used a general
+ * division here (a software sequence of ~30 scalar instructions each). */
+struct FastDiv {
+    uint32_t d, m, l;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while (l < 32 && (1ull << l) < (uint64_t)d) ++l;
+    f.l = l;
+    f.m = (uint32_t)((((uint64_t)1 << 32) * (((uint64_t)1 << l) - d)) / d + 1);
+    return f;
+}
+
+__host__ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
+    const uint32_t t = (uint32_t)(((uint64_t)n * f.m) >> 32);   /* a multiply-high on the device */
+    return (t + n) >> f.l;
+}
+
 /* Broadcast of lane L's double (L wave-uniform) into scalar registers. */
 __device__ __forceinline__ double readlane_d(double v, int L) {
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);

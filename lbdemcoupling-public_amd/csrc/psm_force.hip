@@ -27,6 +27,7 @@ namespace psm {
 struct ForceArgs {
     int n[3];
     int nb[3];
+    FastDiv div_nb2, div_nb12;   /* bin -> (bx, by, bz) */
     const int32_t* bin_cand_start;
     const int32_t* bin_cand_count;
     const uint32_t* key;
@@ -48,22 +49,29 @@ struct ForceArgs {
  * per lane, overlapping the cell loads.  Then, entry by entry, each lane
  * adds F and d x F of its cells set by that entry (in x order) and the
  * wavefront reduces the six sums with wave_sum8_scatter. */
+/* bins (one wavefront each) per workgroup: one, so that a bin's wavefront
+ * is scheduled on its own (measured faster than 2 or 4 per workgroup) */
+#ifndef FORCE_WAVES
+#define FORCE_WAVES 1
+#endif
+
 template <typename FT>
-__global__ __launch_bounds__(256) void k_force_bin(ForceArgs a) {
+__global__ __launch_bounds__(64 * FORCE_WAVES) void k_force_bin(ForceArgs a) {
     const int lane = (int)(threadIdx.x & 63);
     /* neighbouring bins (z fastest) share the lines of the cell-ordered
      * arrays: keep them in one workgroup and on one XCD's L2 */
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const int64_t b = (int64_t)xcd_block((int)blockIdx.x, (int)gridDim.x) * 4 + wv;
+    const int64_t b = (int64_t)xcd_block((int)blockIdx.x, (int)gridDim.x) * FORCE_WAVES + wv;
     if (b >= a.nbins) return;
     const int ncand = a.bin_cand_count[b];
     if (ncand == 0) return;
     const int cstart = a.bin_cand_start[b];
     if ((int64_t)cstart + ncand > a.ent_cap) return;
     const int bi = (int)b;
-    const int bz = bi % a.nb[2];
-    const int by = (bi / a.nb[2]) % a.nb[1];
-    const int bx = bi / (a.nb[2] * a.nb[1]);
+    const int bxy = (int)fdiv((uint32_t)bi, a.div_nb2);
+    const int bz = bi - bxy * a.nb[2];
+    const int bx = (int)fdiv((uint32_t)bi, a.div_nb12);
+    const int by = bxy - bx * a.nb[1];
     const int Y = by * BIN + (lane >> 3), Z = bz * BIN + (lane & 7);
     const bool yz = Y < a.n[1] && Z < a.n[2];
     /* Three rounds of loads, each issued at once: the eight liveness
@@ -312,6 +320,8 @@ int run_force_sum(lbdem_psm* h, const double* xs, double* out) {
     prof_begin(h, 2, &ev);
     ForceArgs a;
     for (int d = 0; d < 3; ++d) { a.n[d] = h->g.n[d]; a.nb[d] = h->g.nb[d]; }
+    a.div_nb2 = make_fastdiv((uint32_t)h->g.nb[2]);
+    a.div_nb12 = make_fastdiv((uint32_t)(h->g.nb[2] * h->g.nb[1]));
     a.bin_cand_start = h->bin_cand_start;
     a.bin_cand_count = h->bin_cand_count;
     a.key = h->cand_key;
@@ -324,9 +334,9 @@ int run_force_sum(lbdem_psm* h, const double* xs, double* out) {
     a.partial = h->partial;
     a.ent_cap = h->ent_cap;
     a.nbins = h->nbins;
-    const dim3 grid((unsigned)((h->nbins + 3) / 4));
-    if (a.f32) hipLaunchKernelGGL(k_force_bin<float>, grid, dim3(256), 0, h->stream, a);
-    else hipLaunchKernelGGL(k_force_bin<double>, grid, dim3(256), 0, h->stream, a);
+    const dim3 grid((unsigned)((h->nbins + FORCE_WAVES - 1) / FORCE_WAVES));
+    if (a.f32) hipLaunchKernelGGL(k_force_bin<float>, grid, dim3(64 * FORCE_WAVES), 0, h->stream, a);
+    else hipLaunchKernelGGL(k_force_bin<double>, grid, dim3(64 * FORCE_WAVES), 0, h->stream, a);
     if (h->np > 0) {
         hipLaunchKernelGGL(k_force_particle, dim3((unsigned)((h->np + 3) / 4)), dim3(256), 0, h->stream,
                            h->np, h->p_ent_start, h->p_nent, h->owner_ent, h->partial, pft, h->ent_cap);

@@ -30,6 +30,16 @@
 
 namespace psm {
 
+#ifndef FOLD_WAVES
+#define FOLD_WAVES 1
+#endif
+/* wavefronts (x-slices of a bin) per fold workgroup: a bin is 8 / FOLD_WAVES
+ * workgroups.  One: the slices of a bin carry very different amounts of
+ * counting work, and a workgroup holds its slots until its slowest wavefront
+ * ends; single-wavefront workgroups took the fold from 5.0 to 4.2 ms at
+ * 512x512x768 (8 per workgroup before) */
+constexpr int FOLD_SPLIT = 8 / FOLD_WAVES;
+
 struct VoxArgs {
     int n[3];
     int per[3];
@@ -279,6 +289,7 @@ __global__ __launch_bounds__(256) void k_cand_gather(VoxArgs a, const uint32_t* 
 struct FoldArgs {
     int n[3];
     int nb[3];
+    FastDiv div_nb2, div_nb12;   /* bin -> (bx, by, bz) */
     int N;
     double sw, frac;
     double rc, vc;             /* cell sphere of the analytic overlap */
@@ -345,30 +356,36 @@ __device__ __forceinline__ unsigned wave_count(double dx, double dy, double dz, 
 
 /* Sub-cell counts of the pending lanes of one fold wavefront (N = 8).  The
  * wavefront is one x-slice of a bin with lane = 8 y + z, so the offset dx
- * and the radius are the same on every lane, and lanes of one z-row share
- * dz bit for bit.  The pending lanes are taken z-row by z-row: the row's
- * eight dz_sq[k] are read once (wave-uniform); each query then forms
- * dx_sq[i] + dy_sq[j] on lane 8 i + j and tests the eight k.  Every sample
- * test is the same FP64 expression as wave_count's,
- * (dx_sq[i] + dy_sq[j]) + dz_sq[k] < r^2, so the counts are the same. */
+ * and the radius are the same on every lane, lanes of one z-row share dz
+ * bit for bit and lanes of one y-row share dy.  Two per-wavefront LDS
+ * tables hold the squared sample offsets: zsq[8 r + k] = (dz_r + d_k)^2 for
+ * z-row r, ysq[8 y + j] = (dy_y + d_j)^2 for y-row y (lane 8 y + j forms
+ * its own entry: its dy and its d_j).  The pending lanes are taken z-row by
+ * z-row: the row's eight zsq[k] are read once (wave-uniform); each query
+ * then has lane 8 i + j read ysq of the query's y-row, form
+ * dx_sq[i] + dy_sq[j] and test the eight k.  Every sample test is the same
+ * FP64 expression as wave_count's, (dx_sq[i] + dy_sq[j]) + dz_sq[k] < r^2,
+ * so the counts are the same.  A query's count goes into the querying
+ * lane by a select (no branch); the fractions follow after. */
 __device__ __forceinline__ void slice_counts8(unsigned long long pend, double dx, double dy, double dz,
                                               double rsq, double sw, double frac, double di, double dj, int& n,
                                               double& sf) {
-    /* dz_sq[k] of every z-row of the slice, one value per lane (lane
-     * 8 r + k), in LDS: a z-row then reads its eight values on the LDS pipe
-     * instead of forming them with 16 FP64 instructions.  One region per
-     * wavefront of the 512-lane fold workgroup; the wavefront's own LDS
-     * accesses stay in program order. */
-    __shared__ double ztab[8][64];
+    __shared__ double ztab[FOLD_WAVES][64];
+    __shared__ double ytab[FOLD_WAVES][64];
     const int lane = (int)__lane_id();
     double* zt = ztab[threadIdx.x >> 6];
+    double* yt = ytab[threadIdx.x >> 6];
     {
         const double dzr = __shfl(dz, lane >> 3, 64);   /* dz of z-row lane / 8 */
         const double az = dzr + dj;
         zt[lane] = az * az;
+        const double ay = dy + dj;                      /* lane 8 y + j: its own dy and d_j */
+        yt[lane] = ay * ay;
     }
     const double ax = readlane_d(dx, __ffsll(pend) - 1) + di;
     const double xsq = ax * ax;
+    const unsigned long long counted = pend;
+    int cnt_lane = n;
     while (pend) {
         const int L = __ffsll(pend) - 1;
         const unsigned long long row = pend & (0x0101010101010101ull << (L & 7));
@@ -380,14 +397,17 @@ __device__ __forceinline__ void slice_counts8(unsigned long long pend, double dx
         unsigned long long q = row;
         while (q) {
             const int M = __ffsll(q) - 1;
-            q &= q - 1;
-            const double ay = readlane_d(dy, M) + dj;
-            const double sxy = xsq + ay * ay;
+            q ^= 1ull << M;
+            const double sxy = xsq + yt[(M & 0x38) + (lane & 7)];
             unsigned cnt = 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) cnt += (unsigned)__popcll(__ballot(sxy + zsq[k] < rsq));
-            if (lane == M) { n = (int)cnt; sf = frac * (double)cnt; }
+            cnt_lane = lane == M ? (int)cnt : cnt_lane;   /* a select, not a branch */
         }
+    }
+    if ((counted >> lane) & 1ull) {
+        n = cnt_lane;
+        sf = frac * (double)cnt_lane;
     }
 }
 
@@ -485,18 +505,20 @@ enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
  * of SetSingleSphere3D::process (src/ibProcessors3D.hh:85-105) and written
  * back in place. */
 template <int NT, bool INITVEL>
-__global__ __launch_bounds__(512) void k_fold(FoldArgs a) {
+__global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
     /* neighbouring bins (z fastest) share the lines of the cell-ordered
      * arrays: keep them on one XCD's L2 */
-    const int64_t b = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int blk = xcd_block((int)blockIdx.x, (int)gridDim.x);
+    const int b = blk / FOLD_SPLIT;
     const int ncand = a.bin_cand_count[b];
     if (ncand == 0 && a.cnt_old[b] == 0u) return;   /* nothing here, nothing was here */
     const int cstart = a.bin_cand_start[b];
     if ((int64_t)cstart + ncand > a.ent_cap) return;   /* overflow already flagged */
-    const int bz = (int)(b % a.nb[2]);
-    const int by = (int)((b / a.nb[2]) % a.nb[1]);
-    const int bx = (int)(b / ((int64_t)a.nb[2] * a.nb[1]));
-    const int l = threadIdx.x;
+    const int bxy = (int)fdiv((uint32_t)b, a.div_nb2);
+    const int bz = b - bxy * a.nb[2];
+    const int bx = (int)fdiv((uint32_t)b, a.div_nb12);
+    const int by = bxy - bx * a.nb[1];
+    const int l = (int)threadIdx.x + (blk % FOLD_SPLIT) * (64 * FOLD_WAVES);   /* cell of the bin */
     const int X = bx * BIN + (l >> 6), Y = by * BIN + ((l >> 3) & 7), Z = bz * BIN + (l & 7);
     const bool valid = X < a.n[0] && Y < a.n[1] && Z < a.n[2];
     const int64_t cidx = valid ? ((int64_t)X * a.n[1] + Y) * a.n[2] + Z : 0;
@@ -731,6 +753,8 @@ int run_voxelize(lbdem_psm* h) {
 
     FoldArgs fa;
     for (int d = 0; d < 3; ++d) { fa.n[d] = h->g.n[d]; fa.nb[d] = h->g.nb[d]; }
+    fa.div_nb2 = make_fastdiv((uint32_t)h->g.nb[2]);
+    fa.div_nb12 = make_fastdiv((uint32_t)(h->g.nb[2] * h->g.nb[1]));
     fa.N = h->desc.slices_per_dim;
     fa.sw = 1.0 / (double)fa.N;
     fa.frac = 1.0 / (double)(fa.N * fa.N * fa.N);
@@ -752,7 +776,7 @@ int run_voxelize(lbdem_psm* h) {
     fa.orphan_cap = h->orphan_cap;
     fa.dv_cell = init_vel ? h->dv_cell : nullptr;
     fa.dv_us = h->dv_us;
-    dim3 grid((unsigned)nbins), block(512);
+    dim3 grid((unsigned)(nbins * FOLD_SPLIT)), block(64 * FOLD_WAVES);
     if (overlap) {
         if (init_vel) hipLaunchKernelGGL((k_fold<-1, true>), grid, block, 0, h->stream, fa);
         else hipLaunchKernelGGL((k_fold<-1, false>), grid, block, 0, h->stream, fa);
