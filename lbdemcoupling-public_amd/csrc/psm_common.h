@@ -278,18 +278,20 @@ __host__ __device__ inline void record_mirror(double sf, const double* us, float
     out4[3] = (float)us[2];
 }
 
-/* A candidate particle image of one bin, gathered in key order. */
+/* A candidate particle image of one bin, gathered in key order (entry e of
+ * the candidate array).  The first 80 bytes are what a fold wavefront reads
+ * for every candidate it tests, one contiguous block of scalar loads; the
+ * velocities are read once per cell the candidate sets. */
 struct Cand {
     double x[3];     /* image centre */
-    double r;
-    double v[3];
-    double w[3];
     double rp2, rm2, rsq;
     double ri2, ro2;    /* squared radii of the exact no-count band (sub-cell rule) */
     uint64_t yzmask;    /* clipped bounding box within the entry's bin: bit 8 y + z */
     uint32_t xmask;     /* ... and bit x (bin-local coordinates) */
     int id;
-    int e;              /* entry index */
+    double r;
+    double v[3];
+    double w[3];
 };
 
 /* Record slot of the cell (x, y, z): the cell index itself, so that a

@@ -92,7 +92,7 @@ __device__ __forceinline__ ImageRanges lane_image(const VoxArgs& a, int64_t q, b
         double xi[3];
         ok = image_box(x, a.pr[q], lane, a, lo, hi, xi);
     }
-    ir.mask = __ballot(ok);
+    ir.mask = __builtin_amdgcn_ballot_w64(ok);
     ir.b0 = lo[0] / BIN; ir.c0 = lo[1] / BIN; ir.d0 = lo[2] / BIN;
     ir.nyr = ok ? hi[1] / BIN - ir.c0 + 1 : 0;
     ir.nzr = ok ? hi[2] / BIN - ir.d0 + 1 : 0;
@@ -281,7 +281,6 @@ __global__ __launch_bounds__(256) void k_cand_gather(VoxArgs a, const uint32_t* 
         cd.rsq = r * r;
         exact_band(r, esamp, cd.ri2, cd.ro2);
         cd.id = pid[q];
-        cd.e = (int)e;
         out[e] = cd;
     }
 }
@@ -336,18 +335,18 @@ __device__ __forceinline__ unsigned wave_count(double dx, double dy, double dz, 
 #pragma unroll
             for (int k = 0; k < (NT > 0 ? NT : 1); ++k) {
                 const double az = dz + (-0.5 + ((double)k + 0.5) * sw);
-                total += (unsigned)__popcll(__ballot(sxy + az * az < rsq));
+                total += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(sxy + az * az < rsq));
             }
         } else if (NT > 0) {
 #pragma unroll
             for (int k = 0; k < (NT > 0 ? NT : 1); ++k) {
                 const double az = dz + (-0.5 + ((double)k + 0.5) * sw);
-                total += (unsigned)__popcll(__ballot(act && (sxy + az * az < rsq)));
+                total += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(act && (sxy + az * az < rsq)));
             }
         } else {
             for (int k = 0; k < N; ++k) {
                 const double az = dz + (-0.5 + ((double)k + 0.5) * sw);
-                total += (unsigned)__popcll(__ballot(act && (sxy + az * az < rsq)));
+                total += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(act && (sxy + az * az < rsq)));
             }
         }
     }
@@ -401,7 +400,7 @@ __device__ __forceinline__ void slice_counts8(unsigned long long pend, double dx
             const double sxy = xsq + yt[(M & 0x38) + (lane & 7)];
             unsigned cnt = 0;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) cnt += (unsigned)__popcll(__ballot(sxy + zsq[k] < rsq));
+            for (int k = 0; k < 8; ++k) cnt += (unsigned)__popcll(__builtin_amdgcn_ballot_w64(sxy + zsq[k] < rsq));
             cnt_lane = lane == M ? (int)cnt : cnt_lane;   /* a select, not a branch */
         }
     }
@@ -446,7 +445,7 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
     bool need = band && !allin && !(dist > ro2);
     n = (active && core) || allin ? N3 : 0;
     double sf = active && core ? 1.0 : (allin ? frac * (double)N3 : 0.0);
-    if (__ballot(need) != 0ull) {
+    if (__builtin_amdgcn_ballot_w64(need) != 0ull) {
         const double s = -0.5 + ((double)(N - 1) + 0.5) * sw;   /* outermost sample offset */
         const double tol = 1e-9 * (1.0 + rsq);
         const double ex = fabs(dx), ey = fabs(dy), ez = fabs(dz);
@@ -458,7 +457,7 @@ __device__ __forceinline__ double coop_solid_fraction(bool active, double dx, do
         sf = hin ? frac * (double)N3 : sf;
         need = need && !hin && !hout;
     }
-    unsigned long long pend = __ballot(need);
+    unsigned long long pend = __builtin_amdgcn_ballot_w64(need);
     if constexpr (SLICE) {
         static_assert(NT == 8, "one sample column per lane");
         if (pend) slice_counts8(pend, dx, dy, dz, rsq, sw, frac, di, dj, n, sf);
@@ -497,7 +496,7 @@ __device__ __forceinline__ double solid_fraction(bool active, double dx, double 
     }
 }
 
-enum { ST_ZERO = 0, ST_OLD = 1, ST_SET = 2 };
+enum { ST_NONE = -2, ST_KEPT = -1, ST_SET_FROM = 0 };
 
 /* One workgroup of 512 lanes per 8^3 bin, one cell per lane (a wavefront
  * is one x-slice of the bin).  The cell's record of the previous step is
@@ -527,7 +526,10 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
     const int wl = __builtin_amdgcn_readfirstlane(l >> 6);
 
     const bool had = valid && a.sflag[cidx] != 0;
-    int kind = had ? ST_OLD : ST_ZERO, sid = 0, se = -1, sj = -1;
+    /* The record's state through the candidate loop: ST_SET_FROM + j when
+     * candidate j set it last, ST_KEPT when it is the previous step's record
+     * untouched, ST_NONE when there is none (never was or was reset). */
+    int sj = had ? ST_KEPT : ST_NONE, sid = 0;
     double ssf = 0.0, sus[3] = {0.0, 0.0, 0.0};
     bool dv = false;
     double dvus[3] = {0.0, 0.0, 0.0};
@@ -541,20 +543,28 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
     }
     const double osf = ssf;
     const int oid = sid;
+    const double fX = (double)X, fY = (double)Y, fZ = (double)Z;
 
     /* candidates in key order; their data are wave-uniform */
     const Cand* __restrict__ cands = a.cand + cstart;
-    {
-        for (int j = 0; j < ncand; ++j) {
+    const int lane = (int)__lane_id();
+    for (int c0 = 0; c0 < ncand; c0 += 64) {
+        /* whole-wavefront rejection against the candidates' boxes (their
+         * x-slices within the bin), 64 candidates at a time: lane c reads
+         * candidate c0 + c's x mask, and the ballot lists the candidates
+         * this x-slice has to take, in key order */
+        const int cl = c0 + lane < ncand ? c0 + lane : ncand - 1;
+        unsigned long long todo = __builtin_amdgcn_ballot_w64(c0 + lane < ncand && ((cands[cl].xmask >> wl) & 1u) != 0u);
+        while (todo) {
+            const int j = c0 + (__ffsll(todo) - 1);
+            todo &= todo - 1;
             const Cand& cd = cands[j];
-            /* whole-wavefront rejection against the candidate's box (its
-             * x-slices within the bin), then the lane's cell against its
-             * (y, z) cells within the bin */
-            if (((cd.xmask >> wl) & 1u) == 0u) continue;
-            const bool inb = valid & (((cd.yzmask >> (l & 63)) & 1ull) != 0ull);
-            const double dx = (double)X - cd.x[0];
-            const double dy = (double)Y - cd.x[1];
-            const double dz = (double)Z - cd.x[2];
+            /* the lane's cell against the candidate's (y, z) cells within
+             * the bin */
+            const bool inb = valid & (((cd.yzmask >> lane) & 1ull) != 0ull);
+            const double dx = fX - cd.x[0];
+            const double dy = fY - cd.x[1];
+            const double dz = fZ - cd.x[2];
             /* Nothing changes for a lane outside the reach r + h of the
              * sphere (its solid fraction is 0) unless it holds a record of
              * this particle (the particle has left the cell, case 2 of the
@@ -565,7 +575,7 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
                  * with its centre, not behind a branch each */
                 const bool near = inb & (dx * dx + dy * dy + dz * dz <= cd.rp2);
                 const bool mine = inb & (ssf > SOLFRAC_MIN) & (sid == cd.id);
-                if (__ballot(near || mine) == 0ull) continue;
+                if (__builtin_amdgcn_ballot_w64(near | mine) == 0ull) continue;
             }
             int nn, br;
             const double sf = solid_fraction<NT, NT == 8>(inb, dx, dy, dz, cd.rp2, cd.rm2, cd.rsq, cd.r, cd.ri2, cd.ro2,
@@ -574,13 +584,11 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
              * 0 -> zero, 1 -> set, 2 -> zero if own record, 3 -> set if
              * larger or own record */
             const bool pnew = sf > SOLFRAC_MIN, pold = ssf > SOLFRAC_MIN, own = sid == cd.id;
-            const bool zero = inb && !pnew && (!pold || own);
-            const bool set = inb && pnew && (!pold || sf > ssf || own);
-            kind = set ? ST_SET : (zero ? ST_ZERO : kind);
+            const bool zero = inb & !pnew & (!pold | own);
+            const bool set = inb & pnew & (!pold | (sf > ssf) | own);
+            sj = set ? ST_SET_FROM + j : (zero ? ST_NONE : sj);
             ssf = set ? sf : (zero ? 0.0 : ssf);
             sid = set ? cd.id : (zero ? 0 : sid);
-            se = set ? cd.e : (zero ? -1 : se);
-            sj = set ? j : (zero ? -1 : sj);
             if constexpr (INITVEL) {
                 /* setValues, src/ibProcessors3D.hh:162-174, here because
                  * initVel reads the velocity of the record in force */
@@ -593,7 +601,7 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
             }
             if (INITVEL && inb && sf > SOLFRAC_MAX) {
                 dv = true;
-                if (kind == ST_OLD) {
+                if (sj == ST_KEPT) {
                     dvus[0] = a.rec.us[ridx];
                     dvus[1] = a.rec.us[a.rcap + ridx];
                     dvus[2] = a.rec.us[2 * a.rcap + ridx];
@@ -604,21 +612,22 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
         }
     }
 
-    const bool keep = valid && kind != ST_ZERO;
-    const unsigned long long m = __ballot(keep);
+    const bool keep = valid && sj != ST_NONE;
+    const unsigned long long m = __builtin_amdgcn_ballot_w64(keep);
     /* one atomic per wavefront on the bin's own counter (distinct addresses
      * across bins; a single global counter would serialize every wave) */
-    if (__lane_id() == 0 && m) atomicAdd(&a.cnt_new[b], (uint32_t)__popcll(m));
+    if (lane == 0 && m) atomicAdd(&a.cnt_new[b], (uint32_t)__popcll(m));
     if (!valid) return;
-    if (kind == ST_SET) {
+    if (sj >= ST_SET_FROM) {
+        const int js = sj - ST_SET_FROM;
         if constexpr (!INITVEL) {
             /* setValues (src/ibProcessors3D.hh:162-174) of the candidate
              * that set the record last, once per cell: the same FP64
              * expressions as in the candidate loop */
-            const Cand& cw = cands[sj];
-            const double dx = (double)X - cw.x[0];
-            const double dy = (double)Y - cw.x[1];
-            const double dz = (double)Z - cw.x[2];
+            const Cand& cw = cands[js];
+            const double dx = fX - cw.x[0];
+            const double dy = fY - cw.x[1];
+            const double dz = fZ - cw.x[2];
             sus[0] = cw.v[0] + (cw.w[1] * dz - cw.w[2] * dy);
             sus[1] = cw.v[1] + (-cw.w[0] * dz + cw.w[2] * dx);
             sus[2] = cw.v[2] + (cw.w[0] * dy - cw.w[1] * dx);
@@ -629,13 +638,13 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
         a.rec.us[ridx] = sus[0];
         a.rec.us[a.rcap + ridx] = sus[1];
         a.rec.us[2 * a.rcap + ridx] = sus[2];
-        a.rec.cand[ridx] = se;
+        a.rec.cand[ridx] = cstart + js;   /* the setter's entry index */
         if (a.rec.m4) {
             float m[4];
             record_mirror(ssf, sus, m);
             *(float4*)(a.rec.m4 + 4 * ridx) = make_float4(m[0], m[1], m[2], m[3]);
         }
-    } else if (kind == ST_OLD) {
+    } else if (sj == ST_KEPT) {
         /* untouched this step: values stay, summed on the orphan path */
         a.rec.cand[ridx] = -1;
         const unsigned long long p = atomicAdd((unsigned long long*)&a.counters[2], 1ull);
