@@ -363,34 +363,78 @@ __device__ __forceinline__ double readlane_d(double v, int L) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+/* Cross-lane moves of a double that stay in the VALU (no LDS round trip).
+ * swap32(a, b) exchanges lanes 32-63 of a with lanes 0-31 of b
+ * (v_permlane32_swap); afterwards a + b is, on lanes 0-31, a[l] + a[l + 32]
+ * and, on lanes 32-63, b[l - 32] + b[l].  swap16 does the same between
+ * odd and even rows of 16 lanes (v_permlane16_swap). */
+__device__ __forceinline__ void swap32(double& a, double& b) {
+    const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+    const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+    a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+__device__ __forceinline__ void swap16(double& a, double& b) {
+    const unsigned long long ua = (unsigned long long)__double_as_longlong(a);
+    const unsigned long long ub = (unsigned long long)__double_as_longlong(b);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+    a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+}
+/* DPP move of a double inside a row of 16 lanes: lanes whose bank (four
+ * lanes) is in bank_mask take src from the lane ctrl names, the others keep
+ * old. */
+template <int CTRL, int BANKS>
+__device__ __forceinline__ double dpp_d(double old, double src) {
+    const unsigned long long uo = (unsigned long long)__double_as_longlong(old);
+    const unsigned long long us = (unsigned long long)__double_as_longlong(src);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)uo, (int)(unsigned)us, CTRL, 0xf, BANKS, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(uo >> 32), (int)(unsigned)(us >> 32), CTRL,
+                                                              0xf, BANKS, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+constexpr int DPP_QUAD_XOR1 = 0xB1;       /* quad_perm [1, 0, 3, 2] */
+constexpr int DPP_QUAD_XOR2 = 0x4E;       /* quad_perm [2, 3, 0, 1] */
+constexpr int DPP_ROW_ROR8 = 0x128;       /* row_ror:8, lane l <- lane l ^ 8 of its row */
+constexpr int DPP_ROW_HALF_MIRROR = 0x141;   /* lane l <- lane 7 - l of its eight */
+
 /* Sum of v[0..7] over the 64 lanes of a wavefront, reduce-scatter style:
  * three halving exchanges (lane bits 5, 4, 3) leave each lane one value,
  * index m = 4 b5 + 2 b4 + b3 of the lane, summed over the eight lanes that
- * share those bits; three butterflies over bits 2, 1, 0 finish it.  Ten
- * shuffles of a double in place of 36 for six separate trees, and the order
- * of every addition is fixed (a + b and b + a round alike), so the sum is
- * deterministic. */
+ * share those bits; three more steps inside each group of eight (quad
+ * exchanges, then the two quads) finish it.  The exchanges are permlane
+ * swaps and DPP moves, so the whole tree runs in the VALU without waiting
+ * on the LDS.  Every addition has a fixed pair of operands (a + b and b + a
+ * round alike), so the sum is deterministic.  All 64 lanes must be
+ * active. */
 __device__ __forceinline__ double wave_sum8_scatter(const double (&v)[8], int lane) {
     double w4[4], w2[2];
-    const bool u5 = (lane & 32) != 0, u4 = (lane & 16) != 0, u3 = (lane & 8) != 0;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        const double keep = u5 ? v[4 + m] : v[m];
-        const double send = u5 ? v[m] : v[4 + m];
-        w4[m] = keep + __shfl_xor(send, 32, 64);
+        /* lanes 0-31 sum value m, lanes 32-63 value 4 + m */
+        double a = v[m], b = v[4 + m];
+        swap32(a, b);
+        w4[m] = a + b;
     }
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-        const double keep = u4 ? w4[2 + m] : w4[m];
-        const double send = u4 ? w4[m] : w4[2 + m];
-        w2[m] = keep + __shfl_xor(send, 16, 64);
+        /* even rows value m (of the half), odd rows 2 + m */
+        double a = w4[m], b = w4[2 + m];
+        swap16(a, b);
+        w2[m] = a + b;
     }
-    const double keep = u3 ? w2[1] : w2[0];
-    const double send = u3 ? w2[0] : w2[1];
-    double w = keep + __shfl_xor(send, 8, 64);
-    w += __shfl_xor(w, 4, 64);
-    w += __shfl_xor(w, 2, 64);
-    w += __shfl_xor(w, 1, 64);
+    /* lanes 0-7 of a row keep w2[0] and add it from lanes 8-15; lanes 8-15
+     * keep w2[1] and add it from lanes 0-7 */
+    const bool u3 = (lane & 8) != 0;
+    double part = dpp_d<DPP_ROW_ROR8, 0x3>(0.0, w2[0]);
+    part = dpp_d<DPP_ROW_ROR8, 0xC>(part, w2[1]);
+    double w = (u3 ? w2[1] : w2[0]) + part;
+    w += dpp_d<DPP_QUAD_XOR1, 0xF>(0.0, w);
+    w += dpp_d<DPP_QUAD_XOR2, 0xF>(0.0, w);
+    w += dpp_d<DPP_ROW_HALF_MIRROR, 0xF>(0.0, w);
     return w;
 }
 
