@@ -427,7 +427,7 @@ int lbdem_psm_destroy(lbdem_psm* h) {
     dfree(h->p_ent_start); dfree(h->owner_ent); dfree(h->partial); dfree(h->pft); dfree(h->psum_x);
     dfree(h->scan_tmp); dfree(h->counters); dfree(h->red_tmp); dfree(h->orphan);
     dfree(h->orph_cnt); dfree(h->orph_fill); dfree(h->orph_start); dfree(h->orph_slot); dfree(h->orph_sorted);
-    dfree(h->dv_cell); dfree(h->dv_us); dfree(h->pp.buf);
+    dfree(h->dv_cell); dfree(h->dv_us); dfree(h->pp.buf); dfree(h->pp.cells);
     if (h->h_counters) (void)hipHostFree(h->h_counters);
     for (auto& st : h->stage) {
         if (st.buf) (void)hipHostFree(st.buf);

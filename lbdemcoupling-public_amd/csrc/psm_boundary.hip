@@ -69,6 +69,25 @@ __global__ __launch_bounds__(256) void k_plane_rhobar(const typename Store<PREC>
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+/* The same partial sums from the per-cell rhoBar the collide wrote for one
+ * plane (cells[k], k = (a - a0) nb + (b - b0)): the same grid, the same k per
+ * thread and the same tree as k_plane_rhobar, so the sums agree to the bit. */
+__global__ __launch_bounds__(256) void k_plane_cells_partial(const double* __restrict__ cells, int64_t ncell,
+                                                             double* __restrict__ part) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ncell;
+         k += (int64_t)gridDim.x * blockDim.x)
+        acc += cells[k];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
 /* Fixed-order sum of n partials into out[0]. */
 __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ part, int n,
                                                       double* __restrict__ out) {
@@ -180,6 +199,42 @@ int periodic_pressure_post_buf(lbdem_psm* h, void* f) {
 }
 
 int periodic_pressure_pre(lbdem_psm* h) { return periodic_pressure_pre_buf(h, h->f[h->cur_f]); }
+
+int periodic_pressure_cells(lbdem_psm* h, double** plane_rho, int* dim, int* a0, int* a1, int* b0, int* b1, int* nb,
+                            int64_t* ncell) {
+    *plane_rho = nullptr;
+    if (!h->pp.enabled) return LBDEM_PSM_OK;
+    const PlaneArgs p = plane_args(h);
+    const int64_t n = (int64_t)(p.a1 - p.a0 + 1) * (p.b1 - p.b0 + 1);
+    if (h->pp.cells_cap < 2 * n) {
+        if (h->pp.cells) PSM_HIP(h, hipFree(h->pp.cells));
+        h->pp.cells = nullptr;
+        h->pp.cells_cap = 0;
+        PSM_HIP(h, hipMalloc((void**)&h->pp.cells, 2 * n * sizeof(double)));
+        h->pp.cells_cap = 2 * n;
+    }
+    *plane_rho = h->pp.cells;
+    *dim = p.dim;
+    *a0 = p.a0; *a1 = p.a1; *b0 = p.b0; *b1 = p.b1;
+    *nb = p.b1 - p.b0 + 1;
+    *ncell = n;
+    return LBDEM_PSM_OK;
+}
+
+int periodic_pressure_sum_cells(lbdem_psm* h) {
+    if (!h->pp.enabled) return LBDEM_PSM_OK;
+    const PlaneArgs p = plane_args(h);
+    const int64_t n = (int64_t)(p.a1 - p.a0 + 1) * (p.b1 - p.b0 + 1);
+    const unsigned nb = plane_blocks(p);
+    /* [0] inlet sum, [1] outlet sum, partials from [8] on */
+    for (int pl = 0; pl < 2; ++pl) {
+        hipLaunchKernelGGL(k_plane_cells_partial, dim3(nb), dim3(256), 0, h->stream, h->pp.cells + pl * n, n,
+                           h->pp.buf + 8);
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, h->stream, h->pp.buf + 8, (int)nb, h->pp.buf + pl);
+    }
+    PSM_HIP(h, hipGetLastError());
+    return LBDEM_PSM_OK;
+}
 
 int periodic_pressure_post(lbdem_psm* h) { return periodic_pressure_post_buf(h, h->f[h->cur_f]); }
 

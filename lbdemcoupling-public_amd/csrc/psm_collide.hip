@@ -169,6 +169,9 @@ static CollideArgs make_args(lbdem_psm* h) {
     while ((collide_wg(h->g.L.zc) << a.sub_shift) < h->g.L.zc) ++a.sub_shift;
     a.div_ny = make_fastdiv((uint32_t)h->g.n[1]);
     a.rm4 = h->rec.m4;
+    a.plane_rho = nullptr;
+    a.pdim = a.plast = a.pa0 = a.pa1 = a.pb0 = a.pb1 = a.pnb = 0;
+    a.pncell = 0;
     return a;
 }
 
@@ -198,14 +201,19 @@ static void launch_any(lbdem_psm* h, const CollideArgs& a, const void* fin, void
 
 int launch_collide_stream(lbdem_psm* h) {
     CollideArgs a = make_args(h);
-    int rc = periodic_pressure_pre(h);
+    /* with the periodic pressure on, the collide writes the plane cells'
+     * rhoBar of f(t) as it loads them; they are summed before the rescale */
+    int rc = periodic_pressure_cells(h, &a.plane_rho, &a.pdim, &a.pa0, &a.pa1, &a.pb0, &a.pb1, &a.pnb, &a.pncell);
     if (rc) return rc;
+    a.plast = h->g.n[a.pdim] - 1;
     hipEvent_t ev = nullptr;
     prof_begin(h, 0, &ev);
     launch_any(h, a, h->f[h->cur_f], h->f[1 - h->cur_f]);
     PSM_HIP(h, hipGetLastError());
     prof_end(h, 0, ev);
     h->cur_f = 1 - h->cur_f;
+    rc = periodic_pressure_sum_cells(h);
+    if (rc) return rc;
     return periodic_pressure_post(h);
 }
 

@@ -31,7 +31,45 @@ struct CollideArgs {
     const float* rm4;  /* float view of the records (FP32/FP16 storage) */
     FastDiv div_nzc, div_ny;   /* block -> (row, chunk), row -> (x, y) */
     int sub_shift;             /* workgroups per (row, chunk) block: 1 << sub_shift */
+    /* periodic pressure: each inlet / outlet plane cell of the cross-section
+     * box writes its rhoBar (FP64 sum of its 19 pre-collision populations)
+     * to plane_rho[plane * pncell + (a - pa0) * pnb + (b - pb0)]; none when
+     * plane_rho is null */
+    double* plane_rho;
+    int pdim, plast, pa0, pa1, pb0, pb1, pnb;   /* plast = n[pdim] - 1 */
+    int64_t pncell;
 };
+
+/* The plane sums of the periodic pressure from the collide's own loads:
+ * the lanes of the inlet (index 0 along pdim) and outlet (index n-1) plane
+ * cells inside the box write the cell's rhoBar, summed in FP64 in
+ * population order (the order of the separate plane-sum kernel, so the sums
+ * are the same to the bit).  Wave-uniform tests first: only the wavefronts
+ * that hold such a cell go on. */
+template <typename T>
+__device__ __forceinline__ void plane_rhobar(const CollideArgs& a, const T* fp, int x, int y, int z) {
+    if (a.plane_rho == nullptr) return;
+    const int lane = (int)__lane_id();
+    const int zw0 = __builtin_amdgcn_readfirstlane(z - lane);   /* z of lane 0 of this wavefront */
+    bool near;
+    if (a.pdim == 2)
+        near = (zw0 == 0 || zw0 + 63 >= a.nz - 1) && x >= a.pa0 && x <= a.pa1 && y >= a.pb0 && y <= a.pb1;
+    else if (a.pdim == 0)
+        near = (x == 0 || x == a.nx - 1) && y >= a.pa0 && y <= a.pa1;
+    else
+        near = (y == 0 || y == a.ny - 1) && x >= a.pa0 && x <= a.pa1;
+    if (!near) return;
+    const int ci = a.pdim == 0 ? x : (a.pdim == 1 ? y : z);
+    const int pl = ci == 0 ? 0 : (ci == a.plast ? 1 : -1);
+    const int ca = a.pdim == 0 ? y : x;
+    const int cb = a.pdim == 2 ? y : z;
+    if (pl >= 0 && ca >= a.pa0 && ca <= a.pa1 && cb >= a.pb0 && cb <= a.pb1) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) s = s + (double)fp[i];
+        a.plane_rho[pl * a.pncell + (int64_t)(ca - a.pa0) * a.pnb + (cb - a.pb0)] = s;
+    }
+}
 
 #ifndef COLLIDE_WG
 #define COLLIDE_WG 256
@@ -359,6 +397,7 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     T fp[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) fp[i] = (T)raw[i];
+    plane_rhobar<T>(a, fp, x, y, z);
     T fo[Q];
     double Fc[3] = {0.0, 0.0, 0.0};
     if constexpr (sizeof(T) == 4) {

@@ -566,6 +566,8 @@ struct lbdem_psm {
         double rho_in = 1.0, rho_out = 1.0;
         int range[4] = {0, 0, 0, 0};   /* inclusive cross-section box on the other two axes */
         double* buf = nullptr;         /* [0] inlet rhoBar sum, [1] outlet sum, partials from [8] */
+        double* cells = nullptr;       /* 2 x box cells: each plane cell's rhoBar, written by the collide */
+        int64_t cells_cap = 0;
     } pp;
 
     /* differentiable window */
@@ -621,6 +623,12 @@ int launch_init_equilibrium(lbdem_psm* h, const double* d_rho, const double* d_u
 int launch_pop_convert(lbdem_psm* h, double* d_f64, bool to_device_store);
 int launch_pop_convert_buf(lbdem_psm* h, void* buf, double* d_f64, bool to_device_store);
 int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, void* rF);
+/* Periodic pressure with the plane sums taken inside the collide: the
+ * buffer and box the collide writes to (plane_rho null when the pressure is
+ * off), then the fixed-order sums of what it wrote. */
+int periodic_pressure_cells(lbdem_psm* h, double** plane_rho, int* dim, int* a0, int* a1, int* b0, int* b1, int* nb,
+                            int64_t* ncell);
+int periodic_pressure_sum_cells(lbdem_psm* h);
 int diff_free(lbdem_psm* h);
 int periodic_pressure_pre(lbdem_psm* h);
 int launch_macroscopic(lbdem_psm* h, double* d_rho, double* d_u);
