@@ -671,6 +671,31 @@ def summarize_profiles(d, wl):
         json.dump(t, f, indent=1)
 
 
+ADJ_TRACE_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace_adjoint128 -o run -- "
+                 "python3 bench.py --adjoint-only")
+
+
+def summarize_adjoint_trace(d):
+    """Kernel-trace stats of the --adjoint-only run -> profiles/kernels_adjoint128_trace.md."""
+    import glob
+    import sqlite3
+    dbs = glob.glob(os.path.join(d, "trace_adjoint128", "*results.db"))
+    if not dbs:
+        return
+    con = sqlite3.connect(dbs[0])
+    rows = list(con.execute("select name,total_calls,total_duration,average,percentage from top_kernels"))
+    with open(os.path.join(ROOT, "profiles", "kernels_adjoint128_trace.md"), "w") as f:
+        f.write(f"# Kernel durations, {ADJOINT_CASE['desc']}\n\n")
+        f.write("Command: `" + ADJ_TRACE_CMD + "` (rocprofv3 kernel trace, ROCm 7.2, one MI355X).\n")
+        f.write("Per precision (FP64, then FP32-dev) the run does one window to allocate the state pool "
+                "(500 steps forward with a checkpoint every 50, then the backward pass), the timed window "
+                "of the same shape, and 2 + 500 plain steps of the same frozen geometry. Every collide "
+                "here runs with the body force.\n\n")
+        f.write("| kernel | calls | total us | average us | % |\n|---|---|---|---|---|\n")
+        for name, n, tot, avg, pct in rows[:24]:
+            f.write(f"| {_short(name)} | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |\n")
+
+
 def kernel_probe(key, out_dir=None):
     """Short run for hardware-counter profiles: the copy k_copy16 of the FP64
     population store (known size, for calibration), then 3 coupled steps per
@@ -702,6 +727,9 @@ def main():
                     help="the headline workload only (no other workloads, adjoint, exchange loop or CPU baseline)")
     ap.add_argument("--no-extra", action="store_true", help="skip the other workloads")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiable-window lines")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only the 128x128x256 differentiable window, FP64 and FP32-dev (for its kernel trace); "
+                         "with --summarize-profiles DIR: summarize DIR/trace_adjoint128")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--summarize-profiles", metavar="DIR",
                     help="turn rocprofv3 outputs under DIR (trace_<wl>/, fetch_<wl>/, write_<wl>/, "
@@ -713,7 +741,10 @@ def main():
                     help="where --kernel-probe writes the calibration copy's size")
     args = ap.parse_args()
     if args.summarize_profiles:
-        summarize_profiles(args.summarize_profiles, args.workload)
+        if args.adjoint_only:
+            summarize_adjoint_trace(args.summarize_profiles)
+        else:
+            summarize_profiles(args.summarize_profiles, args.workload)
         return
     if args.gpus != 1:
         print(f"bench.py: the library runs on one GPU; --gpus {args.gpus} is not supported "
@@ -721,6 +752,15 @@ def main():
         sys.exit(2)
     if args.kernel_probe:
         kernel_probe(args.workload, args.probe_out)
+        return
+    if args.adjoint_only:
+        adj = {}
+        for prec in ("fp64", "fp32_dev"):
+            log(f"adjoint window {prec}: {ADJOINT_CASE['steps']} steps forward + backward")
+            adj[prec] = run_adjoint(prec)
+            adj[prec].pop("_g")
+            log(f"adjoint {prec}: forward {adj[prec]['forward_s']:.3f} s, backward {adj[prec]['backward_s']:.3f} s")
+        print(json.dumps({"workload": ADJOINT_CASE["desc"], **adj}))
         return
 
     key = args.workload
