@@ -465,15 +465,24 @@ def test_periodic_pressure_channel_matches_oracle():
     assert rel_err(g.get_populations(), o.get_populations()) < 1e-12
 
 
+# cross-section boxes (a0, a1, b0, b1) over the two other axes in increasing
+# order, touching a lattice edge on one side and not the other
+BOXES = {0: (0, 9, 3, 15), 1: (2, 13, 0, 11), 2: (0, 13, 1, 6)}
+
+
+@pytest.mark.parametrize("box", [False, True])
 @pytest.mark.parametrize("dim", [0, 1, 2])
-def test_periodic_pressure_every_axis_matches_oracle(dim):
+def test_periodic_pressure_every_axis_matches_oracle(dim, box):
     """The Zhang-Kwok rescale along each axis of a fully periodic box with
     spheres, 6 steps, against the oracle (the inlet/outlet boxes are x, y
-    or z planes; each axis maps the box onto the lattice differently)."""
+    or z planes; each axis maps the box onto the lattice differently), with
+    the default interior cross-section and with an uneven box.  The device
+    takes the plane sums from the collide's own loads of the plane cells
+    inside the box."""
     n = (14, 12, 16)
     g, o = make_pair(n, periodic=(1, 1, 1), tau=0.8, N=5)
     for lat in (g, o):
-        lat.set_periodic_pressure(dim, 1.02, 0.98)
+        lat.set_periodic_pressure(dim, 1.02, 0.98, range4=BOXES[dim] if box else None)
     x = np.array([[6.3, 5.2, 3.1], [7.6, 4.1, 10.4], [4.2, 8.7, 14.6]])
     v = np.array([[0.01, -0.004, 0.003], [-0.006, 0.002, 0.008], [0.003, 0.005, -0.007]])
     w = np.zeros((3, 3))
