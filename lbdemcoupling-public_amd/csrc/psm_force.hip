@@ -42,7 +42,7 @@ struct ForceArgs {
     int64_t nbins;
 };
 
-/* One wavefront per bin, four bins per workgroup.  Lane (y, z) of the bin
+/* One wavefront per bin (FORCE_WAVES bins per workgroup).  Lane (y, z) of the bin
  * holds its eight cells (x, y, z), x = 0..7, in registers: the liveness
  * flag, the candidate entry that set the record and the force.  The
  * particle positions of the bin's candidate entries are fetched one entry

@@ -13,8 +13,8 @@
  * previous step.  Here every particle image is binned into 8^3-cell bins
  * (key = image rank << 27 | particle index, which is the canonical order:
  * the given particles by index, then periodic images by shift rank), each
- * bin's candidate list is sorted, and one workgroup per bin (one lane per
- * cell, one wavefront per x-slice) folds them, per cell and in key order,
+ * bin's candidate list is sorted, and one wavefront per x-slice of a bin
+ * (one lane per cell) folds them, per cell and in key order,
  * through the reference's four-case rule; the candidates' data are
  * wave-uniform and come in through scalar loads.  Each cell's record lives
  * at the cell's own index and is updated in place (the fold of a cell
@@ -498,8 +498,8 @@ __device__ __forceinline__ double solid_fraction(bool active, double dx, double 
 
 enum { ST_NONE = -2, ST_KEPT = -1, ST_SET_FROM = 0 };
 
-/* One workgroup of 512 lanes per 8^3 bin, one cell per lane (a wavefront
- * is one x-slice of the bin).  The cell's record of the previous step is
+/* One wavefront per x-slice of an 8^3 bin (8 x 8 cells, one per lane;
+ * FOLD_WAVES slices per workgroup).  The cell's record of the previous step is
  * folded with the bin's candidates in key order through the four-case rule
  * of SetSingleSphere3D::process (src/ibProcessors3D.hh:85-105) and written
  * back in place. */
