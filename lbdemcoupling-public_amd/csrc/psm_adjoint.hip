@@ -241,8 +241,7 @@ struct AdjArgs {
     double omega, ooo;
     double acc[3];
     const uint8_t* sflag;
-    const double* rsf;
-    const double* rus;
+    const double* rsus;         /* 4 per record slot: sf, us[3] */
     const int32_t* rid;
     int64_t rcap;
     const double* gft;          /* 6 per particle for this step, or null */
@@ -332,10 +331,10 @@ __device__ __forceinline__ void adjoint_record_cell(const typename Store<PREC>::
     pull_adjoint<PREC>(lin, a, x, y, z, lam);
 #pragma unroll
     for (int i = 0; i < Q; ++i) fP[i] = Store<PREC>::load(fin + pop_at(row, i, z, a.L));
-    const double sf = a.rsf[r];
+    const double sf = a.rsus[4 * r];
     const bool full = sf > SOLFRAC_MAX;
     const T B = full ? T(1) : T((sf * a.ooo) / ((1.0 - sf) + a.ooo));
-    const T us[3] = {T(a.rus[r]), T(a.rus[a.rcap + r]), T(a.rus[2 * a.rcap + r])};
+    const T us[3] = {T(a.rsus[4 * r + 1]), T(a.rsus[4 * r + 2]), T(a.rsus[4 * r + 3])};
     T gF[3] = {T(0), T(0), T(0)};
     double gT[3] = {0.0, 0.0, 0.0};
     bool obj = false;
@@ -585,7 +584,7 @@ __global__ __launch_bounds__(256) void k_sum_vec(const double* __restrict__ in, 
  * sf) + o), o = tau - 1/2) and the tau partial sum_r dJ/dB dB/do. */
 __global__ __launch_bounds__(256) void k_grad_cells(
                                                     int64_t rcap, const uint8_t* __restrict__ sflag,
-                                                    const double* __restrict__ rsf,
+                                                    const double* __restrict__ rsus,
                                                     const double* __restrict__ gacc,
                                                     const int32_t* __restrict__ rpos, double ooo,
                                                     double* __restrict__ cell_out, int64_t nc,
@@ -597,7 +596,7 @@ __global__ __launch_bounds__(256) void k_grad_cells(
         const int64_t c = r;   /* record slot = cell index */
         double gB = 0.0, gsf = 0.0;
         if (sflag[c] && rpos[c] >= 0) {
-            const double sf = rsf[r];
+            const double sf = rsus[4 * r];
             if (!(sf > SOLFRAC_MAX)) {
                 gB = gacc[rpos[c]];
                 const double den = (1.0 - sf) + ooo;
@@ -616,7 +615,7 @@ struct GradArgs {
     int n[3];
     int nb[3];
     const uint8_t* sflag;
-    const double* rsf;
+    const double* rsus;         /* 4 per record slot: sf, us[3] */
     const int32_t* rcand;
     const int32_t* rid;
     int64_t rcap;
@@ -660,7 +659,7 @@ __device__ __forceinline__ void record_grad(const GradArgs& a, int64_t r, int X,
     double gx[3] = {(w[1] * g[2] - w[2] * g[1]) - fxt[0], (w[2] * g[0] - w[0] * g[2]) - fxt[1],
                     (w[0] * g[1] - w[1] * g[0]) - fxt[2]};
     if (a.overlap) {
-        const double sf = a.rsf[r];
+        const double sf = a.rsus[4 * r];
         if (!(sf > SOLFRAC_MAX)) {
             const double den = (1.0 - sf) + a.ooo;
             const double gsf = a.gacc[p] * (a.ooo * (1.0 + a.ooo)) / (den * den);
@@ -757,7 +756,7 @@ __global__ __launch_bounds__(256) void k_grad_orphans(GradArgs a, int64_t np, in
  * (SOLFRAC_MIN, 1] or dsf is non-zero on a cell without a record. */
 __global__ __launch_bounds__(256) void k_perturb_sf(int n0, int n1, int n2,
                                                     const uint8_t* __restrict__ sflag,
-                                                    double* __restrict__ rsf,
+                                                    double* __restrict__ rsus,
                                                     const double* __restrict__ dsf,
                                                     int64_t* __restrict__ counters) {
     const int64_t nc = (int64_t)n0 * n1 * n2;
@@ -769,9 +768,9 @@ __global__ __launch_bounds__(256) void k_perturb_sf(int n0, int n1, int n2,
             atomicOr((unsigned long long*)&counters[1], 16ull);
             continue;
         }
-        const double v = rsf[c] + d;   /* record slot = cell index */
+        const double v = rsus[4 * c] + d;   /* record slot = cell index */
         if (!(v > SOLFRAC_MIN && v <= 1.0)) atomicOr((unsigned long long*)&counters[1], 16ull);
-        else rsf[c] = v;
+        else rsus[4 * c] = v;
     }
 }
 
@@ -842,7 +841,7 @@ static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lo
     a.omega = h->omega; a.ooo = h->ooo;
     for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
     a.sflag = h->sflag;
-    a.rsf = h->rec.sf; a.rus = h->rec.us; a.rid = h->rec.id; a.rcap = h->rec_cap;
+    a.rsus = h->rec.sus; a.rid = h->rec.id; a.rcap = h->rec_cap;
     a.gft = gft;
     a.id_map = h->id_map; a.id_map_size = h->id_map_size;
     a.xs = h->psum_x;
@@ -925,7 +924,7 @@ int lbdem_psm_perturb_solid_fraction(lbdem_psm* h, const double* dsf) {
     if (e == hipSuccess) e = hipMemsetAsync(h->counters + 1, 0, sizeof(int64_t), h->stream);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_perturb_sf, dim3((unsigned)std::min<int64_t>(8192, (nc + 255) / 256)), dim3(256), 0,
-                           h->stream, h->g.n[0], h->g.n[1], h->g.n[2], h->sflag, h->rec.sf,
+                           h->stream, h->g.n[0], h->g.n[1], h->g.n[2], h->sflag, h->rec.sus,
                            d, h->counters);
         e = hipGetLastError();
     }
@@ -1116,7 +1115,7 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
     }
     for (int k = 0; k < 2; ++k)
         if (!D.lam[k]) PSM_HIP(h, dalloc_d(&D.lam[k], h->pop_bytes));
-    if (!D.Fscr) PSM_HIP(h, dalloc_d(&D.Fscr, 3 * rcap * sizeof(double)));
+    if (!D.Fscr) PSM_HIP(h, dalloc_d(&D.Fscr, 4 * rcap * sizeof(double)));
     /* per-record adjoint arrays at the records' list positions: sized by the
      * window's record count */
     const size_t tsz = h->precision == LBDEM_PSM_FP64 ? sizeof(double) : sizeof(float);
@@ -1233,14 +1232,14 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
     /* reductions */
     double* red = h->red_tmp;   /* [0] omega adjoint, [1] tau adjoint through B */
     hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, h->stream, D.opart, nblk, red);
-    hipLaunchKernelGGL(k_grad_cells, dim3((unsigned)gblocks), dim3(256), 0, h->stream, rcap, h->sflag, h->rec.sf,
+    hipLaunchKernelGGL(k_grad_cells, dim3((unsigned)gblocks), dim3(256), 0, h->stream, rcap, h->sflag, h->rec.sus,
                        D.gacc, D.rpos, h->ooo, D.cell_out, nc, D.tau_part);
     hipLaunchKernelGGL(k_sum_vec, dim3(1), dim3(256), 0, h->stream, D.tau_part, (int64_t)gblocks, red + 1);
     if (np > 0) {
         GradArgs ga;
         for (int d = 0; d < 3; ++d) { ga.n[d] = h->g.n[d]; ga.nb[d] = h->g.nb[d]; }
         ga.sflag = h->sflag;
-        ga.rsf = h->rec.sf;
+        ga.rsus = h->rec.sus;
         ga.rcand = h->rec.cand;
         ga.rid = h->rec.id;
         ga.rcap = rcap;

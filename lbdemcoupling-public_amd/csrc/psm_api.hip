@@ -69,10 +69,9 @@ static void dfree(void* p) {
 
 static int alloc_records(lbdem_psm* h, Records& r) {
     const int64_t cap = h->rec_cap;
-    PSM_HIP(h, dalloc(&r.sf, cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&r.sus, 4 * cap * sizeof(double)));
     PSM_HIP(h, dalloc(&r.id, cap * sizeof(int32_t)));
-    PSM_HIP(h, dalloc(&r.us, 3 * cap * sizeof(double)));
-    PSM_HIP(h, dalloc(&r.F, 3 * cap * sizeof(double)));
+    PSM_HIP(h, dalloc(&r.F, 4 * cap * (h->precision == LBDEM_PSM_FP64 ? sizeof(double) : sizeof(float))));
     PSM_HIP(h, dalloc(&r.cand, cap * sizeof(int32_t)));
     r.m4 = nullptr;
     if (h->precision != LBDEM_PSM_FP64) PSM_HIP(h, dalloc(&r.m4, 4 * cap * sizeof(float)));
@@ -80,7 +79,7 @@ static int alloc_records(lbdem_psm* h, Records& r) {
 }
 
 static void free_records(Records& r) {
-    dfree(r.sf); dfree(r.id); dfree(r.us); dfree(r.F); dfree(r.cand); dfree(r.m4);
+    dfree(r.sus); dfree(r.id); dfree(r.F); dfree(r.cand); dfree(r.m4);
     memset(&r, 0, sizeof(r));
 }
 
@@ -683,18 +682,17 @@ int lbdem_psm_get_cell_records(lbdem_psm* h, double* sf, int64_t* id, double* us
     PSM_HIP(h, hipStreamSynchronize(h->stream));
     std::vector<uint8_t> flag(nc);
     PSM_HIP(h, hipMemcpy(flag.data(), h->sflag, nc, hipMemcpyDeviceToHost));
-    std::vector<double> rsf(cap), rus(3 * cap), rF(3 * cap);
+    std::vector<double> rsus(4 * cap), rF(4 * cap);
     std::vector<int32_t> rid(cap);
-    PSM_HIP(h, hipMemcpy(rsf.data(), h->rec.sf, cap * sizeof(double), hipMemcpyDeviceToHost));
+    PSM_HIP(h, hipMemcpy(rsus.data(), h->rec.sus, 4 * cap * sizeof(double), hipMemcpyDeviceToHost));
     PSM_HIP(h, hipMemcpy(rid.data(), h->rec.id, cap * sizeof(int32_t), hipMemcpyDeviceToHost));
-    PSM_HIP(h, hipMemcpy(rus.data(), h->rec.us, 3 * cap * sizeof(double), hipMemcpyDeviceToHost));
     if (h->precision == LBDEM_PSM_FP64) {
-        PSM_HIP(h, hipMemcpy(rF.data(), h->rec.F, 3 * cap * sizeof(double), hipMemcpyDeviceToHost));
+        PSM_HIP(h, hipMemcpy(rF.data(), h->rec.F, 4 * cap * sizeof(double), hipMemcpyDeviceToHost));
     } else {
         /* FP32/FP16 storage keeps the per-cell force as float */
-        std::vector<float> f32(3 * cap);
-        PSM_HIP(h, hipMemcpy(f32.data(), h->rec.F, 3 * cap * sizeof(float), hipMemcpyDeviceToHost));
-        for (int64_t k = 0; k < 3 * cap; ++k) rF[k] = (double)f32[k];
+        std::vector<float> f32(4 * cap);
+        PSM_HIP(h, hipMemcpy(f32.data(), h->rec.F, 4 * cap * sizeof(float), hipMemcpyDeviceToHost));
+        for (int64_t k = 0; k < 4 * cap; ++k) rF[k] = (double)f32[k];
     }
     for (int X = 0; X < n0; ++X)
         for (int Y = 0; Y < n1; ++Y)
@@ -702,11 +700,11 @@ int lbdem_psm_get_cell_records(lbdem_psm* h, double* sf, int64_t* id, double* us
                 const int64_t c = ((int64_t)X * n1 + Y) * n2 + Z;
                 const bool live = flag[c] != 0;
                 const int64_t r = rec_index(X, Y, Z, n1, n2);
-                if (sf) sf[c] = live ? rsf[r] : 0.0;
+                if (sf) sf[c] = live ? rsus[4 * r] : 0.0;
                 if (id) id[c] = live ? rid[r] : 0;
                 for (int d = 0; d < 3; ++d) {
-                    if (us) us[d * nc + c] = live ? rus[d * cap + r] : 0.0;
-                    if (F) F[d * nc + c] = live ? rF[d * cap + r] : 0.0;
+                    if (us) us[d * nc + c] = live ? rsus[4 * r + 1 + d] : 0.0;
+                    if (F) F[d * nc + c] = live ? rF[4 * r + d] : 0.0;
                 }
             }
     return LBDEM_PSM_OK;
@@ -717,7 +715,7 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
     const int64_t nc = h->g.nc, cap = h->rec_cap, nbins = h->nbins;
     const int n0 = h->g.n[0], n1 = h->g.n[1], n2 = h->g.n[2];
     std::vector<uint8_t> flag(nc, 0);
-    std::vector<double> rsf(cap, 0.0), rus(3 * cap, 0.0);
+    std::vector<double> rsus(4 * cap, 0.0);
     std::vector<int32_t> rid(cap, 0), cand(cap, -1), orph;
     std::vector<uint32_t> any(nbins, 0u);   /* live records per bin */
     for (int X = 0; X < n0; ++X)
@@ -730,9 +728,9 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
                 if (pid < 0 || pid >= (int64_t)INT32_MAX) return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "bad record id");
                 const int64_t r = rec_index(X, Y, Z, n1, n2);
                 flag[c] = 1;
-                rsf[r] = sf[c];
+                rsus[4 * r] = sf[c];
                 rid[r] = (int32_t)pid;
-                for (int d = 0; d < 3; ++d) rus[d * cap + r] = us ? us[d * nc + c] : 0.0;
+                for (int d = 0; d < 3; ++d) rsus[4 * r + 1 + d] = us ? us[d * nc + c] : 0.0;
                 any[((int64_t)(X >> 3) * h->g.nb[1] + (Y >> 3)) * h->g.nb[2] + (Z >> 3)] += 1u;
                 orph.push_back((int32_t)r);
             }
@@ -742,9 +740,8 @@ int lbdem_psm_set_cell_records(lbdem_psm* h, const double* sf, const int64_t* id
         return set_err(h, LBDEM_PSM_ERR_CAPACITY, "records exceed max_solid_cells");
     PSM_HIP(h, hipStreamSynchronize(h->stream));
     PSM_HIP(h, hipMemcpy(h->sflag, flag.data(), nc, hipMemcpyHostToDevice));
-    PSM_HIP(h, hipMemcpy(h->rec.sf, rsf.data(), cap * sizeof(double), hipMemcpyHostToDevice));
+    PSM_HIP(h, hipMemcpy(h->rec.sus, rsus.data(), 4 * cap * sizeof(double), hipMemcpyHostToDevice));
     PSM_HIP(h, hipMemcpy(h->rec.id, rid.data(), cap * sizeof(int32_t), hipMemcpyHostToDevice));
-    PSM_HIP(h, hipMemcpy(h->rec.us, rus.data(), 3 * cap * sizeof(double), hipMemcpyHostToDevice));
     PSM_HIP(h, hipMemcpy(h->rec.cand, cand.data(), cap * sizeof(int32_t), hipMemcpyHostToDevice));
     PSM_HIP(h, hipMemcpy(h->bin_any[h->cur_any], any.data(), nbins * sizeof(uint32_t), hipMemcpyHostToDevice));
     if (m > 0) PSM_HIP(h, hipMemcpy(h->orphan, orph.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));

@@ -163,7 +163,7 @@ static CollideArgs make_args(lbdem_psm* h) {
     a.omega = h->omega; a.ooo = h->ooo;
     for (int d = 0; d < 3; ++d) a.acc[d] = h->desc.body_accel[d];
     a.sflag = h->sflag;
-    a.rsf = h->rec.sf; a.rus = h->rec.us; a.rF = h->rec.F; a.rcap = h->rec_cap;
+    a.rsus = h->rec.sus; a.rF = h->rec.F;
     a.div_nzc = make_fastdiv((uint32_t)h->g.L.nzc);
     a.sub_shift = 0;
     while ((collide_wg(h->g.L.zc) << a.sub_shift) < h->g.L.zc) ++a.sub_shift;
@@ -218,7 +218,7 @@ int launch_collide_stream(lbdem_psm* h) {
 }
 
 /* One collide+stream between caller-chosen buffers (with the periodic-
- * pressure rescale when enabled); the per-cell forces go to rF (3 per
+ * pressure rescale when enabled); the per-cell forces go to rF (4 per
  * record slot) instead of the records. */
 int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, void* rF) {
     CollideArgs a = make_args(h);

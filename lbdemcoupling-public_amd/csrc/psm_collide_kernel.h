@@ -24,10 +24,9 @@ struct CollideArgs {
     double omega, ooo;
     double acc[3];
     const uint8_t* sflag;
-    const double* rsf;
-    const double* rus;
-    void* rF;        /* per-cell force: double for FP64 storage, float otherwise */
-    int64_t rcap;
+    const double* rsus;   /* 4 per record slot: sf, us[3] */
+    void* rF;             /* 4 per record slot: the cell's force, double for FP64 storage,
+                           * float otherwise, and a padding zero */
     const float* rm4;  /* float view of the records (FP32/FP16 storage) */
     FastDiv div_nzc, div_ny;   /* block -> (row, chunk), row -> (x, y) */
     int sub_shift;             /* workgroups per (row, chunk) block: 1 << sub_shift */
@@ -388,10 +387,13 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     if constexpr (sizeof(T) == 4) {
         rm = *(const float4*)(a.rm4 + 4 * s);
     } else {
-        rsf = a.rsf[s];
-        rus[0] = a.rus[s];
-        rus[1] = a.rus[a.rcap + s];
-        rus[2] = a.rus[2 * a.rcap + s];
+        /* the 32-byte slot in two loads */
+        const double2 r0 = *(const double2*)(a.rsus + 4 * s);
+        const double2 r1 = *(const double2*)(a.rsus + 4 * s + 2);
+        rsf = r0.x;
+        rus[0] = r0.y;
+        rus[1] = r1.x;
+        rus[2] = r1.y;
     }
     __builtin_amdgcn_sched_barrier(0);
     T fp[Q];
@@ -476,30 +478,27 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
             Fc[2] = double(F[2]);
         }
     }
-    /* The lanes of every 128-byte line (of one force component) that holds a
-     * solid cell write the force of all their cells, zero for the fluid ones
-     * (the reference resets the force of every cell in each collision,
+    /* The lanes of every 128-byte line of the force slots that holds a solid
+     * cell write the force of all their cells, zero for the fluid ones (the
+     * reference resets the force of every cell in each collision,
      * src/ibCompositeDynamics3D.hh:137): whole lines, not the partial-line
      * writes of the solid cells alone, which cost a read-modify-write each,
      * and not the whole wavefront either (the lines without a solid cell are
-     * left alone: at 512x512x768 this took the FP64 collide from 12.7 to
-     * 11.4 ms).  Readers look at solid cells only. */
+     * left alone).  A slot is 32 bytes in double, 16 in float, so a line is
+     * four or eight cells: a short run of solid cells writes little more
+     * than its own slots.  Readers look at solid cells only. */
     const unsigned long long solid_bal = __builtin_amdgcn_ballot_w64(solid);
-    constexpr int FG = sizeof(T) == 4 ? 32 : 16;   /* lanes per line of one force component */
+    constexpr int FG = sizeof(T) == 4 ? 8 : 4;   /* lanes per 128-byte line */
     const int lane_id = (int)__lane_id();
     if ((solid_bal >> (lane_id & ~(FG - 1))) & ((1ull << FG) - 1)) {
         if constexpr (sizeof(T) == 4) {
             /* the FP32 arithmetic's force, stored exactly in float: half the
              * bytes here and in the force sum */
-            float* rF = (float*)a.rF;
-            rF[c] = (float)Fc[0];
-            rF[a.rcap + c] = (float)Fc[1];
-            rF[2 * a.rcap + c] = (float)Fc[2];
+            *((float4*)a.rF + c) = make_float4((float)Fc[0], (float)Fc[1], (float)Fc[2], 0.0f);
         } else {
-            double* rF = (double*)a.rF;
-            rF[c] = Fc[0];
-            rF[a.rcap + c] = Fc[1];
-            rF[2 * a.rcap + c] = Fc[2];
+            double* q = (double*)a.rF + 4 * c;
+            *(double2*)q = make_double2(Fc[0], Fc[1]);
+            *(double2*)(q + 2) = make_double2(Fc[2], 0.0);
         }
     }
     if (push.near_wall(chunk, nzc)) {

@@ -255,15 +255,19 @@ template <> struct Store<LBDEM_PSM_FP16_STORE> {
     static __device__ __forceinline__ void store(S* p, T v) { *p = (_Float16)v; }
 };
 
-/* Coupling records (IBdynamicsParticleData), structure of arrays with one
- * slot per cell (slot = cell index, rec_index).  A record is live when the
- * cell's solid flag is set. */
+/* Coupling records (IBdynamicsParticleData), one slot per cell (slot = cell
+ * index, rec_index).  A record is live when the cell's solid flag is set.
+ * The values the collide reads (sf, u_s) and the force it writes sit
+ * together per slot, 32 bytes (16 in float): a wavefront whose solid cells
+ * are a short run of its lanes touches the few lines that hold them, where
+ * one array per component spread the same run over a line of every array. */
 struct Records {
-    double* sf;      /* solid fraction */
+    double* sus;     /* 4*cap: per slot (sf, us_x, us_y, us_z): the solid fraction and the
+                      * particle velocity at the cell */
     int32_t* id;     /* particle tag */
-    double* us;      /* 3*cap: particle velocity at the cell, component-major */
-    double* F;       /* 3*cap: hydrodynamic force of the last collision (FP64 storage; with
-                      * FP32/FP16 storage the first 3*cap floats of the buffer hold it) */
+    double* F;       /* 4*cap per slot (F_x, F_y, F_z, 0): the hydrodynamic force of the last
+                      * collision, double with FP64 storage; with FP32/FP16 storage the first
+                      * 4*cap floats of the buffer hold it as float */
     int32_t* cand;   /* candidate entry of the setter this step, -1 = none */
     float* m4;       /* FP32/FP16 storage only, 4*cap: per cell (s, us) in float for the
                       * collide, s = 2 for sf > SOLFRAC_MAX, else (float)sf; valid where the

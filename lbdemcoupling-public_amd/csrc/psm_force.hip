@@ -33,7 +33,6 @@ struct ForceArgs {
     const uint32_t* key;
     const uint8_t* sflag;
     Records rec;
-    int64_t rcap;
     int f32;               /* per-cell forces stored as float (FP32/FP16 storage) */
     const int32_t* powner;
     const double* xs;      /* positions for the torque arms */
@@ -54,6 +53,19 @@ struct ForceArgs {
 #ifndef FORCE_WAVES
 #define FORCE_WAVES 1
 #endif
+
+/* The three force components of one record slot (its fourth element is
+ * padding): one 12-byte load in float, a 16-byte and an 8-byte one in
+ * double. */
+__device__ __forceinline__ void load_force3(const float* p, float& x, float& y, float& z) {
+    typedef float f3 __attribute__((ext_vector_type(3)));
+    const f3 v = *(const f3*)p;
+    x = v.x; y = v.y; z = v.z;
+}
+__device__ __forceinline__ void load_force3(const double* p, double& x, double& y, double& z) {
+    const double2 v = *(const double2*)p;
+    x = v.x; y = v.y; z = p[2];
+}
 
 template <typename FT>
 __global__ __launch_bounds__(64 * FORCE_WAVES) void k_force_bin(ForceArgs a) {
@@ -102,9 +114,7 @@ __global__ __launch_bounds__(64 * FORCE_WAVES) void k_force_bin(ForceArgs a) {
         if (fl[x]) {
             const int64_t r = (int64_t)(bx * BIN + x) * plane + lo;
             ce[x] = a.rec.cand[r];
-            fx[x] = F[r];
-            fy[x] = F[a.rcap + r];
-            fz[x] = F[2 * a.rcap + r];
+            load_force3(F + 4 * r, fx[x], fy[x], fz[x]);
         }
     }
     double px = a.xs[3 * po], py = a.xs[3 * po + 1], pz = a.xs[3 * po + 2];
@@ -294,9 +304,9 @@ __global__ __launch_bounds__(256) void k_force_orphans(ForceArgs a, int64_t np, 
         double fx, fy, fz;
         if (a.f32) {
             const float* F32 = (const float*)a.rec.F;
-            fx = F32[r]; fy = F32[a.rcap + r]; fz = F32[2 * a.rcap + r];
+            fx = F32[4 * r]; fy = F32[4 * r + 1]; fz = F32[4 * r + 2];
         } else {
-            fx = a.rec.F[r]; fy = a.rec.F[a.rcap + r]; fz = a.rec.F[2 * a.rcap + r];
+            fx = a.rec.F[4 * r]; fy = a.rec.F[4 * r + 1]; fz = a.rec.F[4 * r + 2];
         }
         acc[0] += fx;
         acc[1] += fy;
@@ -327,7 +337,6 @@ int run_force_sum(lbdem_psm* h, const double* xs, double* out) {
     a.key = h->cand_key;
     a.sflag = h->sflag;
     a.rec = h->rec;
-    a.rcap = h->rec_cap;
     a.f32 = h->precision == LBDEM_PSM_FP64 ? 0 : 1;
     a.powner = h->powner;
     a.xs = xs;

@@ -297,7 +297,6 @@ struct FoldArgs {
     const Cand* cand;
     int64_t ent_cap;
     Records rec;
-    int64_t rcap;
     uint8_t* sflag;
     const uint32_t* cnt_old;   /* per bin: live records of the previous step */
     uint32_t* cnt_new;         /* per bin: live records after this fold */
@@ -538,10 +537,9 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
     const double sdi = -0.5 + ((double)((l & 63) >> 3) + 0.5) * a.sw;
     const double sdj = -0.5 + ((double)(l & 7) + 0.5) * a.sw;
     if (had) {
-        ssf = a.rec.sf[ridx];
+        ssf = a.rec.sus[4 * ridx];
         sid = a.rec.id[ridx];
     }
-    const double osf = ssf;
     const int oid = sid;
     const double fX = (double)X, fY = (double)Y, fZ = (double)Z;
 
@@ -602,9 +600,9 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
             if (INITVEL && inb && sf > SOLFRAC_MAX) {
                 dv = true;
                 if (sj == ST_KEPT) {
-                    dvus[0] = a.rec.us[ridx];
-                    dvus[1] = a.rec.us[a.rcap + ridx];
-                    dvus[2] = a.rec.us[2 * a.rcap + ridx];
+                    dvus[0] = a.rec.sus[4 * ridx + 1];
+                    dvus[1] = a.rec.sus[4 * ridx + 2];
+                    dvus[2] = a.rec.sus[4 * ridx + 3];
                 } else {
                     dvus[0] = sus[0]; dvus[1] = sus[1]; dvus[2] = sus[2];
                 }
@@ -632,12 +630,12 @@ __global__ __launch_bounds__(64 * FOLD_WAVES) void k_fold(FoldArgs a) {
             sus[1] = cw.v[1] + (-cw.w[0] * dz + cw.w[2] * dx);
             sus[2] = cw.v[2] + (cw.w[0] * dy - cw.w[1] * dx);
         }
-        /* a live record whose fraction and owner did not change keeps them */
-        if (!had || ssf != osf) a.rec.sf[ridx] = ssf;
+        /* the whole 32-byte slot, the fraction with the velocity (whole
+         * lines where the wavefront's cells are live); a live record whose
+         * owner did not change keeps its id */
+        *(double2*)(a.rec.sus + 4 * ridx) = make_double2(ssf, sus[0]);
+        *(double2*)(a.rec.sus + 4 * ridx + 2) = make_double2(sus[1], sus[2]);
         if (!had || sid != oid) a.rec.id[ridx] = sid;
-        a.rec.us[ridx] = sus[0];
-        a.rec.us[a.rcap + ridx] = sus[1];
-        a.rec.us[2 * a.rcap + ridx] = sus[2];
         a.rec.cand[ridx] = cstart + js;   /* the setter's entry index */
         if (a.rec.m4) {
             float m[4];
@@ -776,7 +774,6 @@ int run_voxelize(lbdem_psm* h) {
     fa.cand = h->cand_data;
     fa.ent_cap = h->ent_cap;
     fa.rec = h->rec;
-    fa.rcap = h->rec_cap;
     fa.sflag = h->sflag;
     fa.cnt_old = h->bin_any[h->cur_any];
     fa.cnt_new = cnt_new;
@@ -812,15 +809,14 @@ int run_voxelize(lbdem_psm* h) {
 /* The float view of every live record, after a writer other than the fold
  * (host records, a checkpoint, the sf perturbation of the adjoint checks). */
 __global__ __launch_bounds__(256) void k_mirror_records(int64_t nc, const uint8_t* __restrict__ sflag,
-                                                        const double* __restrict__ sf,
-                                                        const double* __restrict__ us, int64_t rcap,
+                                                        const double* __restrict__ sus,
                                                         float* __restrict__ m4) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nc;
          c += (int64_t)gridDim.x * blockDim.x) {
         if (!sflag[c]) continue;
-        const double u[3] = {us[c], us[rcap + c], us[2 * rcap + c]};
+        const double u[3] = {sus[4 * c + 1], sus[4 * c + 2], sus[4 * c + 3]};
         float m[4];
-        record_mirror(sf[c], u, m);
+        record_mirror(sus[4 * c], u, m);
         *(float4*)(m4 + 4 * c) = make_float4(m[0], m[1], m[2], m[3]);
     }
 }
@@ -829,7 +825,7 @@ int mirror_records(lbdem_psm* h) {
     if (!h->rec.m4) return LBDEM_PSM_OK;
     const int64_t nc = h->g.nc;
     hipLaunchKernelGGL(k_mirror_records, dim3((unsigned)std::min<int64_t>(8192, (nc + 255) / 256)), dim3(256), 0,
-                       h->stream, nc, h->sflag, h->rec.sf, h->rec.us, h->rec_cap, h->rec.m4);
+                       h->stream, nc, h->sflag, h->rec.sus, h->rec.m4);
     PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }
