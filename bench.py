@@ -60,11 +60,14 @@ WORKLOADS = {
                        desc="channel 512x512x768, walls x/y, pressure-driven along z (Zhang-Kwok "
                             "periodic pressure rho 1 -> 1-1e-4, Poiseuille initial field u_max 0.05), "
                             "tau 0.55, 100000 spheres r~U[3,6], v~N(0,0.002), w~N(0,2e-4)"),
-    "channel512_slab": dict(n=(512, 512, 96), periodic=(0, 0, 1), tau=0.55, count=12500,
+    # one eighth of the channel cut along x, the axis the population store's
+    # rows are ordered by (row = x * ny + y): one GPU's share of an 8-way
+    # split, a contiguous range of rows
+    "channel512_slab": dict(n=(64, 512, 768), periodic=(0, 0, 1), tau=0.55, count=12500,
                             law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
-                            pressure=dict(dim=2, rho_hi=1.0, rho_lo=1.0 - 1.25e-5, u_max=0.05),
+                            pressure=dict(dim=2, rho_hi=1.0, rho_lo=1.0 - 1e-4, u_max=0.05),
                             sigma_v=0.002, sigma_w=2e-4, wall_gap=2.0,
-                            desc="slab 512x512x96 of the channel (1/8 of its length, same sphere "
+                            desc="slab 64x512x768 of the channel (1/8 of it along x, same sphere "
                                  "density and pressure gradient): 12500 spheres r~U[3,6]"),
     "cube512": dict(n=(512, 512, 512), periodic=(1, 1, 1), tau=0.55, count=66667,
                     law=L.RADIUS_UNIFORM, a=3.0, b=6.0, lo=0.0, hi=0.0, body=(0.0, 0.0, 0.0),
@@ -789,7 +792,8 @@ def main():
         head["exchange_loop"] = ex
     del res
     if extra_on:
-        for k2 in ("cube512", "dense256", "poly384"):
+        # the channel's slab first: one GPU's share of an 8-way split along x
+        for k2 in ("channel512_slab", "cube512", "dense256", "poly384"):
             if k2 == key:
                 continue
             steps2 = args.steps
