@@ -75,9 +75,12 @@ __global__ __launch_bounds__(64 * FORCE_WAVES) void k_force_bin(ForceArgs a) {
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t b = (int64_t)xcd_block((int)blockIdx.x, (int)gridDim.x) * FORCE_WAVES + wv;
     if (b >= a.nbins) return;
+    /* both ends of the candidate range in one round of loads (the empty asm
+     * keeps the second load from sinking below the test of the first) */
     const int ncand = a.bin_cand_count[b];
-    if (ncand == 0) return;
     const int cstart = a.bin_cand_start[b];
+    asm volatile("" ::"s"(ncand), "s"(cstart));
+    if (ncand == 0) return;
     if ((int64_t)cstart + ncand > a.ent_cap) return;
     const int bi = (int)b;
     const int bxy = (int)fdiv((uint32_t)bi, a.div_nb2);
@@ -89,8 +92,10 @@ __global__ __launch_bounds__(64 * FORCE_WAVES) void k_force_bin(ForceArgs a) {
     /* Three rounds of loads, each issued at once: the eight liveness
      * flags and the bin's candidate keys (one entry per lane); the records
      * of the live cells and the entries' owners; the owners' positions.
-     * The empty asm statements pin the loaded values in registers after
-     * the last round, so no conversion or test waits inside a round. */
+     * The scheduling barriers keep each round's address arithmetic behind
+     * the previous round's loads, and the empty asm statements pin the
+     * loaded values in registers after the last round, so no conversion or
+     * test waits inside a round. */
     const int64_t plane = (int64_t)a.n[1] * a.n[2];
     const int64_t lo = yz ? (int64_t)Y * a.n[2] + Z : 0;
     const int tl = lane < ncand ? lane : 0;
@@ -103,21 +108,30 @@ __global__ __launch_bounds__(64 * FORCE_WAVES) void k_force_bin(ForceArgs a) {
         fl[x] = a.sflag[xb + lo];
         if (!yz || X >= a.n[0]) fl[x] = 0;
     }
+    __builtin_amdgcn_sched_barrier(0);
     const int po = a.powner[(int)(kk & QMASK)];
     const FT* __restrict__ F = (const FT*)a.rec.F;
     int ce[BIN];
     FT fx[BIN], fy[BIN], fz[BIN];
+    /* Every lane loads, without a branch: a cell without a live record reads
+     * slot 0 (one line shared by the wavefront) and drops it below.  Loads
+     * under a per-cell branch made the compiler wait for the first cell's
+     * record before it issued the others (the branch's merge copies it). */
 #pragma unroll
     for (int x = 0; x < BIN; ++x) {
-        ce[x] = -1;
-        fx[x] = fy[x] = fz[x] = (FT)0;
-        if (fl[x]) {
-            const int64_t r = (int64_t)(bx * BIN + x) * plane + lo;
-            ce[x] = a.rec.cand[r];
-            load_force3(F + 4 * r, fx[x], fy[x], fz[x]);
-        }
+        const int64_t r = fl[x] ? (int64_t)(bx * BIN + x) * plane + lo : 0;
+        ce[x] = a.rec.cand[r];
+        load_force3(F + 4 * r, fx[x], fy[x], fz[x]);
     }
     double px = a.xs[3 * po], py = a.xs[3 * po + 1], pz = a.xs[3 * po + 2];
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int x = 0; x < BIN; ++x) {
+        if (!fl[x]) {
+            ce[x] = -1;
+            fx[x] = fy[x] = fz[x] = (FT)0;
+        }
+    }
 #pragma unroll
     for (int x = 0; x < BIN; ++x) asm volatile("" : "+v"(ce[x]), "+v"(fx[x]), "+v"(fy[x]), "+v"(fz[x]));
     asm volatile("" : "+v"(px), "+v"(py), "+v"(pz));
