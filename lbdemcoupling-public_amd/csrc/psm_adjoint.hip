@@ -1038,17 +1038,21 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
         PSM_HIP(h, hipStreamSynchronize(h->stream));
     }
     for (int t = 0, nextc = 0; t < nsteps; ++t) {
-        if (nextc < ncp && t == t0[nextc]) {
-            PSM_HIP(h, hipMemcpyAsync(D.ckpt[nextc], h->f[h->cur_f], h->pop_bytes, hipMemcpyDeviceToDevice,
-                                      h->stream));
-            ++nextc;
-        }
+        const bool segment_start = nextc < ncp && t == t0[nextc];
         rc = launch_collide_stream(h);
         if (rc) return rc;
         if (np > 0) {
             /* straight into the window's force history */
             rc = run_force_sum(h, h->psum_x, D.d_ft + (size_t)t * np * 6);
             if (rc) return rc;
+        }
+        if (segment_start) {
+            /* the step only read its input f(t), which is this segment's
+             * checkpoint: that buffer becomes the checkpoint, and the
+             * checkpoint's old buffer the next step's output (no copy of
+             * the state; all of them are pop_bytes device buffers) */
+            std::swap(D.ckpt[nextc], h->f[1 - h->cur_f]);
+            ++nextc;
         }
     }
     /* the handle's force buffer holds the last step's sums, as after a
@@ -1057,7 +1061,10 @@ int lbdem_psm_diff_forward(lbdem_psm* h, int32_t nsteps, int32_t K, double* ft) 
         PSM_HIP(h, hipMemcpyAsync(h->pft, D.d_ft + (size_t)(nsteps - 1) * np * 6, np * 6 * sizeof(double),
                                   hipMemcpyDeviceToDevice, h->stream));
     PSM_HIP(h, hipStreamSynchronize(h->stream));
-    if (ft && np > 0) {
+    if (ft && np > 0 && h->owner_identity) {
+        /* every particle is its own map(tag): the history is the fold */
+        PSM_HIP(h, hipMemcpy(ft, D.d_ft, (size_t)nsteps * np * 6 * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (ft && np > 0) {
         std::vector<double> tmp((size_t)nsteps * np * 6);
         PSM_HIP(h, hipMemcpy(tmp.data(), D.d_ft, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int t = 0; t < nsteps; ++t) {

@@ -555,10 +555,12 @@ int lbdem_psm_set_spheres(lbdem_psm* h, int64_t n, const double* x3, const doubl
         std::fill(idmap, idmap + (max_id + 1), -1);
         /* map(tag): the first index holding each id (owned atoms precede ghosts) */
         h->h_owner.assign(n, 0);
+        h->owner_identity = true;
         for (int64_t q = 0; q < n; ++q) {
             if (idmap[id[q]] < 0) idmap[id[q]] = (int32_t)q;
             owner[q] = idmap[id[q]];
             h->h_owner[q] = owner[q];
+            h->owner_identity = h->owner_identity && owner[q] == q;
             pid[q] = (int32_t)id[q];
             excl[q] = 0;
             if (type)

@@ -519,6 +519,7 @@ struct lbdem_psm {
     int32_t* id_map = nullptr;       /* id -> first particle index */
     int64_t id_map_size = 0;
     std::vector<int32_t> h_owner;    /* host copy of powner */
+    bool owner_identity = true;      /* h_owner[q] == q for every q (no user ghosts) */
     std::vector<int64_t> h_ids;
     /* the mapping inputs the device's pid / pexcl / powner / id_map were built
      * from (reused while a set_spheres call passes the same ones) */
