@@ -259,6 +259,11 @@ struct AdjArgs {
     const int32_t* rpos;        /* per cell: position in the record list */
     const int32_t* rown;        /* per list position: the record's particle index, or -1 */
     int64_t pcap;
+    /* periodic pressure: the plane-mean terms of its adjoint, added to every
+     * population of the inlet (pp_add[0]) and outlet (pp_add[1]) box cells of
+     * the output; none when pp_add is null */
+    const double* pp_add;
+    int pdim, plast, pa0, pa1, pb0, pb1;
 };
 
 /* Adjoint of the pushed populations at (x, y, z): pull from x + c_i, or
@@ -561,8 +566,28 @@ __global__ __launch_bounds__(256) void k_adjoint(const typename Store<PREC>::S* 
             for (int i = 0; i < Q; ++i) lam[i] = fP[i] + lam[i];
         }
         char* orow = (char*)(lout + pop_at(row, 0, chunk * a.L.zc, a.L));
+        bool pp_cell = false;
+        double ppc = 0.0;
+        if (a.pp_add != nullptr) {
+            /* after the collide+stream adjoint: each population of an inlet
+             * (outlet) box cell gains the outlet's (inlet's) plane-mean term,
+             * added in double to the stored value and rounded once; the plane
+             * mapping as selects */
+            const int ci = a.pdim == 0 ? x : (a.pdim == 1 ? y : z);
+            const int pl = ci == 0 ? 0 : (ci == a.plast ? 1 : -1);
+            const int ca = a.pdim == 0 ? y : x;
+            const int cb = a.pdim == 2 ? y : z;
+            pp_cell = pl >= 0 && ca >= a.pa0 && ca <= a.pa1 && cb >= a.pb0 && cb <= a.pb1;
+            if (pp_cell) ppc = a.pp_add[pl];
+        }
+        if (pp_cell) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) Store<PREC>::store((S*)(orow + i * qstride + zoff), lam[i]);
+            for (int i = 0; i < Q; ++i)
+                Store<PREC>::store((S*)(orow + i * qstride + zoff), (T)((double)(S)lam[i] + ppc));
+        } else {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) Store<PREC>::store((S*)(orow + i * qstride + zoff), lam[i]);
+        }
         wbar = (double)om;
     }
     const double s = block_sum_256(wbar, sh);
@@ -853,6 +878,16 @@ static int adjoint_step(lbdem_psm* h, const void* fin, const void* lin, void* lo
     a.rpos = h->diff.rpos;
     a.rown = h->diff.rown;
     a.pcap = h->diff.pcap;
+    a.pp_add = nullptr;
+    a.pdim = a.plast = a.pa0 = a.pa1 = a.pb0 = a.pb1 = 0;
+    if (h->pp.enabled) {
+        /* the terms periodic_pressure_adjoint_pre left in its buffer */
+        a.pp_add = h->pp.buf + 4;
+        a.pdim = h->pp.dim;
+        a.plast = h->g.n[a.pdim] - 1;
+        a.pa0 = h->pp.range[0]; a.pa1 = h->pp.range[1];
+        a.pb0 = h->pp.range[2]; a.pb1 = h->pp.range[3];
+    }
     const int64_t nrec = h->diff.nrec;
     dim3 gridr((unsigned)std::max<int64_t>(1, (nrec + 255) / 256)), blockr(256);
     const int32_t* list = h->diff.rec_list;
@@ -1216,10 +1251,6 @@ int lbdem_psm_diff_backward(lbdem_psm* h, const double* g_ft, const double* g_fT
             rc = adjoint_step(h, st[j], D.lam[cur], D.lam[1 - cur],
                               have_gft ? D.d_ft + (size_t)t * np * 6 : nullptr);
             if (rc) { dfree_d(tmp64); return rc; }
-            if (pp) {
-                rc = periodic_pressure_adjoint_post(h, D.lam[1 - cur]);
-                if (rc) { dfree_d(tmp64); return rc; }
-            }
             cur = 1 - cur;
         }
         if (pp && s > 0) {

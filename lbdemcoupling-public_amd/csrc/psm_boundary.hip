@@ -174,18 +174,6 @@ static void pp_post_t(lbdem_psm* h, const PlaneArgs& p, void* fv) {
                        (const double*)(h->pp.buf));
 }
 
-int periodic_pressure_pre_buf(lbdem_psm* h, const void* f) {
-    if (!h->pp.enabled) return LBDEM_PSM_OK;
-    const PlaneArgs p = plane_args(h);
-    switch (h->precision) {
-        case LBDEM_PSM_FP64: pp_pre_t<LBDEM_PSM_FP64>(h, p, f); break;
-        case LBDEM_PSM_FP32_DEV: pp_pre_t<LBDEM_PSM_FP32_DEV>(h, p, f); break;
-        default: pp_pre_t<LBDEM_PSM_FP16_STORE>(h, p, f); break;
-    }
-    PSM_HIP(h, hipGetLastError());
-    return LBDEM_PSM_OK;
-}
-
 int periodic_pressure_post_buf(lbdem_psm* h, void* f) {
     if (!h->pp.enabled) return LBDEM_PSM_OK;
     const PlaneArgs p = plane_args(h);
@@ -197,8 +185,6 @@ int periodic_pressure_post_buf(lbdem_psm* h, void* f) {
     PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }
-
-int periodic_pressure_pre(lbdem_psm* h) { return periodic_pressure_pre_buf(h, h->f[h->cur_f]); }
 
 int periodic_pressure_cells(lbdem_psm* h, double** plane_rho, int* dim, int* a0, int* a1, int* b0, int* b1, int* nb,
                             int64_t* ncell) {
@@ -342,28 +328,6 @@ __global__ __launch_bounds__(256) void k_pp_adj_scale(typename Store<PREC>::S* _
     }
 }
 
-/* lam_i += c on every population of one plane box. */
-template <int PREC>
-__global__ __launch_bounds__(256) void k_pp_adj_add(typename Store<PREC>::S* __restrict__ lam, PlaneArgs p,
-                                                    int plane, const double* __restrict__ coef) {
-    typedef typename Store<PREC>::T T;
-    const int na = p.a1 - p.a0 + 1, nb = p.b1 - p.b0 + 1;
-    const int64_t A = (int64_t)na * nb;
-    const double c = coef[0];
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < A;
-         k += (int64_t)gridDim.x * blockDim.x) {
-        const int a = p.a0 + (int)(k / nb), b = p.b0 + (int)(k % nb);
-        int x, y, z;
-        plane_cell(p.dim, plane, a, b, x, y, z);
-        const int64_t row = (int64_t)x * p.n[1] + y;
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            typename Store<PREC>::S* q = lam + pop_at(row, i, z, p.L);
-            Store<PREC>::store(q, (T)((double)Store<PREC>::load(q) + c));
-        }
-    }
-}
-
 int64_t periodic_pressure_plane_elems(const lbdem_psm* h) {
     const PlaneArgs p = plane_args(h);
     return 2 * (int64_t)Q * (p.a1 - p.a0 + 1) * (p.b1 - p.b0 + 1);
@@ -412,17 +376,6 @@ static void pp_adj_pre_t(lbdem_psm* h, const PlaneArgs& p, const void* fin, cons
                        (const double*)v);
 }
 
-template <int PREC>
-static void pp_adj_post_t(lbdem_psm* h, const PlaneArgs& p, void* lam) {
-    typedef typename Store<PREC>::S S;
-    const unsigned nb = plane_blocks(p);
-    const int last = h->g.n[p.dim] - 1;
-    hipLaunchKernelGGL(k_pp_adj_add<PREC>, dim3(nb), dim3(256), 0, h->stream, (S*)lam, p, 0,
-                       (const double*)(h->pp.buf + 4));
-    hipLaunchKernelGGL(k_pp_adj_add<PREC>, dim3(nb), dim3(256), 0, h->stream, (S*)lam, p, last,
-                       (const double*)(h->pp.buf + 5));
-}
-
 int periodic_pressure_adjoint_pre(lbdem_psm* h, const void* fin, const void* gplanes, void* lam) {
     if (!h->pp.enabled) return LBDEM_PSM_OK;
     const PlaneArgs p = plane_args(h);
@@ -434,16 +387,6 @@ int periodic_pressure_adjoint_pre(lbdem_psm* h, const void* fin, const void* gpl
     return LBDEM_PSM_OK;
 }
 
-int periodic_pressure_adjoint_post(lbdem_psm* h, void* lam) {
-    if (!h->pp.enabled) return LBDEM_PSM_OK;
-    const PlaneArgs p = plane_args(h);
-    switch (h->precision) {
-        case LBDEM_PSM_FP64: pp_adj_post_t<LBDEM_PSM_FP64>(h, p, lam); break;
-        default: pp_adj_post_t<LBDEM_PSM_FP32_DEV>(h, p, lam); break;
-    }
-    PSM_HIP(h, hipGetLastError());
-    return LBDEM_PSM_OK;
-}
 
 /* PoiseuilleProfileAndPressureGradient::operator() and poiseuilleVelocity
  * (src/periodicPressureFunctionals3D.hh:272-298), then the equilibrium at

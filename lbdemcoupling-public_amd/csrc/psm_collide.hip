@@ -223,10 +223,15 @@ int launch_collide_stream(lbdem_psm* h) {
 int launch_collide_stream_buf(lbdem_psm* h, const void* fin, void* fout, void* rF) {
     CollideArgs a = make_args(h);
     a.rF = rF;
-    int rc = periodic_pressure_pre_buf(h, fin);
+    /* the plane sums of fin from the collide's own loads, as in
+     * launch_collide_stream (the same sums, bit for bit) */
+    int rc = periodic_pressure_cells(h, &a.plane_rho, &a.pdim, &a.pa0, &a.pa1, &a.pb0, &a.pb1, &a.pnb, &a.pncell);
     if (rc) return rc;
+    a.plast = h->g.n[a.pdim] - 1;
     launch_any(h, a, fin, fout);
     PSM_HIP(h, hipGetLastError());
+    rc = periodic_pressure_sum_cells(h);
+    if (rc) return rc;
     return periodic_pressure_post_buf(h, fout);
 }
 

@@ -635,19 +635,17 @@ int periodic_pressure_cells(lbdem_psm* h, double** plane_rho, int* dim, int* a0,
                             int64_t* ncell);
 int periodic_pressure_sum_cells(lbdem_psm* h);
 int diff_free(lbdem_psm* h);
-int periodic_pressure_pre(lbdem_psm* h);
 int launch_macroscopic(lbdem_psm* h, double* d_rho, double* d_u);
 int periodic_pressure_post(lbdem_psm* h);
 /* the same on caller-chosen population buffers (the differentiable window) */
-int periodic_pressure_pre_buf(lbdem_psm* h, const void* f);
 int periodic_pressure_post_buf(lbdem_psm* h, void* f);
-/* adjoint of the rescale: before the collide+stream adjoint (scales lam =
- * dJ/df(t+1) in place, gplanes = compact planes of f(t+1)) and after it
- * (adds the plane-mean terms to lam = dJ/df(t)) */
+/* adjoint of the rescale, before the collide+stream adjoint: scales lam =
+ * dJ/df(t+1) in place (gplanes = compact planes of f(t+1)) and leaves the
+ * plane-mean terms that the collide+stream adjoint adds to dJ/df(t) in
+ * pp.buf[4], pp.buf[5] */
 int periodic_pressure_extract(lbdem_psm* h, const void* f, void* out);
 int64_t periodic_pressure_plane_elems(const lbdem_psm* h);
 int periodic_pressure_adjoint_pre(lbdem_psm* h, const void* fin, const void* gplanes, void* lam);
-int periodic_pressure_adjoint_post(lbdem_psm* h, void* lam);
 int launch_conservation(lbdem_psm* h, double* mass, double* mom3, double* max_u,
                         int64_t* n_nonfinite);
 int run_voxelize(lbdem_psm* h);
