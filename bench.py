@@ -354,6 +354,58 @@ def device_memory_gb():
         return None
 
 
+def time_adjoint_c5(spheres, xw, markers=False):
+    """The timed part of the 512x512x768 adjoint line: a warm-up window (the
+    state pool and adjoint buffers), 150 coupled steps, the 50-step window
+    forward and backward (d(total z-drag)), then 50 plain steps of the same
+    frozen geometry.  Returns (forward s, backward s, plain s, forces,
+    gradients, device memory, state GB).  markers: one untimed copy of the
+    population store (k_copy16, known size) before the forward, the backward
+    and the plain steps, to split a kernel trace into the three phases."""
+    wl, c, pr = WORKLOADS["channel512"], ADJOINT_C5, WORKLOADS["channel512"]["pressure"]
+    x, v, w, r, ids = spheres
+    lat = new_lattice(wl, "fp32_dev", voxel_mode="overlap")
+    try:
+        lat.set_spheres(x, v, w, r, ids)
+        # warm-up of both passes at the same budget (one segment: all M states):
+        # allocates the state pool and the adjoint buffers, which the timed
+        # window reuses; then the same start as objective() from a fresh history
+        lat.diff_forward(c["budget"], -c["budget"])
+        lat.diff_backward(np.zeros((c["budget"], len(r), 6)), want_cells=False)
+        lat.clear_records()
+        lat.init_poiseuille(pr["dim"], pr["rho_hi"], pr["rho_lo"], pr["u_max"])
+        lat.set_spheres(x, v, w, r, ids)
+        lat.step(c["pre_steps"], advance=True)
+        lat.set_spheres(xw, v, w, r, ids)
+        if markers:
+            lat.copy_bandwidth(1)
+        lat.synchronize()
+        t0 = time.perf_counter()
+        ft = lat.diff_forward(c["window"], -c["budget"])
+        t1 = time.perf_counter()
+        G = np.zeros_like(ft)
+        G[:, :, 2] = 1.0
+        if markers:
+            lat.copy_bandwidth(1)
+            lat.synchronize()
+        t1b = time.perf_counter()
+        g = lat.diff_backward(G, want_cells=False)
+        t2 = time.perf_counter()
+        mem = device_memory_gb()
+        lat.diff_release()
+        if markers:
+            lat.copy_bandwidth(1)
+        lat.synchronize()
+        t3 = time.perf_counter()
+        lat.step(c["window"], frozen=True)
+        lat.synchronize()
+        t4 = time.perf_counter()
+        state_gb = lat.nc * 19 * 4 / 1e9
+    finally:
+        lat.close()
+    return t1 - t0, t2 - t1b, t4 - t3, ft, g, mem, state_gb
+
+
 def run_adjoint_c5(spheres):
     """Adjoint of the last 50 steps of the 512x512x768 channel (FP32)."""
     wl, c, pr = WORKLOADS["channel512"], ADJOINT_C5, WORKLOADS["channel512"]["pressure"]
@@ -378,37 +430,7 @@ def run_adjoint_c5(spheres):
             lat.close()
         return float(ft[:, :, 2].sum())
 
-    lat = new_lattice(wl, "fp32_dev", voxel_mode="overlap")
-    try:
-        lat.set_spheres(x, v, w, r, ids)
-        # warm-up of both passes at the same budget (one segment: all M states):
-        # allocates the state pool and the adjoint buffers, which the timed
-        # window reuses; then the same start as objective() from a fresh history
-        lat.diff_forward(c["budget"], -c["budget"])
-        lat.diff_backward(np.zeros((c["budget"], len(r), 6)), want_cells=False)
-        lat.clear_records()
-        lat.init_poiseuille(pr["dim"], pr["rho_hi"], pr["rho_lo"], pr["u_max"])
-        lat.set_spheres(x, v, w, r, ids)
-        lat.step(c["pre_steps"], advance=True)
-        lat.set_spheres(xw, v, w, r, ids)
-        lat.synchronize()
-        t0 = time.perf_counter()
-        ft = lat.diff_forward(c["window"], -c["budget"])
-        t1 = time.perf_counter()
-        G = np.zeros_like(ft)
-        G[:, :, 2] = 1.0
-        g = lat.diff_backward(G, want_cells=False)
-        t2 = time.perf_counter()
-        mem = device_memory_gb()
-        lat.diff_release()
-        lat.synchronize()
-        t3 = time.perf_counter()
-        lat.step(c["window"], frozen=True)
-        lat.synchronize()
-        t4 = time.perf_counter()
-        state_gb = lat.nc * 19 * 4 / 1e9
-    finally:
-        lat.close()
+    fwd, bwd, plain, ft, g, mem, state_gb = time_adjoint_c5(spheres, xw)
     fd = {}
     for name, kw, eps, pred in (("tau", "dtau", 1e-4, g["tau"]),
                                 ("all_v_z", "dvz", 1e-4, float(np.sum(g["v"][:, 2]))),
@@ -418,7 +440,6 @@ def run_adjoint_c5(spheres):
         fdv = (jp - jm) / (2 * eps)
         fd[name] = {"adjoint": pred, "central_difference": fdv, "eps": eps, "J_plus": jp, "J_minus": jm,
                     "rel_err": abs(pred - fdv) / max(abs(fdv), 1e-300)}
-    fwd, bwd, plain = t1 - t0, t2 - t1, t4 - t3
     return {"workload": c["desc"], "forward_s": round(fwd, 4), "backward_s": round(bwd, 4),
             "plain_forward_s": round(plain, 4), "backward_over_forward": round(bwd / fwd, 3),
             "forward_over_plain_forward": round(fwd / plain, 3),
@@ -699,6 +720,81 @@ def summarize_adjoint_trace(d):
             f.write(f"| {_short(name)} | {n} | {tot:.1f} | {avg:.1f} | {pct:.2f} |\n")
 
 
+ADJ_C5_TRACE_CMD = ("rocprofv3 --kernel-trace --stats -d DIR/trace_adjoint_channel512 -o run -- "
+                    "python3 bench.py --adjoint-window-only")
+ADJ_C5_PMC_CMD = ("rocprofv3 --pmc {ctr} --output-format csv -d DIR/{tag}_adjoint_channel512 -o run -- "
+                  "python3 bench.py --adjoint-window-only --probe-out DIR")
+
+
+def summarize_adjoint_c5(d):
+    """Kernel trace and HBM counters of --adjoint-window-only ->
+    profiles/kernels_adjoint_channel512_trace.md: per phase (the marker
+    copies k_copy16 split forward, backward and plain steps) the kernels'
+    time, and bytes per dispatch of the backward's kernels."""
+    import csv
+    import glob
+    import sqlite3
+    dbs = glob.glob(os.path.join(d, "trace_adjoint_channel512", "*results.db"))
+    if not dbs:
+        return
+    con = sqlite3.connect(dbs[0])
+    rows = list(con.execute("select name, start, end from kernels order by start"))
+    marks = [i for i, r in enumerate(rows) if "k_copy16" in r[0]]
+    if len(marks) < 6:
+        return
+    # each marker is copy_bandwidth(1): a warm-up copy and a timed one
+    phases = (("forward (50 steps, budget 10)", marks[1] + 1, marks[2]),
+              ("backward", marks[3] + 1, marks[4]),
+              ("plain frozen steps (50)", marks[5] + 1, len(rows)))
+    c = ADJOINT_C5
+    lines = [f"# Kernel durations, {c['desc'].split(', d(total')[0]}: the 50-step window\n",
+             "Command: `" + ADJ_C5_TRACE_CMD + "` (rocprofv3 kernel trace, ROCm 7.2, one MI355X); counters: `" +
+             ADJ_C5_PMC_CMD.format(ctr="FETCH_SIZE", tag="fetch") + "`, then WRITE_SIZE (tag write).",
+             "The run: a warm-up window (state pool, adjoint buffers), 150 coupled steps, then the window's "
+             "forward, its backward (d(total z-drag)) and 50 plain steps of the same frozen geometry, with a "
+             "copy of the population store (k_copy16) between the phases to mark them.\n"]
+    for label, a, b in phases:
+        acc = {}
+        for name, s, e in rows[a:b]:
+            k = _short(name)
+            v = acc.setdefault(k, [0, 0.0])
+            v[0] += 1
+            v[1] += (e - s) / 1e6
+        span = (rows[b - 1][2] - rows[a][1]) / 1e6
+        tot = sum(v[1] for v in acc.values())
+        lines += [f"## {label}: {span:.1f} ms from first to last kernel, kernels {tot:.1f} ms\n",
+                  "| kernel | calls | total ms | average ms |", "|---|---|---|---|"]
+        for k, v in sorted(acc.items(), key=lambda kv: -kv[1][1])[:10]:
+            lines.append(f"| {k} | {v[0]} | {v[1]:.2f} | {v[1] / v[0]:.3f} |")
+        lines.append("")
+    res = {}
+    for tag, ctr in (("fetch", "FETCH_SIZE"), ("write", "WRITE_SIZE")):
+        path = os.path.join(d, f"{tag}_adjoint_channel512", "run_counter_collection.csv")
+        if not os.path.exists(path):
+            continue
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] == ctr:
+                res.setdefault(_short(r["Kernel_Name"]), {}).setdefault(ctr, []).append(float(r["Counter_Value"]))
+    try:
+        store = int(open(os.path.join(d, "probe_adjoint_channel512.txt")).read().split()[0])
+    except (OSError, ValueError, IndexError):
+        store = None
+    cp = res.get("k_copy16", {})
+    if store and cp.get("FETCH_SIZE") and cp.get("WRITE_SIZE"):
+        ff = store / (np.mean(cp["FETCH_SIZE"]) * 1024.0)
+        wf = store / (np.mean(cp["WRITE_SIZE"]) * 1024.0)
+        lines += [f"## HBM bytes per dispatch (scaled by the marker copy of the FP32 store, {store} B each way: "
+                  f"FETCH_SIZE x {ff:.4f}, WRITE_SIZE x {wf:.4f})\n",
+                  "| kernel | dispatches | read GB | written GB |", "|---|---|---|---|"]
+        for k, v in sorted(res.items(), key=lambda kv: -np.mean(kv[1].get("FETCH_SIZE", [0]))):
+            if "FETCH_SIZE" not in v or np.mean(v["FETCH_SIZE"]) * 1024 * ff < 1e8:
+                continue
+            lines.append(f"| {k} | {len(v['FETCH_SIZE'])} | {np.mean(v['FETCH_SIZE']) * 1024 * ff / 1e9:.2f} | "
+                         f"{np.mean(v.get('WRITE_SIZE', [0])) * 1024 * wf / 1e9:.2f} |")
+    with open(os.path.join(ROOT, "profiles", "kernels_adjoint_channel512_trace.md"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def kernel_probe(key, out_dir=None):
     """Short run for hardware-counter profiles: the copy k_copy16 of the FP64
     population store (known size, for calibration), then 3 coupled steps per
@@ -733,6 +829,11 @@ def main():
     ap.add_argument("--adjoint-only", action="store_true",
                     help="only the 128x128x256 differentiable window, FP64 and FP32-dev (for its kernel trace); "
                          "with --summarize-profiles DIR: summarize DIR/trace_adjoint128")
+    ap.add_argument("--adjoint-window-only", action="store_true",
+                    help="only the timed 512x512x768 adjoint window (forward, backward, plain steps; no central "
+                         "differences), with a marker copy between the phases (for its kernel trace and "
+                         "counters); with --summarize-profiles DIR: summarize DIR/trace_adjoint_channel512, "
+                         "DIR/fetch_adjoint_channel512 and DIR/write_adjoint_channel512")
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--summarize-profiles", metavar="DIR",
                     help="turn rocprofv3 outputs under DIR (trace_<wl>/, fetch_<wl>/, write_<wl>/, "
@@ -744,7 +845,9 @@ def main():
                     help="where --kernel-probe writes the calibration copy's size")
     args = ap.parse_args()
     if args.summarize_profiles:
-        if args.adjoint_only:
+        if args.adjoint_window_only:
+            summarize_adjoint_c5(args.summarize_profiles)
+        elif args.adjoint_only:
             summarize_adjoint_trace(args.summarize_profiles)
         else:
             summarize_profiles(args.summarize_profiles, args.workload)
@@ -755,6 +858,19 @@ def main():
         sys.exit(2)
     if args.kernel_probe:
         kernel_probe(args.workload, args.probe_out)
+        return
+    if args.adjoint_window_only:
+        wl, c = WORKLOADS["channel512"], ADJOINT_C5
+        sph = make_spheres(wl)
+        xw = np.array(sph[0], dtype=np.float64)
+        for _ in range(c["pre_steps"]):
+            xw = L.advance_positions(xw, sph[1], wl["n"], wl["periodic"])
+        fwd, bwd, plain, _, _, mem, state_gb = time_adjoint_c5(sph, xw, markers=True)
+        os.makedirs(args.probe_out, exist_ok=True)
+        with open(os.path.join(args.probe_out, "probe_adjoint_channel512.txt"), "w") as f:
+            f.write(f"{int(round(state_gb * 1e9 / 4)) * 4}\n")
+        print(json.dumps({"workload": c["desc"], "forward_s": round(fwd, 4), "backward_s": round(bwd, 4),
+                          "plain_forward_s": round(plain, 4), "device_memory_used_after_backward_gb": mem[0] if mem else None}))
         return
     if args.adjoint_only:
         adj = {}
