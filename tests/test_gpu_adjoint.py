@@ -151,6 +151,41 @@ def test_adjoint_checkpoint_interval_invariance():
     g.diff_release()
 
 
+@pytest.mark.parametrize("K", [3, -4])
+def test_window_forward_leaves_the_lattice_as_plain_steps(K):
+    """The window's forward keeps each segment's checkpoint by taking over
+    the buffer its first step read: afterwards the lattice holds exactly the
+    state, forces and records of the same steps run without a window, and it
+    steps on identically; the checkpoints still reproduce the gradients."""
+    nsteps = 7
+    f0 = initial_populations(CASE["n"])
+    x, v, w, r, ids = spheres()
+    a, b = gpu_lattice("overlap"), gpu_lattice("overlap")
+    try:
+        for lat in (a, b):
+            lat.set_populations(f0)
+            lat.set_spheres(x, v, w, r, ids)
+        ft = a.diff_forward(nsteps, K)
+        b.step(nsteps, frozen=True)
+        assert np.array_equal(a.get_populations(), b.get_populations())
+        Fb, Tb = b.particle_forces()
+        assert np.array_equal(ft[-1][:, :3], Fb) and np.array_equal(ft[-1][:, 3:], Tb)
+        G = np.random.default_rng(5).normal(size=(nsteps, 3, 6))
+        g1 = a.diff_backward(G, want_f0=True)
+        for lat in (a, b):
+            lat.step(3, frozen=True)
+        assert np.array_equal(a.get_populations(), b.get_populations())
+        # a second window from the stepped state reuses the state pool
+        a.set_populations(f0)
+        a.diff_forward(nsteps, K)
+        g2 = a.diff_backward(G, want_f0=True)
+        for k in ("x", "v", "w", "sf", "B", "f0"):
+            assert np.array_equal(g1[k], g2[k]), k
+    finally:
+        a.close()
+        b.close()
+
+
 def test_adjoint_state_budget_too_small():
     x, v, w, r, ids = spheres()
     g = gpu_lattice("overlap")
