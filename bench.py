@@ -817,9 +817,10 @@ def kernel_probe(key, out_dir=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--repeats", type=int, default=3,
+    # the measurement protocol: >= 20 warm-up and >= 100 timed steps, median of 5 fresh lattices
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--repeats", type=int, default=5,
                     help="fresh lattices per precision of the headline workload; value is the median")
     ap.add_argument("--workload", default="channel512", choices=sorted(k for k in WORKLOADS if k != "channel512_slab"))
     ap.add_argument("--only-headline", action="store_true",
