@@ -465,6 +465,46 @@ def test_periodic_pressure_channel_matches_oracle():
     assert rel_err(g.get_populations(), o.get_populations()) < 1e-12
 
 
+@pytest.mark.parametrize("precision,pop_tol,force_tol", [("fp32_dev", 1e-5, 2e-5),
+                                                         ("fp16_store", 5e-3, 2e-3)])
+@pytest.mark.parametrize("driver", ["body", "pressure"])
+def test_walled_channel_precisions_track_oracle(precision, pop_tol, force_tol, driver):
+    """The reduced-precision lattices with walls on y and z (half-way
+    bounce-back) and moving spheres against the FP64 oracle: driven by a body
+    force, or by the Zhang-Kwok periodic pressure on x. Populations and
+    velocity are compared on the scale of their largest deviation from rest.
+    Measured: FP32 within 2e-6 (populations, velocity) and 4e-6 (forces); FP16
+    storage within 1.2e-3 and 1.4e-4."""
+    n = (24, 18, 20)
+    body = (2e-6, 0.0, 0.0) if driver == "body" else (0.0, 0.0, 0.0)
+    g, o = make_pair(n, periodic=(1, 0, 0), tau=0.6, N=5, body=body, precision=precision)
+    if driver == "pressure":
+        for lat in (g, o):
+            lat.init_poiseuille(0, 1.001, 0.999, 0.02, n3=(n[0] - 1, n[1], n[2]))
+            lat.set_periodic_pressure(0, 1.001, 0.999)
+    x = np.array([[8.2, 9.1, 10.3], [17.6, 7.4, 9.8], [3.1, 13.0, 4.2]])
+    v = np.array([[0.004, 0.0, 0.001], [-0.002, 0.003, 0.0], [0.0, -0.001, 0.002]])
+    w = np.array([[0.0, 0.0, 0.001], [0.0, 0.0, 0.0], [0.002, 0.0, 0.0]])
+    r = np.array([3.5, 3.0, 2.6])
+    ids = np.array([1, 2, 3])
+    g.set_spheres(x, v, w, r, ids)
+    o.set_spheres(x, v, w, r, ids)
+    g.collide_stream(16)
+    o.collide_stream(16, 1)
+    Fg, Tg = g.sum_force_torque(x)
+    Fo, To = o.sum_force_torque(x)
+    fg, fo = g.get_populations(), o.get_populations()
+    _, ug = L.macroscopic(fg)
+    _, uo = L.macroscopic(fo)
+    print(precision, driver, "pop", rel_err(fg, fo), "u", rel_err(ug, uo),
+          "F", rel_err(Fg, Fo), "T", torque_err(Tg, To, Fo))
+    assert rel_err(fg, fo) < pop_tol
+    assert rel_err(ug, uo) < pop_tol
+    assert rel_err(Fg, Fo) < force_tol
+    assert torque_err(Tg, To, Fo) < force_tol
+    g.close()
+
+
 # cross-section boxes (a0, a1, b0, b1) over the two other axes in increasing
 # order, touching a lattice edge on one side and not the other
 BOXES = {0: (0, 9, 3, 15), 1: (2, 13, 0, 11), 2: (0, 13, 1, 6)}
