@@ -182,12 +182,21 @@ void launch_cs_prec<LBDEM_PSM_FP64>(lbdem_psm* h, const CollideArgs& a, const vo
     S* fout = (S*)vout;
     const int wgz = collide_wg(a.L.zc);
     dim3 grid((unsigned)((int64_t)a.nx * a.ny * a.L.nzc * (a.L.zc / wgz))), block((unsigned)wgz);
+    if (!h->cs_lds_ready) {
+        const int dev = h->desc.device, cap = collide_wave_cap(LBDEM_PSM_FP64);
+        h->cs_lds[1][1] = collide_lds_for(k_collide_stream<LBDEM_PSM_FP64, true, true>, dev, wgz, cap);
+        h->cs_lds[1][0] = collide_lds_for(k_collide_stream<LBDEM_PSM_FP64, true, false>, dev, wgz, cap);
+        h->cs_lds[0][1] = collide_lds_for(k_collide_stream<LBDEM_PSM_FP64, false, true>, dev, wgz, cap);
+        h->cs_lds[0][0] = collide_lds_for(k_collide_stream<LBDEM_PSM_FP64, false, false>, dev, wgz, cap);
+        h->cs_lds_ready = true;
+    }
+    const unsigned lds = h->cs_lds[h->has_force ? 1 : 0][a.pz ? 1 : 0];
     if (h->has_force) {
-        if (a.pz) hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, true>), grid, block, 0, h->stream, fin, fout, a);
-        else hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, false>), grid, block, 0, h->stream, fin, fout, a);
+        if (a.pz) hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, true>), grid, block, lds, h->stream, fin, fout, a);
+        else hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, true, false>), grid, block, lds, h->stream, fin, fout, a);
     } else {
-        if (a.pz) hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, false, true>), grid, block, 0, h->stream, fin, fout, a);
-        else hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, false, false>), grid, block, 0, h->stream, fin, fout, a);
+        if (a.pz) hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, false, true>), grid, block, lds, h->stream, fin, fout, a);
+        else hipLaunchKernelGGL((k_collide_stream<LBDEM_PSM_FP64, false, false>), grid, block, lds, h->stream, fin, fout, a);
     }
 }
 

@@ -77,6 +77,52 @@ __device__ __forceinline__ void plane_rhobar(const CollideArgs& a, const T* fp, 
  * zc / COLLIDE_WG workgroups (one when zc is smaller). */
 inline int collide_wg(int zc) { return zc < COLLIDE_WG ? zc : COLLIDE_WG; }
 
+/* Resident wavefronts per SIMD the collide is held to (0: as many as its
+ * registers allow).  Fewer wavefronts in flight than the registers allow
+ * moved the store faster: on MI355X, stepping one lattice alternately capped
+ * and uncapped (512x512x768 channel, 512^3 and 256^3 boxes, the channel's
+ * 64x512x768 slab), three per SIMD made the FP64 collide 1-3 % faster than
+ * its register limit of four and the FP32 collide 4-8 % faster than its
+ * eight; FP16 storage ran fastest uncapped (seven; four cost 13 %). */
+inline int collide_wave_cap(int precision) { return precision == LBDEM_PSM_FP16_STORE ? 0 : 3; }
+
+/* Dynamic LDS per workgroup that holds a kernel with wgz-lane workgroups at
+ * `waves` wavefronts per SIMD on a CU of lds_cu bytes of LDS and four SIMDs.
+ * The collide uses no LDS: the reservation only limits how many of its
+ * workgroups share a CU.  The size is taken from the middle of the range
+ * that fits exactly that many workgroups, clear of both ends, because the
+ * hardware rounds each allocation up: on MI355X (160 KiB per CU) 54 272
+ * bytes let only two workgroups in, not three, while 41 984 to 53 248 let
+ * three.  0 when the cap cannot be expressed. */
+inline unsigned collide_cap_lds(int lds_cu, int wgz, int waves) {
+    const int wg_waves = (wgz + 63) / 64;
+    if (waves <= 0 || lds_cu <= 0 || (4 * waves) % wg_waves != 0) return 0;
+    const unsigned k = (unsigned)(4 * waves / wg_waves), cu = (unsigned)lds_cu;
+    const unsigned bytes = ((cu / (k + 1) + cu / k) / 2) & ~511u;
+    const unsigned slack = 4096;   /* above the allocation granularity */
+    return (k + 1) * bytes > cu && k * (bytes + slack) <= cu ? bytes : 0;
+}
+
+/* The cap's LDS for one collide instance, used only when the runtime's
+ * occupancy for the kernel confirms it: exactly the capped number of
+ * workgroups per CU, fewer than its registers allow. */
+template <typename K>
+unsigned collide_lds_for(K kernel, int device, int wgz, int waves) {
+    int lds_cu = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess)
+        return 0;
+    const unsigned bytes = collide_cap_lds(lds_cu, wgz, waves);
+    if (!bytes) return 0;
+    int uncapped = 0, capped = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&uncapped, reinterpret_cast<const void*>(kernel), wgz, 0) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&capped, reinterpret_cast<const void*>(kernel), wgz, bytes) !=
+            hipSuccess)
+        return 0;
+    const int per_cu = 4 * waves / ((wgz + 63) / 64);
+    return capped == per_cu && capped < uncapped ? bytes : 0;
+}
+
 /* Equilibria of a population pair (i, i+9): c_{i+9} = -c_i, so the pair
  * shares c.j, (3 c.j)(c.j) - |j|^2 and its product with 1.5/rho; each value
  * is bit-identical to equilibrium(i, ...) / equilibrium(i+9, ...). */

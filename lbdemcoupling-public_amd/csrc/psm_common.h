@@ -504,6 +504,10 @@ struct lbdem_psm {
     hipStream_t stream = nullptr;
     void* f[2] = {nullptr, nullptr};
     int cur_f = 0;
+    /* dynamic LDS of the collide launches [body force][periodic z] that caps
+     * their occupancy (collide_wave_cap), set at the first launch */
+    unsigned cs_lds[2][2] = {{0u, 0u}, {0u, 0u}};
+    bool cs_lds_ready = false;
     uint8_t* sflag = nullptr;        /* per cell: 1 when the cell holds a coupling record */
     psm::Records rec;                /* dense, bin-permuted, updated in place */
     int64_t rec_cap = 0;             /* one slot per cell */
