@@ -382,13 +382,18 @@ __global__ __launch_bounds__(256) void k_copy16(const u32x4* __restrict__ src, u
     if (k < n16) __builtin_nontemporal_store(__builtin_nontemporal_load(src + k), dst + k);
 }
 
+/* Wavefronts per SIMD the copy is held to: six moved 0.7-3 % more than its
+ * register limit of eight (512x512x768 stores, FP64 and FP32, three lattices
+ * each); five or fewer moved less (four: 5-8 % less). */
+constexpr int COPY_WAVES = 6;
+
 /* The copy as one-pass launches of at most 2^31 words each (the work-item
  * count of a launch is a 32-bit number). */
-static hipError_t copy16_launch(hipStream_t st, const u32x4* src, u32x4* dst, int64_t n16) {
+static hipError_t copy16_launch(hipStream_t st, const u32x4* src, u32x4* dst, int64_t n16, unsigned lds) {
     const int64_t seg = (int64_t)1 << 31;
     for (int64_t o = 0; o < n16; o += seg) {
         const int64_t m = std::min<int64_t>(seg, n16 - o);
-        hipLaunchKernelGGL(k_copy16, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, src + o, dst + o, m);
+        hipLaunchKernelGGL(k_copy16, dim3((unsigned)((m + 255) / 256)), dim3(256), lds, st, src + o, dst + o, m);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -430,9 +435,10 @@ int lbdem_psm_copy_bandwidth(lbdem_psm* h, int32_t reps, double* ms_per_copy, do
     PSM_HIP(h, hipEventCreate(&e0));
     hipError_t e = hipEventCreate(&e1);
     if (e != hipSuccess) { (void)hipEventDestroy(e0); return hip_check(h, e, "hipEventCreate"); }
-    e = copy16_launch(h->stream, src, dst, n16);   /* warm-up */
+    const unsigned lds = collide_lds_for(k_copy16, h->desc.device, 256, COPY_WAVES);
+    e = copy16_launch(h->stream, src, dst, n16, lds);   /* warm-up */
     if (e == hipSuccess) e = hipEventRecord(e0, h->stream);
-    for (int r = 0; r < reps && e == hipSuccess; ++r) e = copy16_launch(h->stream, src, dst, n16);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = copy16_launch(h->stream, src, dst, n16, lds);
     if (e == hipSuccess) e = hipEventRecord(e1, h->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.0f;

@@ -99,13 +99,13 @@ inline unsigned collide_cap_lds(int lds_cu, int wgz, int waves) {
     if (waves <= 0 || lds_cu <= 0 || (4 * waves) % wg_waves != 0) return 0;
     const unsigned k = (unsigned)(4 * waves / wg_waves), cu = (unsigned)lds_cu;
     const unsigned bytes = ((cu / (k + 1) + cu / k) / 2) & ~511u;
-    const unsigned slack = 4096;   /* above the allocation granularity */
+    const unsigned slack = 2048;   /* the allocation granularity or above */
     return (k + 1) * bytes > cu && k * (bytes + slack) <= cu ? bytes : 0;
 }
 
-/* The cap's LDS for one collide instance, used only when the runtime's
- * occupancy for the kernel confirms it: exactly the capped number of
- * workgroups per CU, fewer than its registers allow. */
+/* The cap's LDS for one kernel (a collide instance, the device copy), used
+ * only when the runtime's occupancy for the kernel confirms it: exactly the
+ * capped number of workgroups per CU, fewer than its registers allow. */
 template <typename K>
 unsigned collide_lds_for(K kernel, int device, int wgz, int waves) {
     int lds_cu = 0;
