@@ -416,11 +416,9 @@ __global__ __launch_bounds__(256) void k_collide_stream(const typename Store<PRE
     S raw[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const S* src = (const S*)(frow + i * qstride + zoff);
-        /* FP64 populations are read once: streaming loads measured faster
-         * there (and slower for the narrower stores) */
-        if constexpr (PREC == LBDEM_PSM_FP64) raw[i] = __builtin_nontemporal_load(src);
-        else raw[i] = *src;
+        /* plain loads: at three wavefronts per SIMD streaming loads were
+         * 0-2 % slower for FP64 and 1-2 % slower for FP32 / FP16 */
+        raw[i] = *(const S*)(frow + i * qstride + zoff);
     }
     __builtin_amdgcn_sched_barrier(0);
     const bool solid = flag != 0;
