@@ -236,52 +236,72 @@ __global__ __launch_bounds__(256) void k_cand_gather(VoxArgs a, const uint32_t* 
                                                      const int32_t* __restrict__ bin_count,
                                                      const int32_t* __restrict__ cand_bin,
                                                      int64_t nbins, Cand* __restrict__ out, int64_t cap) {
+    /* Each workgroup forms 256 consecutive records in LDS and writes them
+     * out as contiguous 16-byte words: a record per lane stored straight
+     * from registers is 17 eight-byte stores at a 136-byte stride, one
+     * partial cache line each, and those line writes bounded the kernel. */
+    __shared__ __align__(16) Cand tile[256];
     const int64_t nent = nbins > 0 ? (int64_t)bin_start[nbins - 1] + bin_count[nbins - 1] : 0;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nent && e < cap;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const uint32_t kk = key[e];
-        const int q = (int)(kk & QMASK);
-        const int rank = (int)(kk >> QBITS);
-        const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
-        const double r = a.pr[q];
-        double xi[3];
-        int lo[3], hi[3];
-        Cand cd;
-        if (!image_box(x, r, rank, a, lo, hi, xi)) {
-            lo[0] = lo[1] = lo[2] = 1;
-            hi[0] = hi[1] = hi[2] = 0;
-            xi[0] = xi[1] = xi[2] = 0.0;
-        }
-        for (int d = 0; d < 3; ++d) {
-            cd.x[d] = xi[d];
-            cd.v[d] = pv[3 * q + d]; cd.w[d] = pw[3 * q + d];
-        }
-        /* the box clipped to the entry's bin, as masks over the bin's cells */
-        {
-            const int b = cand_bin[e];
-            const int bz = b % a.nb[2], by = (b / a.nb[2]) % a.nb[1], bx = b / (a.nb[2] * a.nb[1]);
-            const int o[3] = {bx * BIN, by * BIN, bz * BIN};
-            uint32_t m[3];
-            for (int d = 0; d < 3; ++d) {
-                const int s = lo[d] - o[d] > 0 ? lo[d] - o[d] : 0;
-                const int t = hi[d] - o[d] < BIN - 1 ? hi[d] - o[d] : BIN - 1;
-                m[d] = s <= t ? ((2u << t) - 1u) & ~((1u << s) - 1u) : 0u;
+    const int64_t nlim = nent < cap ? nent : cap;
+    const int t = (int)threadIdx.x;
+    for (int64_t e0 = (int64_t)blockIdx.x * 256; e0 < nlim; e0 += (int64_t)gridDim.x * 256) {
+        const int64_t e = e0 + t;
+        if (e < nlim) {
+            const uint32_t kk = key[e];
+            const int q = (int)(kk & QMASK);
+            const int rank = (int)(kk >> QBITS);
+            const double x[3] = {a.px[3 * q], a.px[3 * q + 1], a.px[3 * q + 2]};
+            const double r = a.pr[q];
+            double xi[3];
+            int lo[3], hi[3];
+            Cand cd;
+            if (!image_box(x, r, rank, a, lo, hi, xi)) {
+                lo[0] = lo[1] = lo[2] = 1;
+                hi[0] = hi[1] = hi[2] = 0;
+                xi[0] = xi[1] = xi[2] = 0.0;
             }
-            uint64_t yz = 0;
-            for (int yl = 0; yl < BIN; ++yl)
-                if ((m[1] >> yl) & 1u) yz |= (uint64_t)m[2] << (BIN * yl);
-            cd.xmask = m[0];
-            cd.yzmask = yz;
+            for (int d = 0; d < 3; ++d) {
+                cd.x[d] = xi[d];
+                cd.v[d] = pv[3 * q + d]; cd.w[d] = pw[3 * q + d];
+            }
+            /* the box clipped to the entry's bin, as masks over the bin's cells */
+            {
+                const int b = cand_bin[e];
+                const int bz = b % a.nb[2], by = (b / a.nb[2]) % a.nb[1], bx = b / (a.nb[2] * a.nb[1]);
+                const int o[3] = {bx * BIN, by * BIN, bz * BIN};
+                uint32_t m[3];
+                for (int d = 0; d < 3; ++d) {
+                    const int s0 = lo[d] - o[d] > 0 ? lo[d] - o[d] : 0;
+                    const int s1 = hi[d] - o[d] < BIN - 1 ? hi[d] - o[d] : BIN - 1;
+                    m[d] = s0 <= s1 ? ((2u << s1) - 1u) & ~((1u << s0) - 1u) : 0u;
+                }
+                uint64_t yz = 0;
+                for (int yl = 0; yl < BIN; ++yl)
+                    if ((m[1] >> yl) & 1u) yz |= (uint64_t)m[2] << (BIN * yl);
+                cd.xmask = m[0];
+                cd.yzmask = yz;
+            }
+            cd.r = r;
+            const double r_p = r + h;
+            const double r_m = r - h;
+            cd.rp2 = r_p * r_p;
+            cd.rm2 = r_m * r_m;
+            cd.rsq = r * r;
+            exact_band(r, esamp, cd.ri2, cd.ro2);
+            cd.id = pid[q];
+            tile[t] = cd;
         }
-        cd.r = r;
-        const double r_p = r + h;
-        const double r_m = r - h;
-        cd.rp2 = r_p * r_p;
-        cd.rm2 = r_m * r_m;
-        cd.rsq = r * r;
-        exact_band(r, esamp, cd.ri2, cd.ro2);
-        cd.id = pid[q];
-        out[e] = cd;
+        __syncthreads();
+        /* the workgroup's records, contiguous in out from e0 (16-byte aligned:
+         * 256 records are 34 816 bytes), as 16-byte words and an 8-byte tail */
+        const int64_t n = nlim - e0 < 256 ? nlim - e0 : 256;
+        const int64_t bytes = n * (int64_t)sizeof(Cand);
+        const uint4* src = (const uint4*)tile;
+        uint4* dst = (uint4*)(out + e0);
+        for (int64_t w = t; w < bytes / 16; w += 256) dst[w] = src[w];
+        if ((bytes & 15) != 0 && t == 0)
+            ((uint2*)(out + e0))[bytes / 8 - 1] = ((const uint2*)tile)[bytes / 8 - 1];
+        __syncthreads();
     }
 }
 
