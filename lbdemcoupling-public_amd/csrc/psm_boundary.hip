@@ -74,6 +74,10 @@ __global__ __launch_bounds__(256) void k_plane_rhobar(const typename Store<PREC>
  * thread and the same tree as k_plane_rhobar, so the sums agree to the bit. */
 __global__ __launch_bounds__(256) void k_plane_cells_partial(const double* __restrict__ cells, int64_t ncell,
                                                              double* __restrict__ part) {
+    /* plane blockIdx.y: its ncell values from cells + y * ncell, its
+     * partials to part + y * gridDim.x */
+    cells += (int64_t)blockIdx.y * ncell;
+    part += (int64_t)blockIdx.y * gridDim.x;
     __shared__ double sh[256];
     double acc = 0.0;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ncell;
@@ -88,9 +92,12 @@ __global__ __launch_bounds__(256) void k_plane_cells_partial(const double* __res
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-/* Fixed-order sum of n partials into out[0]. */
+/* Fixed-order sum of n partials into out[0]; with several workgroups,
+ * workgroup g sums part + g * n into out[g]. */
 __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__ part, int n,
                                                       double* __restrict__ out) {
+    part += (int64_t)blockIdx.x * n;
+    out += blockIdx.x;
     __shared__ double sh[256];
     double v = 0.0;
     for (int k = threadIdx.x; k < n; k += blockDim.x) v += part[k];
@@ -105,15 +112,23 @@ __global__ __launch_bounds__(256) void k_sum_partials(const double* __restrict__
 
 /* f_i <- (f_i + t_i) * rho_target / <rho> - t_i for the populations with
  * c_i[dim] == dir on the plane's box; <rho> = 1 + sum / ncell (Palabos'
- * computeAverageDensity: the mean rhoBar, then the full density). */
+ * computeAverageDensity: the mean rhoBar, then the full density).  Both
+ * planes in one launch (blockIdx.y): the inlet (index 0) rescales the
+ * populations arriving with c_d = +1 against the outlet's mean (sums[1]),
+ * the outlet (index last) those arriving with c_d = -1 against the inlet's
+ * mean (sums[0]).  The two sets of populations are disjoint, so the order
+ * between them does not matter. */
 template <int PREC>
-__global__ __launch_bounds__(256) void k_zhang_rescale(typename Store<PREC>::S* __restrict__ f, PlaneArgs p, int plane,
-                                                       int dir, double rho_target,
-                                                       const double* __restrict__ rhobar_sum) {
+__global__ __launch_bounds__(256) void k_zhang_rescale(typename Store<PREC>::S* __restrict__ f, PlaneArgs p, int last,
+                                                       double rho_in, double rho_out,
+                                                       const double* __restrict__ sums) {
     typedef typename Store<PREC>::T T;
+    const bool outlet = blockIdx.y != 0;
+    const int plane = outlet ? last : 0, dir = outlet ? -1 : 1;
+    const double rho_target = outlet ? rho_out : rho_in;
     const int na = p.a1 - p.a0 + 1, nb = p.b1 - p.b0 + 1;
     const int64_t ncell = (int64_t)na * nb;
-    const double avg = 1.0 + rhobar_sum[0] / (double)ncell;
+    const double avg = 1.0 + sums[outlet ? 0 : 1] / (double)ncell;
     const double factor = rho_target / avg;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < ncell;
          k += (int64_t)gridDim.x * blockDim.x) {
@@ -166,12 +181,9 @@ static void pp_post_t(lbdem_psm* h, const PlaneArgs& p, void* fv) {
     S* f = (S*)fv;
     const unsigned nb = plane_blocks(p);
     const int last = h->g.n[p.dim] - 1;
-    /* inlet: populations arriving with c_d = +1, against the outlet's mean */
-    hipLaunchKernelGGL(k_zhang_rescale<PREC>, dim3(nb), dim3(256), 0, h->stream, f, p, 0, 1, h->pp.rho_in,
-                       (const double*)(h->pp.buf + 1));
-    /* outlet: populations arriving with c_d = -1, against the inlet's mean */
-    hipLaunchKernelGGL(k_zhang_rescale<PREC>, dim3(nb), dim3(256), 0, h->stream, f, p, last, -1, h->pp.rho_out,
-                       (const double*)(h->pp.buf));
+    /* [0] inlet sum, [1] outlet sum */
+    hipLaunchKernelGGL(k_zhang_rescale<PREC>, dim3(nb, 2), dim3(256), 0, h->stream, f, p, last, h->pp.rho_in,
+                       h->pp.rho_out, (const double*)h->pp.buf);
 }
 
 int periodic_pressure_post_buf(lbdem_psm* h, void* f) {
@@ -212,12 +224,9 @@ int periodic_pressure_sum_cells(lbdem_psm* h) {
     const PlaneArgs p = plane_args(h);
     const int64_t n = (int64_t)(p.a1 - p.a0 + 1) * (p.b1 - p.b0 + 1);
     const unsigned nb = plane_blocks(p);
-    /* [0] inlet sum, [1] outlet sum, partials from [8] on */
-    for (int pl = 0; pl < 2; ++pl) {
-        hipLaunchKernelGGL(k_plane_cells_partial, dim3(nb), dim3(256), 0, h->stream, h->pp.cells + pl * n, n,
-                           h->pp.buf + 8);
-        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, h->stream, h->pp.buf + 8, (int)nb, h->pp.buf + pl);
-    }
+    /* [0] inlet sum, [1] outlet sum, each plane's partials from [8] on */
+    hipLaunchKernelGGL(k_plane_cells_partial, dim3(nb, 2), dim3(256), 0, h->stream, h->pp.cells, n, h->pp.buf + 8);
+    hipLaunchKernelGGL(k_sum_partials, dim3(2), dim3(256), 0, h->stream, h->pp.buf + 8, (int)nb, h->pp.buf);
     PSM_HIP(h, hipGetLastError());
     return LBDEM_PSM_OK;
 }
@@ -460,7 +469,7 @@ int lbdem_psm_set_periodic_pressure(lbdem_psm* h, int32_t enable, int32_t dimens
     }
     if (r[0] < 0 || r[1] >= h->g.n[ax] || r[0] > r[1] || r[2] < 0 || r[3] >= h->g.n[bx] || r[2] > r[3])
         return set_err(h, LBDEM_PSM_ERR_INVALID_ARG, "cross-section box outside the lattice or empty");
-    if (!h->pp.buf) PSM_HIP(h, hipMalloc((void**)&h->pp.buf, (8 + 1024) * sizeof(double)));
+    if (!h->pp.buf) PSM_HIP(h, hipMalloc((void**)&h->pp.buf, (8 + 2 * 1024) * sizeof(double)));
     h->pp.enabled = true;
     h->geom_gen += 1;   /* the step map changed: a window from before no longer applies */
     h->pp.dim = dimension;
