@@ -154,7 +154,10 @@ __global__ __launch_bounds__(64 * FORCE_WAVES) void k_force_bin(ForceArgs a) {
             const int o = a.powner[(int)(a.key[e] & QMASK)];
             xo = a.xs[3 * o]; yo = a.xs[3 * o + 1]; zo = a.xs[3 * o + 2];
         }
-        double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        /* six scalars, packed for the reduction only after the cell loop: an
+         * eight-element array here was re-zeroed through a chain of ~26 register
+         * moves per entry */
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0;
         const double dy = min_image(dy0 - yo, a.n[1]);
         const double dz = min_image(dz0 - zo, a.n[2]);
 #pragma unroll
@@ -167,14 +170,15 @@ __global__ __launch_bounds__(64 * FORCE_WAVES) void k_force_bin(ForceArgs a) {
                 asm volatile("" : "+v"(h0), "+v"(h1), "+v"(h2), "+s"(X));
                 const double dx = min_image((double)X - xo, a.n[0]);
                 const double gx = (double)h0, gy = (double)h1, gz = (double)h2;
-                acc[0] += gx;
-                acc[1] += gy;
-                acc[2] += gz;
-                acc[3] += dy * gz - dz * gy;
-                acc[4] += -dx * gz + dz * gx;
-                acc[5] += dx * gy - dy * gx;
+                s0 += gx;
+                s1 += gy;
+                s2 += gz;
+                s3 += dy * gz - dz * gy;
+                s4 += -dx * gz + dz * gx;
+                s5 += dx * gy - dy * gx;
             }
         }
+        const double acc[8] = {s0, s1, s2, s3, s4, s5, 0.0, 0.0};
         const double s = wave_sum8_scatter(acc, lane);
         const int m = lane >> 3;
         if ((lane & 7) == 0 && m < 6) a.partial[6 * (int64_t)e + m] = s;
