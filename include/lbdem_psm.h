@@ -208,8 +208,9 @@ int lbdem_psm_profile_read(lbdem_psm* h, double* ms4, int64_t* count4, int32_t r
 /* Measurement utility (no counterpart in the reference): the device copy
  * ceiling of the handle's population store.  Copies the current population
  * buffer into the second one (which holds no state between steps) reps
- * times with a plain 16-byte-per-lane copy kernel on the handle's stream,
- * after one untimed copy; ms_per_copy is the HIP-event time per copy and
+ * times with a plain 16-byte-per-lane copy kernel (streaming loads and
+ * stores, six wavefronts per SIMD) on the handle's stream, after one
+ * untimed copy; ms_per_copy is the HIP-event time per copy and
  * bytes_per_copy the bytes read plus written by one copy.  Populations are
  * unchanged. */
 int lbdem_psm_copy_bandwidth(lbdem_psm* h, int32_t reps, double* ms_per_copy, double* bytes_per_copy);
